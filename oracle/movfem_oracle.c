@@ -1,0 +1,995 @@
+/*
+ * movfem_oracle.c -- CPU restatement of MoVFEM_3DMT's element-integration,
+ * assembly and sort path (reference: MoVFEM_3DMT/src, GPL-3.0; nothing here is
+ * copied, every routine is re-expressed from the reference's behaviour).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product path
+ * is compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product never calls it.
+ *
+ * It reproduces the reference's effective arithmetic, including:
+ *   - float32 Gauss points / 3-point weights      (integration.f90:297-299,400-402)
+ *   - float32 omega in the element matrix         (integration.f90:84)
+ *   - real-part truncation of f1/f2               (integration.f90:154,211)
+ *   - real-part storage of the GPML stretch h     (integration.f90:15,126)
+ *   - GPML zone flags taken from the previously processed element, because
+ *     get_pml runs after int_elem_params          (MoVFEM_3DMT.f90:203-206)
+ *   - 8-node d/dzeta typo                         (n_fem.f90:193)
+ *   - J^-1 divided by |det J|                     (n_fem.f90:378-386)
+ *   - float32 constants in primary field / source (problem.f90:112,250,352-356)
+ *   - the update_sigma linear-index range         (geometry.f90:144-153)
+ *   - complex64 values and the column-only value permutation of ga_sort_sparse
+ *                                                 (global_assembly.f90:152-181)
+ *
+ * Pinning status of each function is stated in the header of oracle/oracle.py.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex zc;
+
+#define ORC_PI 3.1415926535897932384626433
+#define ORC_EPS 8.854187817e-12
+#define ORC_B0 1.0e-9
+
+/* ------------------------------------------------------------------------- */
+/* Reference-element tables.  Data tables that must agree with the reference:
+ * node coordinates (n_fem.f90:113-133), edge (basis node, direction) pairs
+ * (v_fem.f90:491-504), neighbour copy maps (global_assembly.f90:242-265,
+ * 310-351,396-443), which double as the Dirichlet face lists
+ * (boundary_conds.f90:276-388).  All indices 1-based as in the reference.   */
+static const signed char NR27[27][3] = {
+    {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1}, {1, -1, 1}, {1, 1, 1},
+    {-1, 1, 1},  {-1, -1, 1}, {1, 0, -1}, {0, 1, -1},  {-1, 0, -1}, {0, -1, -1},
+    {1, 0, 1},   {0, 1, 1},  {-1, 0, 1}, {0, -1, 1},   {1, -1, 0}, {1, 1, 0},
+    {-1, 1, 0},  {-1, -1, 0}, {1, 0, 0}, {0, 1, 0},    {-1, 0, 0}, {0, -1, 0},
+    {0, 0, -1},  {0, 0, 1},  {0, 0, 0}};
+
+static const unsigned char ED12[12][2] = {{4, 3}, {4, 2}, {8, 2}, {3, 3}, {4, 1}, {8, 1},
+                                          {3, 1}, {7, 1}, {1, 3}, {1, 2}, {5, 2}, {2, 3}};
+static const unsigned char ED36[36][2] = {
+    {4, 3},  {8, 3},  {4, 2},  {3, 2},  {8, 2},  {7, 2},  {3, 3},  {7, 3},  {4, 1},
+    {1, 1},  {8, 1},  {5, 1},  {3, 1},  {2, 1},  {7, 1},  {6, 1},  {1, 3},  {5, 3},
+    {1, 2},  {2, 2},  {5, 2},  {6, 2},  {2, 3},  {6, 3},  {12, 3}, {20, 2}, {12, 2},
+    {16, 2}, {10, 3}, {17, 2}, {11, 3}, {20, 1}, {11, 1}, {15, 1}, {19, 1}, {9, 3}};
+static const unsigned char ED54[54][2] = {
+    {4, 3},  {8, 3},  {4, 2},  {20, 2}, {8, 2},  {11, 3}, {15, 3}, {3, 2},  {19, 2},
+    {7, 2},  {3, 3},  {7, 3},  {4, 1},  {20, 1}, {8, 1},  {11, 1}, {15, 1}, {3, 1},
+    {19, 1}, {7, 1},  {12, 3}, {16, 3}, {12, 2}, {10, 2}, {16, 2}, {14, 2}, {10, 3},
+    {14, 3}, {1, 1},  {17, 1}, {5, 1},  {9, 1},  {13, 1}, {2, 1},  {18, 1}, {6, 1},
+    {1, 3},  {5, 3},  {1, 2},  {17, 2}, {5, 2},  {9, 3},  {13, 3}, {2, 2},  {18, 2},
+    {6, 2},  {2, 3},  {6, 3},  {25, 3}, {26, 3}, {24, 2}, {22, 2}, {23, 1}, {21, 1}};
+
+/* copy maps: [face z,y,x][k] = {dst (this element), src (neighbour at -face)} */
+static const unsigned char CP12[3][4][2] = {
+    {{2, 3}, {5, 6}, {7, 8}, {10, 11}},
+    {{1, 4}, {5, 7}, {6, 8}, {9, 12}},
+    {{1, 9}, {2, 10}, {3, 11}, {4, 12}}};
+static const unsigned char CP36[3][10][2] = {
+    {{3, 5}, {4, 6}, {9, 11}, {10, 12}, {13, 15}, {14, 16}, {19, 21}, {20, 22}, {27, 28}, {33, 34}},
+    {{1, 7}, {2, 8}, {9, 13}, {10, 14}, {11, 15}, {12, 16}, {17, 23}, {18, 24}, {25, 29}, {32, 35}},
+    {{1, 17}, {2, 18}, {3, 19}, {4, 20}, {5, 21}, {6, 22}, {7, 23}, {8, 24}, {26, 30}, {31, 36}}};
+static const unsigned char CP54[3][12][2] = {
+    {{3, 5}, {8, 10}, {13, 15}, {16, 17}, {18, 20}, {23, 25}, {24, 26}, {29, 31}, {32, 33}, {34, 36},
+     {39, 41}, {44, 46}},
+    {{1, 11}, {2, 12}, {13, 18}, {14, 19}, {15, 20}, {21, 27}, {22, 28}, {29, 34}, {30, 35}, {31, 36},
+     {37, 47}, {38, 48}},
+    {{1, 37}, {2, 38}, {3, 39}, {4, 40}, {5, 41}, {6, 42}, {7, 43}, {8, 44}, {9, 45}, {10, 46},
+     {11, 47}, {12, 48}}};
+
+/* ------------------------------------------------------------------------- */
+typedef struct orc_state {
+  int mn, me, nord, ngd, gpt, nf; /* nodes, edges, lattice order, GP/dir, GP, per-face */
+  int nx, ny, nz;                 /* elements along x,y,z (reference g_nx-1 ...) */
+  int nnx, nny, nnz;
+  int64_t nyz, npt, ne;
+  double *xp, *yp, *zp;
+  double *sig; /* complex (6,npt), interleaved re/im, mutable (update_sigma) */
+  double *mu;  /* real (6,npt) */
+  int dirichlet, inimod, gpml_sch;
+  double a0, b0, nn;
+  int nextd, nzl_top;
+  double omega;
+  int el_xa[2], el_xb[2], el_ya[2], el_yb[2], el_za[2], el_zb[2];
+  double xa[2], xb[2], ya[2], yb[2], za[2], zb[2];
+  int in_pml[3]; /* persists across calls like the reference's module variable */
+  double rw[27][4];
+  int32_t *gne; /* (ne, me) element-major: gne[e*me + im] */
+  int32_t nne;
+  int32_t *nzindx; /* (ne*me*me) */
+  int64_t nnze;
+  const unsigned char (*ed)[2];
+  const unsigned char *cp; /* [3][nf][2] */
+} orc_state;
+
+/* ---------------------------- shape functions ---------------------------- */
+/* nodal shape function N_i (n_fem.f90:138-178), i 1-based */
+static double nf_ln(const orc_state *s, int i, const double *x) {
+  const signed char *r = NR27[i - 1];
+  double v = 1.0;
+  if (s->mn == 8) {
+    for (int c = 0; c < 3; c++) v *= (1 + r[c] * x[c]);
+    return v / 8.0;
+  }
+  if (s->mn == 20) {
+    if (i <= 8) {
+      for (int c = 0; c < 3; c++) v *= (1 + r[c] * x[c]);
+      return v * (r[0] * x[0] + r[1] * x[1] + r[2] * x[2] - 2) / 8.0;
+    }
+    for (int c = 0; c < 3; c++) v *= (r[c] == 0) ? (1 - x[c] * x[c]) : (1 + r[c] * x[c]);
+    return v / 4.0;
+  }
+  /* 27: tensor product of 1-D quadratic Lagrange factors */
+  int nz0 = 0;
+  for (int c = 0; c < 3; c++) {
+    if (r[c] == 0) {
+      v *= (1 - x[c] * x[c]);
+      nz0++;
+    } else {
+      v *= (1 + r[c] * x[c]) * (r[c] * x[c]);
+    }
+  }
+  static const double den[4] = {8.0, 4.0, 2.0, 1.0};
+  return v / den[nz0];
+}
+
+/* d N_i / d xi_d (n_fem.f90:181-317); reproduces the 8-node d=3 form (:193) */
+static double nf_dln(const orc_state *s, int d, int i, const double *x) {
+  const signed char *r = NR27[i - 1];
+  int a = (d + 1) % 3, b = (d + 2) % 3; /* the two other coordinates */
+  if (s->mn == 8) {
+    if (d == 2) /* reference uses (1+r1) instead of (1+r1*xi) */
+      return ((1 + r[0]) * (1 + r[1] * x[1]) * r[2]) / 8.0;
+    return r[d] * (1 + r[a] * x[a]) * (1 + r[b] * x[b]) / 8.0;
+  }
+  if (s->mn == 20) {
+    if (i <= 8)
+      return (1 + r[a] * x[a]) * (1 + r[b] * x[b]) *
+             (r[d] * (2 * r[d] * x[d] + r[a] * x[a] + r[b] * x[b] - 1)) / 8.0;
+    if (r[d] == 0) return -x[d] * (1 + r[a] * x[a]) * (1 + r[b] * x[b]) / 2.0;
+    /* derivative along a non-zero coordinate: the zero coordinate gives (1-x^2) */
+    int z = (r[a] == 0) ? a : b, o = (r[a] == 0) ? b : a;
+    return r[d] * (1 - x[z] * x[z]) * (1 + r[o] * x[o]) / 4.0;
+  }
+  /* 27-node: product of 1-D factors, derivative on coordinate d */
+  double v = 1.0;
+  int nz0 = 0;
+  for (int c = 0; c < 3; c++) {
+    if (r[c] == 0) {
+      nz0++;
+      v *= (c == d) ? (-2.0 * x[c]) : (1 - x[c] * x[c]);
+    } else {
+      v *= (c == d) ? (r[c] * (1 + 2 * r[c] * x[c])) : ((1 + r[c] * x[c]) * (r[c] * x[c]));
+    }
+  }
+  static const double den[4] = {8.0, 4.0, 2.0, 1.0};
+  return v / den[nz0];
+}
+
+/* mixed-order edge polynomial phi(node i, direction dir) (v_fem.f90:78-179) */
+static double mix_ln(const orc_state *s, int i, int dir, const double *x) {
+  const signed char *r = NR27[i - 1];
+  int a = (dir + 1) % 3, b = (dir + 2) % 3; /* dir is 0-based; a,b transverse */
+  if (s->mn == 8) return (1 + r[a] * x[a]) * (1 + r[b] * x[b]) / 4.0;
+  if (s->mn == 20) {
+    if (i <= 8)
+      return (1 + r[a] * x[a]) * (1 + r[b] * x[b]) *
+             (r[0] * x[0] + r[1] * x[1] + r[2] * x[2] - 1) / 8.0;
+    int z = (r[a] == 0) ? a : b, o = (r[a] == 0) ? b : a;
+    return (1 - x[z] * x[z]) * (1 + r[o] * x[o]) / 2.0;
+  }
+  /* 27: linear along dir, quadratic Lagrange across */
+  double v = (1 + r[dir] * x[dir]);
+  int nz0 = 0;
+  for (int k = 0; k < 2; k++) {
+    int c = k ? b : a;
+    if (r[c] == 0) {
+      v *= (1 - x[c] * x[c]);
+      nz0++;
+    } else {
+      v *= (1 + r[c] * x[c]) * (r[c] * x[c]);
+    }
+  }
+  static const double den[3] = {8.0, 4.0, 2.0};
+  return v / den[nz0];
+}
+
+/* d phi / d xi_d (v_fem.f90:184-466) */
+static double mix_dln(const orc_state *s, int dir, int d, int i, const double *x) {
+  const signed char *r = NR27[i - 1];
+  int a = (dir + 1) % 3, b = (dir + 2) % 3;
+  if (s->mn == 8) {
+    if (d == dir) return 0.0;
+    int o = (d == a) ? b : a;
+    return r[d] * (1 + r[o] * x[o]) / 4.0;
+  }
+  if (s->mn == 20) {
+    if (i <= 8) {
+      if (d == dir) return (1 + r[a] * x[a]) * (1 + r[b] * x[b]) * r[d] / 8.0;
+      int o = (d == a) ? b : a;
+      return r[d] * (1 + r[o] * x[o]) * (r[0] * x[0] + r[1] * x[1] + r[2] * x[2] + r[d] * x[d]) / 8.0;
+    }
+    if (d == dir) return 0.0;
+    int z = (r[a] == 0) ? a : b, o = (r[a] == 0) ? b : a;
+    if (d == z) return -x[z] * (1 + r[o] * x[o]);
+    return (1 - x[z] * x[z]) * r[o] / 2.0;
+  }
+  double v = (d == dir) ? (double)r[dir] : (1 + r[dir] * x[dir]);
+  int nz0 = 0;
+  for (int k = 0; k < 2; k++) {
+    int c = k ? b : a;
+    if (r[c] == 0) {
+      nz0++;
+      v *= (c == d) ? (-2.0 * x[c]) : (1 - x[c] * x[c]);
+    } else {
+      v *= (c == d) ? (r[c] * (1 + 2 * r[c] * x[c])) : ((1 + r[c] * x[c]) * (r[c] * x[c]));
+    }
+  }
+  static const double den[3] = {8.0, 4.0, 2.0};
+  return v / den[nz0];
+}
+
+/* ------------------------------ Jacobian --------------------------------- */
+/* J(m,n) = sum_l dN_l/dxi_m x_l,n ; inverse by cofactors / |det| (n_fem.f90:355-395) */
+static double orc_det(const double J[3][3]) {
+  return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+         J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+static int orc_jac(const orc_state *s, const double re[][3], const double *x, double J[3][3],
+                   double Ji[3][3], double *det) {
+  for (int m = 0; m < 3; m++)
+    for (int n = 0; n < 3; n++) {
+      double acc = 0.0;
+      for (int l = 0; l < s->mn; l++) acc += nf_dln(s, m, l + 1, x) * re[l][n];
+      J[m][n] = acc;
+    }
+  double dj = orc_det(J);
+  *det = dj;
+  if (dj == 0.0) return -1;
+  double ad = fabs(dj);
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / ad;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / ad;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / ad;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / ad;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / ad;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / ad;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / ad;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / ad;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / ad;
+  return 0;
+}
+
+/* ------------------------------ quadrature ------------------------------- */
+static void orc_quadrature(orc_state *s) {
+  double p[3], w[3];
+  int n = s->ngd;
+  if (n == 2) {
+    p[0] = (double)-0.577350269189625764509148780502f;
+    p[1] = (double)0.577350269189625764509148780502f;
+    w[0] = 1.0;
+    w[1] = 1.0;
+  } else {
+    p[0] = (double)-0.774596669241483377035853079956f;
+    p[1] = 0.0;
+    p[2] = (double)0.774596669241483377035853079956f;
+    w[0] = (double)0.555555555555555555555555555556f;
+    w[1] = (double)0.888888888888888888888888888889f;
+    w[2] = (double)0.555555555555555555555555555556f;
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      for (int k = 0; k < n; k++) {
+        int id = (i * n + j) * n + k;
+        s->rw[id][0] = p[i];
+        s->rw[id][1] = p[j];
+        s->rw[id][2] = p[k];
+        s->rw[id][3] = w[i] * w[j] * w[k];
+      }
+}
+
+/* ------------------------------ state ------------------------------------ */
+static inline double sig_re(const orc_state *s, int64_t node, int c) { return s->sig[2 * (node * 6 + c)]; }
+static inline zc sig_z(const orc_state *s, int64_t node, int c) {
+  return CMPLX(s->sig[2 * (node * 6 + c)], s->sig[2 * (node * 6 + c) + 1]);
+}
+
+orc_state *orc_create(int nx, int ny, int nz, int mn, const double *xp, const double *yp, const double *zp,
+                      const double *sigma, const double *mu, int dirichlet, int inimod, int gpml_sch,
+                      double a0, double b0, double nn, int nextd, int nzl_top) {
+  if (mn != 8 && mn != 20 && mn != 27) return NULL;
+  orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
+  s->mn = mn;
+  s->me = mn == 8 ? 12 : (mn == 20 ? 36 : 54);
+  s->nf = mn == 8 ? 4 : (mn == 20 ? 10 : 12);
+  s->nord = mn == 8 ? 2 : 3;
+  s->ngd = mn == 8 ? 2 : 3;
+  s->gpt = s->ngd * s->ngd * s->ngd;
+  s->ed = mn == 8 ? ED12 : (mn == 20 ? ED36 : ED54);
+  s->cp = mn == 8 ? &CP12[0][0][0] : (mn == 20 ? &CP36[0][0][0] : &CP54[0][0][0]);
+  s->nx = nx;
+  s->ny = ny;
+  s->nz = nz;
+  s->nnx = nx * (s->nord - 1) + 1;
+  s->nny = ny * (s->nord - 1) + 1;
+  s->nnz = nz * (s->nord - 1) + 1;
+  s->nyz = (int64_t)s->nny * s->nnz;
+  s->npt = (int64_t)s->nnx * s->nyz;
+  s->ne = (int64_t)nx * ny * nz;
+  s->xp = (double *)malloc(sizeof(double) * s->nnx);
+  s->yp = (double *)malloc(sizeof(double) * s->nny);
+  s->zp = (double *)malloc(sizeof(double) * s->npt);
+  s->sig = (double *)malloc(sizeof(double) * 12 * s->npt);
+  s->mu = (double *)malloc(sizeof(double) * 6 * s->npt);
+  memcpy(s->xp, xp, sizeof(double) * s->nnx);
+  memcpy(s->yp, yp, sizeof(double) * s->nny);
+  memcpy(s->zp, zp, sizeof(double) * s->npt);
+  memcpy(s->sig, sigma, sizeof(double) * 12 * s->npt);
+  memcpy(s->mu, mu, sizeof(double) * 6 * s->npt);
+  s->dirichlet = dirichlet;
+  s->inimod = inimod;
+  s->gpml_sch = gpml_sch;
+  s->a0 = a0;
+  s->b0 = b0;
+  s->nn = nn;
+  s->nextd = nextd;
+  s->nzl_top = nzl_top;
+  orc_quadrature(s);
+  /* init_gpml (boundary_conds.f90:51-70); g_nx = nx+1 etc. */
+  int gnx = nx + 1, gny = ny + 1, gnz = nz + 1, o = s->nord - 1;
+  s->el_xa[0] = nextd; s->el_xa[1] = gnx - nextd; s->el_xb[0] = 1; s->el_xb[1] = gnx - 1;
+  s->el_ya[0] = nextd; s->el_ya[1] = gny - nextd; s->el_yb[0] = 1; s->el_yb[1] = gny - 1;
+  s->el_za[0] = nextd; s->el_za[1] = gnz - nextd; s->el_zb[0] = 1; s->el_zb[1] = gnz - 1;
+  if (nextd >= 1 && nextd * o + 1 <= s->nnx && nextd * o + 1 <= s->nny && nextd * o + 1 <= s->nnz) {
+    s->xa[0] = s->xp[nextd * o]; s->xa[1] = s->xp[s->nnx - nextd * o - 1];
+    s->ya[0] = s->yp[nextd * o]; s->ya[1] = s->yp[s->nny - nextd * o - 1];
+    int64_t id1 = nextd * o; /* 0-based node id nextd*o+1 */
+    int64_t id2 = (int64_t)(s->nnx - 1) * s->nyz + (int64_t)(s->nny - 1) * s->nnz + s->nnz - nzl_top * o - 1;
+    s->za[0] = s->zp[id1];
+    s->za[1] = (id2 >= 0 && id2 < s->npt) ? s->zp[id2] : 0.0;
+  }
+  s->xb[0] = s->xp[0]; s->xb[1] = s->xp[s->nnx - 1];
+  s->yb[0] = s->yp[0]; s->yb[1] = s->yp[s->nny - 1];
+  s->zb[0] = s->zp[0]; s->zb[1] = s->zp[s->npt - 1];
+  s->in_pml[0] = s->in_pml[1] = s->in_pml[2] = 0;
+  return s;
+}
+
+void orc_free(orc_state *s) {
+  if (!s) return;
+  free(s->xp); free(s->yp); free(s->zp); free(s->sig); free(s->mu);
+  free(s->gne); free(s->nzindx);
+  free(s);
+}
+
+/* update_omega (geometry.f90:137-140) */
+void orc_set_omega(orc_state *s, double omega) { s->omega = omega; }
+
+/* update_sigma (geometry.f90:144-153): the loop runs i<=npt, j<=6 over a
+ * (6,npt) array, so it touches linear entries 0..npt+29 only.  Each touched
+ * entry with a non-zero imaginary part gets imag = float32(eps*omega). */
+void orc_update_sigma(orc_state *s) {
+  int64_t lim = s->npt + 30;
+  if (lim > 6 * s->npt) lim = 6 * s->npt;
+  double im = (double)(float)(ORC_EPS * s->omega);
+  for (int64_t L = 0; L < lim; L++)
+    if (s->sig[2 * L + 1] != 0.0) s->sig[2 * L + 1] = im;
+}
+
+/* read back the (mutated) sigma array */
+void orc_get_sigma(const orc_state *s, double *out) { memcpy(out, s->sig, sizeof(double) * 12 * s->npt); }
+
+/* ------------------------------ numbering -------------------------------- */
+static int edge_face(const orc_state *s, int ie, int je, int ke, int im) {
+  /* edge_bdary (boundary_conds.f90:262-390): faces tested in order -x,-y,-z,+x,+y,+z */
+  const unsigned char *cp = s->cp;
+  int nf = s->nf;
+  int conds[6] = {ie == 1, je == 1, ke == 1, ie == s->nx, je == s->ny, ke == s->nz};
+  for (int f = 0; f < 6; f++) {
+    if (!conds[f]) continue;
+    int axis = (f % 3 == 0) ? 2 : (f % 3 == 1 ? 1 : 0); /* copy table order z,y,x */
+    int col = (f < 3) ? 0 : 1;                            /* -face: dst list, +face: src list */
+    for (int k = 0; k < nf; k++)
+      if (cp[(axis * nf + k) * 2 + col] == im) return f + 1;
+  }
+  return 0;
+}
+
+/* c_gne12/36/54 (global_assembly.f90:231-475) */
+int32_t orc_build_gne(orc_state *s) {
+  int me = s->me, nf = s->nf;
+  free(s->gne);
+  s->gne = (int32_t *)calloc((size_t)s->ne * me, sizeof(int32_t));
+  int32_t indx = 0;
+  int copied[54];
+  for (int ie = 1; ie <= s->nx; ie++)
+    for (int je = 1; je <= s->ny; je++)
+      for (int ke = 1; ke <= s->nz; ke++) {
+        int64_t ide = ((int64_t)(ie - 1) * s->ny + (je - 1)) * s->nz + (ke - 1);
+        int32_t *g = s->gne + ide * me;
+        memset(copied, 0, sizeof(copied));
+        int cond[3] = {ke > 1, je > 1, ie > 1};
+        int64_t nb[3] = {ide - 1, ide - s->nz, ide - (int64_t)s->ny * s->nz};
+        for (int ax = 0; ax < 3; ax++) {
+          if (!cond[ax]) continue;
+          const int32_t *gn = s->gne + nb[ax] * me;
+          for (int k = 0; k < nf; k++) {
+            int dst = s->cp[(ax * nf + k) * 2 + 0], src = s->cp[(ax * nf + k) * 2 + 1];
+            g[dst - 1] = gn[src - 1];
+            copied[dst - 1] = 1;
+          }
+        }
+        int bel = (ie == 1 || ie == s->nx || je == 1 || je == s->ny || ke == 1 || ke == s->nz);
+        for (int im = 1; im <= me; im++) {
+          if (s->dirichlet && bel) {
+            int f = edge_face(s, ie, je, ke, im);
+            if (f) {
+              g[im - 1] = -f;
+              continue;
+            }
+          }
+          if (!copied[im - 1]) g[im - 1] = ++indx;
+        }
+      }
+  int32_t mx = 0;
+  for (int64_t i = 0; i < s->ne * me; i++)
+    if (s->gne[i] > mx) mx = s->gne[i];
+  s->nne = mx;
+  return mx;
+}
+
+/* ga_nzindx + shr_nzindx12/36/54 (global_assembly.f90:197-229,477-1906).
+ * The reference's case tables are instances of one rule, restated here:
+ *   a DOF on this element's +z/+y/+x face(s) is shared forward.  Processing its
+ *   row assigns slots for the own row, then (in order z,y,x) for the neighbour
+ *   across each such face -- first copying the slots of DOFs shared across that
+ *   face, then numbering the rest -- and finally, for a DOF on two + faces,
+ *   the diagonal neighbour, whose face-shared slots are copied first from the
+ *   lower-order face neighbour through the higher-order face map, then from the
+ *   higher-order face neighbour through the lower-order face map.  DOFs on no +
+ *   face ("case default") number their own row's unassigned slots. */
+static int face_pos(const orc_state *s, int ax, int col, int im) {
+  for (int k = 0; k < s->nf; k++)
+    if (s->cp[(ax * s->nf + k) * 2 + col] == im) return k;
+  return -1;
+}
+static int32_t *slot(orc_state *s, int64_t ide, int im, int jm) {
+  return s->nzindx + (ide * s->me + (im - 1)) * s->me + (jm - 1);
+}
+static void own_row(orc_state *s, int64_t ide, int im, int64_t *indx) {
+  const int32_t *g = s->gne + ide * s->me;
+  for (int jm = 1; jm <= s->me; jm++) {
+    if (g[jm - 1] < 0) continue;
+    int32_t *p = slot(s, ide, im, jm);
+    if (*p == -1) *p = (int32_t)(++(*indx));
+  }
+}
+/* copy slots (src elem, row im_s) -> (dst elem, row im_d) through face map ax */
+static void copy_face(orc_state *s, int64_t src, int im_s, int64_t dst, int im_d, int ax) {
+  for (int k = 0; k < s->nf; k++) {
+    int js = s->cp[(ax * s->nf + k) * 2 + 1], jd = s->cp[(ax * s->nf + k) * 2 + 0];
+    int32_t *pd = slot(s, dst, im_d, jd);
+    if (*pd != -1) continue;
+    *pd = *slot(s, src, im_s, js);
+  }
+}
+
+int64_t orc_build_nzindx(orc_state *s) {
+  if (!s->gne) orc_build_gne(s);
+  int me = s->me;
+  free(s->nzindx);
+  size_t n = (size_t)s->ne * me * me;
+  s->nzindx = (int32_t *)malloc(n * sizeof(int32_t));
+  for (size_t i = 0; i < n; i++) s->nzindx[i] = -1;
+  int64_t indx = 0;
+  int64_t stride[3] = {1, s->nz, (int64_t)s->ny * s->nz};
+  for (int ie = 1; ie <= s->nx; ie++)
+    for (int je = 1; je <= s->ny; je++)
+      for (int ke = 1; ke <= s->nz; ke++) {
+        int64_t ide = ((int64_t)(ie - 1) * s->ny + (je - 1)) * s->nz + (ke - 1);
+        int lt[3] = {ke < s->nz, je < s->ny, ie < s->nx};
+        for (int im = 1; im <= me; im++) {
+          if (s->gne[ide * me + im - 1] < 0) continue;
+          int pos[3], on = 0;
+          for (int ax = 0; ax < 3; ax++) {
+            pos[ax] = face_pos(s, ax, 1, im); /* on + face of axis ax ? */
+            if (pos[ax] >= 0) on++;
+          }
+          own_row(s, ide, im, &indx);
+          if (on == 0) continue; /* case default: own row only */
+          int im1[3];
+          for (int ax = 0; ax < 3; ax++) {
+            if (pos[ax] < 0) continue;
+            im1[ax] = s->cp[(ax * s->nf + pos[ax]) * 2 + 0];
+            if (!lt[ax]) continue;
+            int64_t nb = ide + stride[ax];
+            copy_face(s, ide, im, nb, im1[ax], ax);
+            own_row(s, nb, im1[ax], &indx);
+          }
+          if (on == 2) {
+            int lo = -1, hi = -1;
+            for (int ax = 0; ax < 3; ax++)
+              if (pos[ax] >= 0) {
+                if (lo < 0) lo = ax;
+                else hi = ax;
+              }
+            if (lt[lo] && lt[hi]) {
+              int64_t nlo = ide + stride[lo], nhi = ide + stride[hi], nd = ide + stride[lo] + stride[hi];
+              /* row index of the DOF in the diagonal element */
+              int pd = face_pos(s, hi, 1, im1[lo]);
+              int imd = s->cp[(hi * s->nf + pd) * 2 + 0];
+              copy_face(s, nlo, im1[lo], nd, imd, hi);
+              copy_face(s, nhi, im1[hi], nd, imd, lo);
+              own_row(s, nd, imd, &indx);
+            }
+          }
+        }
+      }
+  int64_t mx = 0;
+  for (size_t i = 0; i < n; i++)
+    if (s->nzindx[i] > mx) mx = s->nzindx[i];
+  s->nnze = mx;
+  return mx;
+}
+
+int32_t orc_nne(const orc_state *s) { return s->nne; }
+int64_t orc_nnze(const orc_state *s) { return s->nnze; }
+void orc_get_gne(const orc_state *s, int32_t *out) { memcpy(out, s->gne, sizeof(int32_t) * s->ne * s->me); }
+void orc_get_nzindx(const orc_state *s, int32_t *out) {
+  memcpy(out, s->nzindx, sizeof(int32_t) * s->ne * s->me * s->me);
+}
+
+/* ------------------------------ element ---------------------------------- */
+typedef struct {
+  double re[27][3];
+  int64_t idx[27];
+  double inmu[27][6], dmu[27][6];
+  zc dsig[27][6];
+  zc ep[2][27][3], hp[2][27][3];
+  double wgt[27];
+  double mf1[27][6], mf2[27][6]; /* real parts; imaginary parts never reach f1/f2 */
+  double h[27][3];
+  zc src[2][27][3];
+  double cv1[27][54][3], cv2[27][54][3], ve[27][54][3];
+} orc_ws;
+
+static void inv_tensor(const double *a, double *b) {
+  /* problem.f90:279-306 */
+  double d = a[0] * (a[3] * a[5] - a[4] * a[4]) + a[1] * (a[2] * a[4] - a[1] * a[5]) +
+             a[2] * (a[1] * a[4] - a[3] * a[2]);
+  b[0] = (a[3] * a[5] - a[4] * a[4]) / d;
+  b[1] = (a[2] * a[4] - a[1] * a[5]) / d;
+  b[2] = (a[1] * a[4] - a[2] * a[3]) / d;
+  b[3] = (a[0] * a[5] - a[2] * a[2]) / d;
+  b[4] = (a[2] * a[1] - a[0] * a[4]) / d;
+  b[5] = (a[3] * a[0] - a[1] * a[1]) / d;
+}
+
+/* gpml_h real part (boundary_conds.f90:84-186); Im parts are discarded by the
+ * real-valued gpml storage, scheme 1 therefore has Re h = 1 exactly. */
+static double gpml_re(const orc_state *s, int axis, double r) {
+  int f = s->in_pml[axis];
+  if (f == 0) return 1.0;
+  const double *a = axis == 0 ? s->xa : (axis == 1 ? s->ya : s->za);
+  const double *b = axis == 0 ? s->xb : (axis == 1 ? s->yb : s->zb);
+  int k = (f == -1) ? 0 : 1;
+  double rp = sqrt((b[k] - a[k]) * (b[k] - a[k]));
+  double rr = sqrt((r - a[k]) * (r - a[k]));
+  if (s->gpml_sch == 0) return 1.0 + s->a0 * pow(rr / rp, s->nn);
+  return 1.0;
+}
+
+/* get_pml (boundary_conds.f90:72-82) */
+static void orc_get_pml(orc_state *s, int i, int j, int k) {
+  s->in_pml[0] = s->in_pml[1] = s->in_pml[2] = 0;
+  if (i >= s->el_xa[0] && i <= s->el_xb[0]) s->in_pml[0] = -1;
+  if (i >= s->el_xa[1] && i <= s->el_xb[1]) s->in_pml[0] = 1;
+  if (j >= s->el_ya[0] && j <= s->el_yb[0]) s->in_pml[1] = -1;
+  if (j >= s->el_ya[1] && j <= s->el_yb[1]) s->in_pml[1] = 1;
+  if (k >= s->el_za[0] && k <= s->el_zb[0]) s->in_pml[2] = -1;
+  if (k >= s->el_za[1] && k <= s->el_zb[1]) s->in_pml[2] = 1;
+}
+
+/* nf_get_r + p_elem_fields + int_elem_params for element (ie,je,ke), 1-based */
+static int orc_elem_params(orc_state *s, orc_ws *w, int ie, int je, int ke) {
+  int o = s->nord - 1, mn = s->mn, G = s->gpt, me = s->me;
+  int64_t eno = (int64_t)(ie - 1) * s->nyz * o + (int64_t)(je - 1) * s->nnz * o + (int64_t)(ke - 1) * o;
+  for (int n = 0; n < mn; n++) {
+    const signed char *r = NR27[n];
+    int i1 = (r[0] + 1) * o / 2, j1 = (r[1] + 1) * o / 2, k1 = (r[2] + 1) * o / 2;
+    int64_t id = eno + (int64_t)i1 * s->nyz + (int64_t)j1 * s->nnz + k1;
+    w->idx[n] = id;
+    w->re[n][0] = s->xp[(ie - 1) * o + i1];
+    w->re[n][1] = s->yp[(je - 1) * o + j1];
+    w->re[n][2] = s->zp[id];
+  }
+  /* p_elem_fields (problem.f90:70-87) */
+  double w32 = (double)(float)(s->omega);
+  zc psig = CMPLX(0.0, (double)(float)(s->omega * ORC_EPS)); /* pset_pmodel */
+  double pmu = 4 * ORC_PI * 1.e-7;
+  for (int n = 0; n < mn; n++) {
+    int64_t id = w->idx[n];
+    zc sg[6];
+    for (int c = 0; c < 6; c++) sg[c] = sig_z(s, id, c);
+    zc cd = sg[0] * (sg[3] * sg[5] - sg[4] * sg[4]) + sg[1] * (sg[2] * sg[4] - sg[1] * sg[5]) +
+            sg[2] * (sg[1] * sg[4] - sg[3] * sg[2]);
+    const double *m = s->mu + 6 * id;
+    double md = m[0] * (m[3] * m[5] - m[4] * m[4]) + m[1] * (m[2] * m[4] - m[1] * m[5]) +
+                m[2] * (m[1] * m[4] - m[3] * m[2]);
+    if (cd == 0.0 || md == 0.0) return -2; /* pinv_geomodel stops (problem.f90:260-271) */
+    inv_tensor(m, w->inmu[n]);
+    for (int c = 0; c < 6; c++) {
+      w->dsig[n][c] = sg[c];
+      w->dmu[n][c] = m[c];
+    }
+    w->dsig[n][0] -= psig; w->dsig[n][3] -= psig; w->dsig[n][5] -= psig;
+    w->dmu[n][0] -= pmu; w->dmu[n][3] -= pmu; w->dmu[n][5] -= pmu;
+    /* p_pfields (problem.f90:340-358) */
+    double zz = (double)(float)(s->omega * ORC_B0 * s->zp[id]);
+    double hh = (double)(float)ORC_B0 / (4 * ORC_PI * 1.e-7);
+    for (int d = 0; d < 2; d++)
+      for (int c = 0; c < 3; c++) w->ep[d][n][c] = w->hp[d][n][c] = 0.0;
+    w->ep[0][n][0] = CMPLX(0.0, -zz);
+    w->hp[0][n][1] = CMPLX(hh, 0.0);
+    w->ep[1][n][1] = CMPLX(0.0, zz);
+    w->hp[1][n][0] = CMPLX(hh, 0.0);
+  }
+  /* int_elem_params (integration.f90:60-74) */
+  for (int g = 0; g < G; g++) {
+    const double *x = s->rw[g];
+    double J[3][3], Ji[3][3], det;
+    if (orc_jac(s, w->re, x, J, Ji, &det)) return -1;
+    w->wgt[g] = det * x[3];
+    /* p_intmodels, pe_sch = 1 */
+    for (int c = 0; c < 6; c++) {
+      double a1 = 0.0, a2 = 0.0;
+      for (int n = 0; n < mn; n++) {
+        double N = nf_ln(s, n + 1, x);
+        a1 += N * w->inmu[n][c];
+        a2 += N * sig_re(s, w->idx[n], c);
+      }
+      w->mf1[g][c] = a1;
+      w->mf2[g][c] = a2;
+    }
+    /* GPML stretch at the mapped point, using the zone flags currently held */
+    if (!s->dirichlet) {
+      double gr[3] = {0, 0, 0};
+      for (int n = 0; n < mn; n++) {
+        double N = nf_ln(s, n + 1, x);
+        gr[0] += N * w->re[n][0];
+        gr[1] += N * w->re[n][1];
+        gr[2] += N * w->re[n][2];
+      }
+      for (int c = 0; c < 3; c++) w->h[g][c] = gpml_re(s, c, gr[c]);
+    } else {
+      w->h[g][0] = w->h[g][1] = w->h[g][2] = 1.0;
+    }
+    /* p_source (problem.f90:91-128) */
+    for (int d = 0; d < 2; d++) {
+      zc pc[3] = {0, 0, 0}, dm[3] = {0, 0, 0};
+      for (int n = 0; n < mn; n++) {
+        /* p_pcurl: grad N x (mu^-1 dmu) Hp */
+        double dn[3];
+        for (int a = 0; a < 3; a++) {
+          double acc = 0.0;
+          for (int b = 0; b < 3; b++) acc += Ji[a][b] * nf_dln(s, b, n + 1, x);
+          dn[a] = acc;
+        }
+        const double *im = w->inmu[n], *dmu = w->dmu[n];
+        double M[3][3] = {
+            {im[0] * dmu[0] + im[1] * dmu[1] + im[2] * dmu[2], im[0] * dmu[1] + im[1] * dmu[3] + im[2] * dmu[4],
+             im[0] * dmu[2] + im[1] * dmu[4] + im[2] * dmu[5]},
+            {im[1] * dmu[0] + im[3] * dmu[1] + im[4] * dmu[2], im[1] * dmu[1] + im[3] * dmu[3] + im[4] * dmu[4],
+             im[1] * dmu[2] + im[3] * dmu[4] + im[4] * dmu[5]},
+            {im[2] * dmu[0] + im[4] * dmu[1] + im[5] * dmu[2], im[2] * dmu[1] + im[4] * dmu[3] + im[5] * dmu[4],
+             im[2] * dmu[2] + im[4] * dmu[4] + im[5] * dmu[5]}};
+        zc v[3];
+        for (int a = 0; a < 3; a++) v[a] = M[a][0] * w->hp[d][n][0] + M[a][1] * w->hp[d][n][1] + M[a][2] * w->hp[d][n][2];
+        pc[0] += v[2] * dn[1] - v[1] * dn[2];
+        pc[1] += v[0] * dn[2] - v[2] * dn[0];
+        pc[2] += v[1] * dn[0] - v[0] * dn[1];
+        /* p_dmpf: N * (dsigma . Ep) */
+        const zc *ds = w->dsig[n];
+        const zc *e = w->ep[d][n];
+        zc u0 = ds[0] * e[0] + ds[1] * e[1] + ds[2] * e[2];
+        zc u1 = ds[1] * e[0] + ds[3] * e[1] + ds[4] * e[2];
+        zc u2 = ds[2] * e[0] + ds[4] * e[1] + ds[5] * e[2];
+        double N = nf_ln(s, n + 1, x);
+        dm[0] += N * u0;
+        dm[1] += N * u1;
+        dm[2] += N * u2;
+      }
+      zc mw = CMPLX(0.0, -w32);
+      for (int c = 0; c < 3; c++) w->src[d][g][c] = (dm[c] + pc[c]) * mw;
+    }
+    /* int_edges (integration.f90:139-152) */
+    for (int j = 0; j < me; j++) {
+      int node = s->ed[j][0], dir = s->ed[j][1] - 1;
+      double vij[3] = {Ji[0][dir], Ji[1][dir], Ji[2][dir]};
+      double dni[3];
+      for (int a = 0; a < 3; a++) {
+        double acc = 0.0;
+        for (int b = 0; b < 3; b++) acc += Ji[a][b] * mix_dln(s, dir, b, node, x);
+        dni[a] = acc;
+      }
+      w->cv1[g][j][0] = dni[1] * vij[2];
+      w->cv2[g][j][0] = dni[2] * vij[1];
+      w->cv1[g][j][1] = dni[2] * vij[0];
+      w->cv2[g][j][1] = dni[0] * vij[2];
+      w->cv1[g][j][2] = dni[0] * vij[1];
+      w->cv2[g][j][2] = dni[1] * vij[0];
+      double phi = mix_ln(s, node, dir, x);
+      for (int c = 0; c < 3; c++) w->ve[g][j][c] = phi * vij[c];
+    }
+  }
+  (void)w32;
+  return 0;
+}
+
+/* tensor index of (a,b), 6-vector order [11,12,13,22,23,33] */
+static const int T6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+
+/* f1 (integration.f90:154-209): terms in the reference's order a, b, then
+ * (k,l) = (1,1),(2,1),(1,2),(2,2); GPML weights are the products of h. */
+static double orc_f1(const orc_state *s, const orc_ws *w, int g, int im, int jm) {
+  const double *m = w->mf1[g];
+  double c1[3][2], c2[3][2];
+  for (int a = 0; a < 3; a++) {
+    c1[a][0] = w->cv1[g][im][a]; c1[a][1] = w->cv2[g][im][a];
+    c2[a][0] = w->cv1[g][jm][a]; c2[a][1] = w->cv2[g][jm][a];
+  }
+  static const int KL[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+  double acc = 0.0;
+  if (s->dirichlet) {
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int t = 0; t < 4; t++) {
+          int k = KL[t][0], l = KL[t][1];
+          double v = (m[T6[a][b]] * c1[a][k]) * c2[b][l];
+          acc = (k == l) ? acc + v : acc - v;
+        }
+    return acc;
+  }
+  const double *h = w->h[g];
+  /* weight W(a,k;b,l) = g(a,k) g(b,l) written as the products of h the
+   * reference uses; g^2 table: (1,1)=h1h3/h2 (1,2)=h1h2/h3 (2,1)=h1h2/h3
+   * (2,2)=h2h3/h1 (3,1)=h2h3/h1 (3,2)=h1h3/h2 */
+  double q[3] = {h[0] * h[2] / h[1], h[0] * h[1] / h[2], h[1] * h[2] / h[0]};
+  static const int GI[3][2] = {{0, 1}, {1, 2}, {2, 0}}; /* index into q for g^2(a,k) */
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      for (int t = 0; t < 4; t++) {
+        int k = KL[t][0], l = KL[t][1];
+        int ia = GI[a][k], ib = GI[b][l];
+        double W;
+        if (ia == ib) W = q[ia];
+        else {
+          /* sqrt(q_i q_j) is the h component shared by both products */
+          /* q0*q1 = h1^2, q0*q2 = h3^2, q1*q2 = h2^2 */
+          int other = 3 - ia - ib;
+          W = (other == 2) ? h[0] : (other == 1 ? h[2] : h[1]);
+        }
+        double v = ((W * m[T6[a][b]]) * c1[a][k]) * c2[b][l];
+        acc = (k == l) ? acc + v : acc - v;
+      }
+  return acc;
+}
+
+/* f2 (integration.f90:211-238) */
+static double orc_f2(const orc_state *s, const orc_ws *w, int g, int im, int jm) {
+  const double *m = w->mf2[g];
+  const double *u = w->ve[g][im], *v = w->ve[g][jm];
+  if (s->dirichlet)
+    return (v[0] * m[0] * u[0] + v[1] * m[1] * u[0] + v[2] * m[2] * u[0]) +
+           (v[0] * m[1] * u[1] + v[1] * m[3] * u[1] + v[2] * m[4] * u[1]) +
+           (v[0] * m[2] * u[2] + v[1] * m[4] * u[2] + v[2] * m[5] * u[2]);
+  double h3 = w->h[g][0] * w->h[g][1] * w->h[g][2];
+  return h3 * v[0] * m[0] * u[0] + h3 * v[1] * m[1] * u[0] + h3 * v[2] * m[2] * u[0] +
+         h3 * v[0] * m[1] * u[1] + h3 * v[1] * m[3] * u[1] + h3 * v[2] * m[4] * u[1] +
+         h3 * v[0] * m[2] * u[2] + h3 * v[1] * m[4] * u[2] + h3 * v[2] * m[5] * u[2];
+}
+
+/* alocal (integration.f90:76-86) */
+static zc orc_alocal(const orc_state *s, const orc_ws *w, int im, int jm) {
+  double w32 = (double)(float)s->omega;
+  double ar = 0.0, ai = 0.0;
+  for (int g = 0; g < s->gpt; g++) {
+    ar += w->wgt[g] * orc_f1(s, w, g, im, jm);
+    ai += w->wgt[g] * (w32 * orc_f2(s, w, g, im, jm));
+  }
+  return CMPLX(ar, ai);
+}
+
+/* blocal (integration.f90:88-106) with f3 (:240-265) */
+static void orc_blocal(const orc_state *s, const orc_ws *w, int im, zc out[2]) {
+  for (int d = 0; d < 2; d++) {
+    zc acc = 0.0;
+    for (int g = 0; g < s->gpt; g++) {
+      const double *v = w->ve[g][im];
+      const zc *sr = w->src[d][g];
+      zc f3;
+      if (s->dirichlet) f3 = v[0] * sr[0] + v[1] * sr[1] + v[2] * sr[2];
+      else {
+        double h3 = w->h[g][0] * w->h[g][1] * w->h[g][2];
+        f3 = (h3 * v[0]) * sr[0] + (h3 * v[1]) * sr[1] + (h3 * v[2]) * sr[2];
+      }
+      acc += w->wgt[g] * f3;
+    }
+    out[d] = acc;
+  }
+}
+
+/* Element matrices for one element in reference loop position (ie,je,ke).
+ * Emulates the call sequence of global_vfem: int_elem_params uses the GPML
+ * flags left by the previous get_pml call; get_pml(ie,je,ke) then runs.
+ * aloc: complex (me,me) row-major [im][jm]; bloc: complex (me,2) [im][d]. */
+static orc_ws *g_ws = NULL;
+int orc_element(orc_state *s, int ie, int je, int ke, double *aloc, double *bloc) {
+  if (!g_ws) g_ws = (orc_ws *)malloc(sizeof(orc_ws));
+  orc_ws *w = g_ws;
+  int rc = orc_elem_params(s, w, ie, je, ke);
+  if (!s->dirichlet) orc_get_pml(s, ie, je, ke);
+  if (rc) return rc;
+  int me = s->me;
+  if (aloc)
+    for (int im = 0; im < me; im++)
+      for (int jm = 0; jm < me; jm++) {
+        zc a = orc_alocal(s, w, im, jm);
+        aloc[2 * (im * me + jm)] = creal(a);
+        aloc[2 * (im * me + jm) + 1] = cimag(a);
+      }
+  if (bloc)
+    for (int im = 0; im < me; im++) {
+      zc b[2];
+      orc_blocal(s, w, im, b);
+      for (int d = 0; d < 2; d++) {
+        bloc[2 * (im * 2 + d)] = creal(b[d]);
+        bloc[2 * (im * 2 + d) + 1] = cimag(b[d]);
+      }
+    }
+  return 0;
+}
+
+/* force the GPML flag state (the module variable) for tests */
+void orc_set_in_pml(orc_state *s, int a, int b, int c) {
+  s->in_pml[0] = a; s->in_pml[1] = b; s->in_pml[2] = c;
+}
+
+/* ------------------------------ assembly --------------------------------- */
+/* stable argsort of key[perm[*]] (merge_sort/ga_merge, global_assembly.f90:1908-1962) */
+static void stable_argsort(const int32_t *key, int64_t *perm, int64_t n) {
+  int64_t *tmp = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  for (int64_t w = 1; w < n; w *= 2) {
+    for (int64_t lo = 0; lo < n; lo += 2 * w) {
+      int64_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+      int64_t i = lo, j = mid, k = lo;
+      while (i < mid && j < hi) tmp[k++] = (key[perm[i]] <= key[perm[j]]) ? perm[i++] : perm[j++];
+      while (i < mid) tmp[k++] = perm[i++];
+      while (j < hi) tmp[k++] = perm[j++];
+    }
+    memcpy(perm, tmp, sizeof(int64_t) * n);
+  }
+  free(tmp);
+}
+
+/* global_vfem + local_vfem (MoVFEM_3DMT.f90:167-263) with ga_assemble_nze,
+ * assign_aij/assign_bi and ga_sort_sparse.  ia/ja are 1-based; a, b complex
+ * interleaved; b has length 2*nne (polarization-major).  Returns 0 or <0. */
+int orc_global_vfem(orc_state *s, int32_t *ia, int32_t *ja, double *a, double *b) {
+  if (!s->gne) orc_build_gne(s);
+  if (!s->nzindx) orc_build_nzindx(s);
+  if (s->dirichlet && s->inimod != 1) return -4; /* f_boundary for inimod 2/3 not restated */
+  int me = s->me;
+  int64_t nnze = s->nnze;
+  int32_t nne = s->nne;
+  /* ga_assemble_nze */
+  for (int64_t i = 0; i < nnze; i++) ia[i] = ja[i] = -1;
+  for (int64_t e = 0; e < s->ne; e++) {
+    const int32_t *g = s->gne + e * me;
+    for (int im = 0; im < me; im++) {
+      if (g[im] < 0) continue;
+      for (int jm = 0; jm < me; jm++) {
+        if (g[jm] < 0) continue;
+        int32_t p = s->nzindx[(e * me + im) * me + jm];
+        if (p == -1) continue;
+        if (ia[p - 1] == -1 && ja[p - 1] == -1) {
+          ia[p - 1] = g[im];
+          ja[p - 1] = g[jm];
+        } else if (ia[p - 1] != g[im] || ja[p - 1] != g[jm]) {
+          return -3; /* 'ia, ja index are not same' */
+        }
+      }
+    }
+  }
+  zc *A = (zc *)calloc((size_t)nnze, sizeof(zc));
+  zc *B = (zc *)calloc((size_t)2 * nne, sizeof(zc));
+  double *aloc = (double *)malloc(sizeof(double) * 2 * me * me);
+  double *bloc = (double *)malloc(sizeof(double) * 2 * me * 2);
+  for (int ie = 1; ie <= s->nx; ie++)
+    for (int je = 1; je <= s->ny; je++)
+      for (int ke = 1; ke <= s->nz; ke++) {
+        int64_t e = ((int64_t)(ie - 1) * s->ny + (je - 1)) * s->nz + (ke - 1);
+        int rc = orc_element(s, ie, je, ke, aloc, bloc);
+        if (rc) {
+          free(A); free(B); free(aloc); free(bloc);
+          return rc;
+        }
+        const int32_t *g = s->gne + e * me;
+        for (int im = 0; im < me; im++) {
+          if (g[im] < 0) continue;
+          for (int jm = 0; jm < me; jm++) {
+            if (g[jm] < 0) continue;
+            if (g[im] < g[jm]) continue; /* sym: keep row >= col */
+            int32_t p = s->nzindx[(e * me + im) * me + jm];
+            if (p < 1 || ia[p - 1] != g[im] || ja[p - 1] != g[jm]) {
+              free(A); free(B); free(aloc); free(bloc);
+              return -3;
+            }
+            A[p - 1] += CMPLX(aloc[2 * (im * me + jm)], aloc[2 * (im * me + jm) + 1]);
+          }
+        }
+        for (int im = 0; im < me; im++) {
+          if (g[im] < 0) continue;
+          /* Dirichlet lift: f_boundary is zero for inimod 1 and for GPML */
+          zc bda[2] = {0.0, 0.0};
+          for (int d = 0; d < 2; d++) {
+            zc bi = CMPLX(bloc[2 * (im * 2 + d)], bloc[2 * (im * 2 + d) + 1]) - bda[d];
+            B[(int64_t)d * nne + g[im] - 1] += bi;
+          }
+        }
+      }
+  free(aloc); free(bloc);
+  /* ga_sort_sparse: stable by column (values permuted, complex64), then
+   * stable by row (indices permuted, values left in column order) */
+  int64_t *q = (int64_t *)malloc(sizeof(int64_t) * nnze);
+  int32_t *ti = (int32_t *)malloc(sizeof(int32_t) * nnze), *tj = (int32_t *)malloc(sizeof(int32_t) * nnze);
+  for (int64_t i = 0; i < nnze; i++) q[i] = i;
+  stable_argsort(ja, q, nnze);
+  zc *ta = (zc *)malloc(sizeof(zc) * nnze);
+  for (int64_t i = 0; i < nnze; i++) {
+    ti[i] = ia[q[i]];
+    tj[i] = ja[q[i]];
+    float complex c = (float complex)A[q[i]];
+    ta[i] = (zc)c;
+  }
+  memcpy(ia, ti, sizeof(int32_t) * nnze);
+  memcpy(ja, tj, sizeof(int32_t) * nnze);
+  for (int64_t i = 0; i < nnze; i++) q[i] = i;
+  stable_argsort(ia, q, nnze);
+  for (int64_t i = 0; i < nnze; i++) {
+    ti[i] = ia[q[i]];
+    tj[i] = ja[q[i]];
+  }
+  memcpy(ia, ti, sizeof(int32_t) * nnze);
+  memcpy(ja, tj, sizeof(int32_t) * nnze);
+  for (int64_t i = 0; i < nnze; i++) {
+    float complex c = (float complex)ta[i];
+    a[2 * i] = crealf(c);
+    a[2 * i + 1] = cimagf(c);
+  }
+  for (int64_t i = 0; i < 2 * (int64_t)nne; i++) {
+    b[2 * i] = creal(B[i]);
+    b[2 * i + 1] = cimag(B[i]);
+  }
+  free(q); free(ti); free(tj); free(ta); free(A); free(B);
+  return 0;
+}
+
+/* find_zeros + rem_zeros (global_assembly.f90:123-150): compacts in place,
+ * returns the new length */
+int64_t orc_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (a[2 * i] == 0.0 && a[2 * i + 1] == 0.0) continue;
+    ia[k] = ia[i];
+    ja[k] = ja[i];
+    a[2 * k] = a[2 * i];
+    a[2 * k + 1] = a[2 * i + 1];
+    k++;
+  }
+  return k;
+}

@@ -1,0 +1,175 @@
+"""ctypes front end of the CPU oracle (movfem_oracle.c) and of the reference build.
+
+TEST INFRASTRUCTURE ONLY -- the checker the HIP path is compared against.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it.
+
+Pinning status (what each restated function is checked against):
+  gne numbering (c_gne12/36/54)          -- reference build (_ref) on tiny grids, golden fixtures,
+                                            independent geometric derivation, DOF-count formulas
+  slot map (ga_nzindx/shr_nzindx*)       -- reference build, golden fixtures, sort-rule invariants
+  element matrices (alocal)              -- reference build, golden fixtures, closed-form brick
+                                            matrices (8-node), curl-curl null space K.G = 0
+  element RHS (blocal, p_source)         -- reference build, golden fixtures, closed form of the
+                                            primary field on a box
+  global_vfem (assembly + sort, c64)     -- reference build, golden fixtures
+  update_sigma linear-index semantics    -- reference build
+  Dirichlet lift for inimod 2/3           -- not restated (the oracle refuses inimod 2/3)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = C.CDLL(str(path))
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int32)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int] * 4 + [dp] * 5 + [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] * 2
+        L.orc_free.argtypes = [C.c_void_p]
+        L.orc_set_omega.argtypes = [C.c_void_p, C.c_double]
+        L.orc_update_sigma.argtypes = [C.c_void_p]
+        L.orc_get_sigma.argtypes = [C.c_void_p, dp]
+        L.orc_build_gne.restype = C.c_int32
+        L.orc_build_gne.argtypes = [C.c_void_p]
+        L.orc_build_nzindx.restype = C.c_int64
+        L.orc_build_nzindx.argtypes = [C.c_void_p]
+        L.orc_get_gne.argtypes = [C.c_void_p, ip]
+        L.orc_get_nzindx.argtypes = [C.c_void_p, ip]
+        L.orc_element.restype = C.c_int
+        L.orc_element.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.orc_set_in_pml.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.orc_global_vfem.restype = C.c_int
+        L.orc_global_vfem.argtypes = [C.c_void_p, ip, ip, dp, dp]
+        L.orc_remove_zeros.restype = C.c_int64
+        L.orc_remove_zeros.argtypes = [C.c_int64, ip, ip, dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Oracle:
+    """CPU restatement bound to one model; mirrors the reference's call order:
+    construct (initialize_vfem), set_frequency(i) (update_omega, update_sigma,
+    pset_pmodel), global_vfem()."""
+
+    def __init__(self, m):
+        self.m = m
+        L = lib()
+        self._keep = [np.ascontiguousarray(m.xp, np.float64), np.ascontiguousarray(m.yp, np.float64),
+                      np.ascontiguousarray(m.zp, np.float64),
+                      np.ascontiguousarray(m.sigma.astype(np.complex128)).view(np.float64),
+                      np.ascontiguousarray(m.mu, np.float64)]
+        xp, yp, zp, sg, mu = self._keep
+        self.h = L.orc_create(m.nx, m.ny, m.nz, m.nodes_per_elem, _dp(xp), _dp(yp), _dp(zp), _dp(sg), _dp(mu),
+                              int(m.dirichlet), m.inimod, m.gpml_sch, m.a0, m.b0, m.nn, m.nextd, m.nzl_top)
+        if not self.h:
+            raise ValueError("oracle rejected the model")
+        L.orc_set_omega(self.h, m.omega(0))
+        self.me = m.edges_per_elem
+        self._nne = None
+        self._nnze = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_free(self.h)
+        except Exception:
+            pass
+
+    def set_frequency(self, i: int):
+        L = lib()
+        L.orc_set_omega(self.h, self.m.omega(i))
+        L.orc_update_sigma(self.h)
+
+    def sigma(self) -> np.ndarray:
+        out = np.empty(self.m.npt * 12)
+        lib().orc_get_sigma(self.h, _dp(out))
+        return out.view(np.complex128).reshape(self.m.npt, 6)
+
+    def gne(self) -> np.ndarray:
+        L = lib()
+        self._nne = L.orc_build_gne(self.h)
+        out = np.empty(self.m.ne * self.me, np.int32)
+        L.orc_get_gne(self.h, _ip(out))
+        return out.reshape(self.m.ne, self.me)
+
+    @property
+    def nne(self):
+        if self._nne is None:
+            self.gne()
+        return self._nne
+
+    def nzindx(self) -> np.ndarray:
+        L = lib()
+        if self._nne is None:
+            self.gne()
+        self._nnze = L.orc_build_nzindx(self.h)
+        out = np.empty(self.m.ne * self.me * self.me, np.int32)
+        L.orc_get_nzindx(self.h, _ip(out))
+        return out.reshape(self.m.ne, self.me, self.me)
+
+    @property
+    def nnze(self):
+        if self._nnze is None:
+            self.nzindx()
+        return self._nnze
+
+    def set_in_pml(self, a, b, c):
+        lib().orc_set_in_pml(self.h, a, b, c)
+
+    def element(self, ie, je, ke):
+        """alocal (me, me) and blocal (me, 2) for the element at 1-based loop
+        position (ie, je, ke); advances the GPML flag state like global_vfem."""
+        me = self.me
+        a = np.empty(me * me * 2)
+        b = np.empty(me * 2 * 2)
+        rc = lib().orc_element(self.h, ie, je, ke, _dp(a), _dp(b))
+        if rc:
+            raise RuntimeError(f"orc_element failed ({rc})")
+        return a.view(np.complex128).reshape(me, me), b.view(np.complex128).reshape(me, 2)
+
+    def global_vfem(self):
+        """(ia, ja, a, b) exactly as the reference's global_vfem leaves them:
+        1-based row-major indices of the full pattern, complex64-rounded values
+        in column-sorted order, RHS (2*nne) polarization-major."""
+        n = self.nnze
+        nne = self.nne
+        ia = np.empty(n, np.int32)
+        ja = np.empty(n, np.int32)
+        a = np.empty(2 * n)
+        b = np.empty(4 * nne)
+        rc = lib().orc_global_vfem(self.h, _ip(ia), _ip(ja), _dp(a), _dp(b))
+        if rc:
+            raise RuntimeError(f"orc_global_vfem failed ({rc})")
+        return ia, ja, a.view(np.complex128), b.view(np.complex128)
+
+    @staticmethod
+    def remove_zeros(ia, ja, a):
+        ia, ja = ia.copy(), ja.copy()
+        av = np.ascontiguousarray(a.astype(np.complex128)).view(np.float64).copy()
+        k = lib().orc_remove_zeros(len(ia), _ip(ia), _ip(ja), _dp(av))
+        return ia[:k], ja[:k], av.view(np.complex128)[:k]
+
+
+def ref_available() -> bool:
+    return (HERE / "_ref" / "libmovfem_ref.so").exists()

@@ -1,0 +1,185 @@
+! ref_harness.f90 -- drives MoVFEM_3DMT's own modules (compiled from the
+! reference sources by oracle/Makefile into oracle/_ref/) through the call
+! sequence of the reference's global_vfem / local_vfem (MoVFEM_3DMT.f90:167-263).
+! Those two routines are contained in the main program, which cannot be compiled
+! here because it includes mpif.h and zmumps_struc.h; this file re-expresses
+! only their call sequence.  Every numerical routine called below is the
+! reference's own.
+!
+! TEST INFRASTRUCTURE ONLY: the checker for the oracle restatement.  One setup
+! per process (the reference's init routines allocate module arrays without
+! checking), so callers run each case in a fresh process.
+module mvf_ref_harness
+  use iso_c_binding
+  use kind_param
+  use geometry, only: g_nx, g_ny, g_nz, g_nnx, g_nny, g_nnz, g_nyz, g_npt, g_nordx, g_nordy, &
+       g_nordz, g_xp, g_yp, g_zp, g_sigma, g_mu, g_nsf, g_nzl, g_nf, g_freq, g_ztop, nextd, &
+       omega, pi, update_omega, update_sigma
+  use n_fem, only: init_n_fem, nf_get_r
+  use v_fem, only: init_v_fem
+  use problem, only: init_problem, p_elem_fields, pset_pmodel
+  use boundary_conds, only: gpml_sch, a0, b0, nn, bd_setmodel, bd_updatemodel, get_pml, &
+       dirichlet, f_boundary
+  use integration, only: init_integration, int_elem_params, alocal, blocal
+  use global_assembly, only: ga_init, ga_assemble_nze, ga_sort_sparse, assign_aij, assign_bi, &
+       gne, nzindx, nne, nnze, sym
+  implicit none
+  integer, save :: h_me = 0
+contains
+
+  subroutine ref_setup(nx, ny, nz, mn, xp, yp, zp, sig, mu, dirich, inimod, sch, a0_, b0_, nn_, &
+       nextd_, nzl_top, ztop, nf, freqs) bind(C, name='ref_setup')
+    integer(c_int), value :: nx, ny, nz, mn, dirich, inimod, sch, nextd_, nzl_top, nf
+    real(c_double), value :: a0_, b0_, nn_, ztop
+    real(c_double), intent(in) :: xp(*), yp(*), zp(*), mu(*), freqs(*)
+    complex(c_double_complex), intent(in) :: sig(*)
+    integer :: o, i, k
+    real(kind=double) :: lsig(1), ldz(1)
+
+    g_nx = nx + 1; g_ny = ny + 1; g_nz = nz + 1
+    if (mn == 8) then
+      g_nordx = 2; g_nordy = 2; g_nordz = 2
+    else
+      g_nordx = 3; g_nordy = 3; g_nordz = 3
+    end if
+    o = g_nordx - 1
+    g_nnx = nx*o + 1; g_nny = ny*o + 1; g_nnz = nz*o + 1
+    g_nyz = g_nny*g_nnz; g_npt = g_nnx*g_nyz
+    allocate(g_xp(g_nnx), g_yp(g_nny), g_zp(g_npt), g_sigma(6, g_npt), g_mu(6, g_npt))
+    g_xp = xp(1:g_nnx); g_yp = yp(1:g_nny); g_zp = zp(1:g_npt)
+    do i = 1, g_npt
+      do k = 1, 6
+        g_sigma(k, i) = sig(6*(i - 1) + k)
+        g_mu(k, i) = mu(6*(i - 1) + k)
+      end do
+    end do
+    nextd = nextd_
+    g_nsf = 2
+    allocate(g_nzl(g_nsf)); g_nzl = nzl_top
+    g_nf = nf
+    allocate(g_freq(nf)); g_freq = freqs(1:nf)
+    g_ztop = ztop
+    omega = 2.d0*pi*g_freq(1)
+    gpml_sch = sch; a0 = a0_; b0 = b0_; nn = nn_
+    ! initialize_vfem order (MoVFEM_3DMT.f90:354-388)
+    call init_n_fem(mn)
+    select case (mn)
+    case (8); call init_v_fem(12); h_me = 12
+    case (20); call init_v_fem(36); h_me = 36
+    case (27); call init_v_fem(54); h_me = 54
+    end select
+    call init_problem(1, 2, 1)
+    call init_integration()
+    call ga_init(.true., dirich /= 0, inimod)
+    lsig = 0.01d0; ldz = 0.d0
+    call bd_setmodel(0.01d0, 1, lsig, ldz)
+  end subroutine ref_setup
+
+  ! frequency-loop state update (MoVFEM_3DMT.f90:66-69); write_problem_ugrid is
+  ! output only and skipped
+  subroutine ref_set_frequency(ifreq) bind(C, name='ref_set_frequency')
+    integer(c_int), value :: ifreq
+    call update_omega(ifreq)
+    call update_sigma()
+    call pset_pmodel()
+    call bd_updatemodel()
+  end subroutine ref_set_frequency
+
+  subroutine ref_sizes(o_nne, o_nnze) bind(C, name='ref_sizes')
+    integer(c_int), intent(out) :: o_nne
+    integer(c_int64_t), intent(out) :: o_nnze
+    o_nne = nne; o_nnze = nnze
+  end subroutine ref_sizes
+
+  subroutine ref_gne(out) bind(C, name='ref_gne')
+    integer(c_int), intent(out) :: out(h_me, *)
+    integer :: e, im
+    do e = 1, size(gne, 1)
+      do im = 1, h_me
+        out(im, e) = gne(e, im)
+      end do
+    end do
+  end subroutine ref_gne
+
+  subroutine ref_nzindx(out) bind(C, name='ref_nzindx')
+    integer(c_int), intent(out) :: out(*)
+    out(1:size(nzindx)) = nzindx
+  end subroutine ref_nzindx
+
+  subroutine ref_get_sigma(out) bind(C, name='ref_get_sigma')
+    complex(c_double_complex), intent(out) :: out(6, *)
+    out(:, 1:g_npt) = g_sigma
+  end subroutine ref_get_sigma
+
+  ! one element in loop position (ie,je,ke): the per-element calls of
+  ! global_vfem, then every alocal(im,jm) and blocal(im)
+  subroutine ref_element(ie, je, ke, aloc, bloc) bind(C, name='ref_element')
+    integer(c_int), value :: ie, je, ke
+    complex(c_double_complex), intent(out) :: aloc(h_me, h_me), bloc(2, h_me)
+    integer :: eno, im, jm
+    complex(kind=double) :: b2(2)
+    eno = (ie - 1)*g_nyz*(g_nordx - 1) + (je - 1)*g_nnz*(g_nordy - 1) + (ke - 1)*(g_nordz - 1) + 1
+    call nf_get_r(ie, je, ke, eno)
+    call p_elem_fields()
+    call int_elem_params()
+    if (.not. dirichlet) call get_pml(ie, je, ke)
+    do im = 1, h_me
+      do jm = 1, h_me
+        aloc(jm, im) = alocal(im, jm)
+      end do
+      b2 = blocal(im)
+      bloc(:, im) = b2
+    end do
+  end subroutine ref_element
+
+  ! global_vfem (MoVFEM_3DMT.f90:167-216)
+  subroutine ref_global_vfem(ia, ja, a, b) bind(C, name='ref_global_vfem')
+    integer(c_int), intent(inout) :: ia(*), ja(*)
+    complex(c_double_complex), intent(inout) :: a(*), b(*)
+    integer :: ie, je, ke, ide, eno
+    call ga_assemble_nze(ia, ja)
+    a(1:nnze) = cmplx(0.d0, 0.d0, kind=double)
+    b(1:2*nne) = cmplx(0.d0, 0.d0, kind=double)
+    do ie = 1, g_nx - 1
+      do je = 1, g_ny - 1
+        do ke = 1, g_nz - 1
+          eno = (ie - 1)*g_nyz*(g_nordx - 1) + (je - 1)*g_nnz*(g_nordy - 1) + (ke - 1)*(g_nordz - 1) + 1
+          ide = (ie - 1)*(g_ny - 1)*(g_nz - 1) + (je - 1)*(g_nz - 1) + ke
+          call nf_get_r(ie, je, ke, eno)
+          call p_elem_fields()
+          call int_elem_params()
+          if (.not. dirichlet) call get_pml(ie, je, ke)
+          call local_vfem_h(ide, ia, ja, a, b)
+        end do
+      end do
+    end do
+    deallocate(nzindx)
+    call ga_sort_sparse(ia, ja, a)
+  end subroutine ref_global_vfem
+
+  ! local_vfem (MoVFEM_3DMT.f90:221-263)
+  subroutine local_vfem_h(ide, ia, ja, a, b)
+    integer, intent(in) :: ide
+    integer, dimension(*), intent(inout) :: ia, ja
+    complex(kind=double), dimension(*), intent(inout) :: a, b
+    integer :: im, jm
+    complex(kind=double), dimension(2) :: bda
+    do im = 1, h_me
+      if (gne(ide, im) < 0) cycle
+      do jm = 1, h_me
+        if (gne(ide, jm) < 0) cycle
+        if (sym .and. gne(ide, im) < gne(ide, jm)) cycle
+        call assign_aij(ide, im, jm, alocal(im, jm), a, ia, ja)
+      end do
+    end do
+    do im = 1, h_me
+      if (gne(ide, im) < 0) cycle
+      bda = cmplx(0.d0, 0.d0)
+      do jm = 1, h_me
+        if (gne(ide, jm) >= 0) cycle
+        bda = bda + f_boundary(-gne(ide, jm), jm)*alocal(im, jm)
+      end do
+      call assign_bi(ide, im, (blocal(im) - bda), b)
+    end do
+  end subroutine local_vfem_h
+end module mvf_ref_harness

@@ -1,0 +1,119 @@
+"""Run the reference build (_ref/libmovfem_ref.so) on one model, in its own process.
+
+TEST INFRASTRUCTURE ONLY.  The reference's init routines allocate module arrays
+without checking, so each model needs a fresh process:
+
+    python -m oracle.refrun in.npz out.npz
+
+in.npz holds the model (see dump_model) and a request list; out.npz receives
+gne, nzindx, the element matrices of the requested elements (in request order,
+which also advances the GPML flag state) and global_vfem for each frequency.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+import sys
+import tempfile
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+
+def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=()):
+    np.savez(path, nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp,
+             sigma=m.sigma, mu=m.mu, freqs=m.freqs, dirichlet=int(m.dirichlet), inimod=m.inimod,
+             gpml_sch=m.gpml_sch, a0=m.a0, b0=m.b0, nn=m.nn, nextd=m.nextd, nzl_top=m.nzl_top,
+             ztop=m.ztop, elements=np.asarray(elements, np.int32).reshape(-1, 3),
+             vfem_freqs=np.asarray(vfem_freqs, np.int32), sigma_after=np.asarray(sigma_after, np.int32))
+
+
+def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600):
+    """Returns a dict of reference outputs for model m (fresh subprocess)."""
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = pathlib.Path(td) / "in.npz", pathlib.Path(td) / "out.npz"
+        dump_model(m, fin, elements, vfem_freqs, sigma_after)
+        r = subprocess.run([sys.executable, "-m", "oracle.refrun", str(fin), str(fout)],
+                           cwd=str(HERE.parent), capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError("reference run failed:\n" + r.stdout[-2000:] + r.stderr[-2000:])
+        with np.load(fout) as z:
+            return {k: z[k] for k in z.files}
+
+
+def _main(fin, fout):
+    z = np.load(fin)
+    L = C.CDLL(str(HERE / "_ref" / "libmovfem_ref.so"))
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    L.ref_setup.argtypes = ([C.c_int] * 4 + [dp] * 5 + [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] * 2
+                            + [C.c_double, C.c_int, dp])
+    L.ref_set_frequency.argtypes = [C.c_int]
+    L.ref_sizes.argtypes = [ip, C.POINTER(C.c_int64)]
+    L.ref_gne.argtypes = [ip]
+    L.ref_nzindx.argtypes = [ip]
+    L.ref_get_sigma.argtypes = [dp]
+    L.ref_element.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    L.ref_global_vfem.argtypes = [ip, ip, dp, dp]
+
+    def d(a):
+        return a.ctypes.data_as(dp)
+
+    xp, yp, zp = (np.ascontiguousarray(z[k], np.float64) for k in ("xp", "yp", "zp"))
+    sg = np.ascontiguousarray(z["sigma"].astype(np.complex128)).view(np.float64)
+    mu = np.ascontiguousarray(z["mu"], np.float64)
+    fr = np.ascontiguousarray(z["freqs"], np.float64)
+    L.ref_setup(int(z["nx"]), int(z["ny"]), int(z["nz"]), int(z["mn"]), d(xp), d(yp), d(zp), d(sg), d(mu),
+                int(z["dirichlet"]), int(z["inimod"]), int(z["gpml_sch"]), float(z["a0"]), float(z["b0"]),
+                float(z["nn"]), int(z["nextd"]), int(z["nzl_top"]), float(z["ztop"]), len(fr), d(fr))
+    nne = C.c_int32()
+    nnze = C.c_int64()
+    L.ref_sizes(C.byref(nne), C.byref(nnze))
+    mn = int(z["mn"])
+    me = {8: 12, 20: 36, 27: 54}[mn]
+    ne = int(z["nx"]) * int(z["ny"]) * int(z["nz"])
+    out = {"nne": np.int64(nne.value), "nnze": np.int64(nnze.value)}
+    g = np.empty(ne * me, np.int32)
+    L.ref_gne(g.ctypes.data_as(ip))
+    out["gne"] = g.reshape(ne, me)
+    nz = np.empty(ne * me * me, np.int32)
+    L.ref_nzindx(nz.ctypes.data_as(ip))
+    out["nzindx"] = nz.reshape(ne, me, me)
+    npt = len(zp)
+    L.ref_set_frequency(1)
+    els = z["elements"]
+    if len(els):
+        A = np.empty((len(els), me * me * 2))
+        B = np.empty((len(els), me * 2 * 2))
+        for t, (ie, je, ke) in enumerate(els):
+            a = np.empty(me * me * 2)
+            b = np.empty(me * 4)
+            L.ref_element(int(ie), int(je), int(ke), d(a), d(b))
+            A[t], B[t] = a, b
+        out["aloc"] = A.view(np.complex128).reshape(len(els), me, me)
+        out["bloc"] = B.view(np.complex128).reshape(len(els), me, 2)
+    for fi in z["vfem_freqs"]:
+        fi = int(fi)
+        if fi != 1:
+            L.ref_set_frequency(fi)
+        L.ref_sizes(C.byref(nne), C.byref(nnze))
+        n = nnze.value
+        ia = np.empty(n, np.int32)
+        ja = np.empty(n, np.int32)
+        a = np.empty(2 * n)
+        b = np.empty(4 * nne.value)
+        L.ref_global_vfem(ia.ctypes.data_as(ip), ja.ctypes.data_as(ip), d(a), d(b))
+        out[f"ia_{fi}"], out[f"ja_{fi}"] = ia, ja
+        out[f"a_{fi}"], out[f"b_{fi}"] = a.view(np.complex128), b.view(np.complex128)
+    for fi in z["sigma_after"]:
+        L.ref_set_frequency(int(fi))
+        s = np.empty(npt * 12)
+        L.ref_get_sigma(d(s))
+        out[f"sigma_{int(fi)}"] = s.view(np.complex128).reshape(npt, 6)
+    np.savez(fout, **out)
+
+
+if __name__ == "__main__":
+    _main(sys.argv[1], sys.argv[2])
