@@ -1,0 +1,200 @@
+/*
+ * movfem_gpu.h -- C ABI of the MI355X element-integration, assembly and solve
+ * path for MoVFEM_3DMT.  Plain pointers and sizes only; callable from Fortran
+ * through ISO_C_BINDING (movfem_amd/fortran/movfem_gpu_mod.f90), from C/C++,
+ * and from Python (ctypes).
+ *
+ * Layouts (all kept from the reference):
+ *   node id       (i-1)*nny*nnz + (j-1)*nnz + k, z fastest      (n_fem.f90:81)
+ *   element id    (ie-1)*ny*nz + (je-1)*nz + ke, z fastest     (MoVFEM_3DMT.f90:199)
+ *   6-tensors     [11,12,13,22,23,33]                           (geometry.f90:1055-1060)
+ *   complex       interleaved (re, im) doubles, Fortran complex(8) compatible
+ *   COO indices   1-based (MUMPS convention)                    (MoVFEM_3DMT.f90:72-78)
+ *   RHS/solution  column-major (nne, ndir), polarization d at offset (d-1)*nne
+ *                                                               (global_assembly.f90:70-74)
+ * Errors: every entry point returns 0 or a negative MVF_E_* code and never
+ * aborts the process; mvf_last_error(h) holds the message (the reference
+ * prints and stops: global_assembly.f90:55-57, n_fem.f90:374-377,
+ * problem.f90:260-271).
+ * Threading: one handle per host thread; calls on one handle are serialized
+ * and stream-ordered on the handle's HIP stream.
+ */
+#ifndef MOVFEM_GPU_H
+#define MOVFEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MVF_OK = 0,
+  MVF_E_ARG = -1,        /* invalid argument / size mismatch                       */
+  MVF_E_DEVICE = -2,     /* HIP runtime error                                      */
+  MVF_E_SINGULAR = -3,   /* zero Jacobian or singular sigma/mu tensor at a node   */
+  MVF_E_INDEX = -4,      /* slot/index consistency failure                         */
+  MVF_E_UNSUPPORTED = -5,/* option not implemented (e.g. Dirichlet inimod 2/3)     */
+  MVF_E_NOCONV = -6,     /* Krylov did not reach the tolerance                     */
+  MVF_E_BREAKDOWN = -7,  /* Krylov breakdown (rho or omega vanished)               */
+  MVF_E_MEMORY = -8      /* device allocation failed                               */
+};
+
+/* Grid: the reference's node lattice (geometry.f90:17-26).  nx,ny,nz count
+ * elements (the reference's g_nx-1, g_ny-1, g_nz-1). */
+typedef struct mvf_grid {
+  int32_t nx, ny, nz;
+  int32_t nodes_per_elem; /* 8, 20 or 27 (nf_mn, n_fem.f90:30) */
+  const double *xp;       /* g_xp(nnx) */
+  const double *yp;       /* g_yp(nny) */
+  const double *zp;       /* g_zp(npt), z measured from the grid bottom */
+} mvf_grid;
+
+/* Model: g_sigma complex(6,npt) and g_mu real(6,npt) as the reference holds
+ * them after grid_3d (geometry.f90:1037-1085), i.e. with +i*float32(omega1*eps)
+ * on sigma's diagonal for the first frequency. */
+typedef struct mvf_model {
+  const double *sigma; /* complex (6, npt) interleaved */
+  const double *mu;    /* real    (6, npt) */
+} mvf_model;
+
+/* Boundary conditions (ga_init / init_gpml / bd_setmodel state,
+ * global_assembly.f90:26-41, boundary_conds.f90:51-70). */
+typedef struct mvf_bc {
+  int32_t dirichlet; /* 1: Dirichlet edges removed; 0: GPML                     */
+  int32_t inimod;    /* Dirichlet model: 1 zero (2/3 -> MVF_E_UNSUPPORTED)      */
+  int32_t gpml_sch;  /* 0 Fang, 1 Zhou (Re h = 1 under the reference semantics) */
+  int32_t nextd;     /* extension cells; GPML zone = last nextd elements        */
+  int32_t nzl_top;   /* g_nzl(g_nsf): top-zone layers, locates za(2)            */
+  int32_t pad_;
+  double a0, b0, nn; /* GPML constants                                          */
+} mvf_bc;
+
+/* Emulation switches (default: all on = the reference's effective semantics). */
+enum {
+  MVF_EMU_GPML_PREV_FLAGS = 1, /* GPML zone of element e taken from element e-1 (get_pml after
+                                  int_elem_params, MoVFEM_3DMT.f90:203-206), carried between calls */
+  MVF_EMU_UPDATE_SIGMA = 2,    /* update_sigma touches linear entries < npt+30 only (geometry.f90:146-152) */
+  MVF_EMU_ALL = 3
+};
+
+typedef struct mvf_options {
+  int32_t emulate;    /* MVF_EMU_* bit set */
+  int32_t device;     /* HIP device ordinal */
+  int32_t verbose;    /* 0 quiet */
+  int32_t pad_;
+} mvf_options;
+
+typedef struct mvf_handle mvf_handle;
+
+/* ---- setup (replaces ga_init + init_* of initialize_vfem,
+ *      MoVFEM_3DMT.f90:354-388; global_assembly.f90:26-41) -------------------
+ * Uploads lattice and model, numbers the DOFs (c_gne*, global_assembly.f90:
+ * 183-475), builds the sparsity pattern and the element->entry gather lists
+ * (ga_nzindx, :197-229), integrates the frequency-independent element
+ * curl-curl (K) and mass (M) matrices and assembles them on the device. */
+int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc,
+             const mvf_options *opts, mvf_handle **out);
+void mvf_free(mvf_handle *h);
+const char *mvf_last_error(const mvf_handle *h);
+
+/* sizes: nne (global DOFs, ga_cgne :194), nnze (full pattern entries, the
+ * reference's slot count :228), nnz_triu (entries with row <= col) */
+int mvf_sizes(const mvf_handle *h, int32_t *nne, int64_t *nnze, int64_t *nnz_triu);
+
+/* DOF numbering gne(ne, me), element-major, negative = Dirichlet face code
+ * (global_assembly.f90:17,266-285). */
+int mvf_get_gne(mvf_handle *h, int32_t *gne_out);
+
+/* ---- per frequency (update_omega, update_sigma, pset_pmodel, bd_updatemodel;
+ *      MoVFEM_3DMT.f90:66-69, geometry.f90:137-153, problem.f90:247-252) ---- */
+int mvf_set_frequency(mvf_handle *h, double freq_hz);
+
+/* ---- element level (int_elem_params / alocal / blocal,
+ *      integration.f90:60-106) ----------------------------------------------
+ * For elements [e0, e1) (0-based, reference loop order): K and M upper-packed
+ * (row-major i<=j, E(E+1)/2 doubles per element; A_ij = K_ij + i*fl32(omega)*M_ij
+ * is alocal) and b (E x 2 complex per element, blocal) at the current
+ * frequency.  Any output pointer may be NULL.  Host buffers. */
+int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me, double *be);
+
+/* ---- drop-in for global_vfem(ia, ja, a, b) (MoVFEM_3DMT.f90:167-216) -------
+ * Fills exactly what the reference's global_vfem leaves after ga_sort_sparse
+ * and before zero removal: ia/ja 1-based row-major over the full pattern,
+ * a the complex64-rounded values in the reference's column-sorted order, b the
+ * 2*nne RHS.  nnze/nne/ndir must match mvf_sizes; ndir must be 2. */
+int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze,
+                    int32_t nne, int32_t ndir);
+
+/* find_zeros + rem_zeros (global_assembly.f90:123-150): in-place compaction of
+ * a COO triplet, returns the new length in *n_out. Host only. */
+int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_out);
+
+/* ---- solve (replaces zmumps(job=6), MoVFEM_3DMT.f90:113-115) --------------- */
+enum { MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1 };
+enum { MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1 };  /* MIXED: FP32 inner Krylov, FP64 refinement */
+enum { MVF_MATRIX_C64REF = 0, MVF_MATRIX_FP64 = 1 };/* which operator the residual is judged on */
+enum { MVF_PC_NONE = 0, MVF_PC_JACOBI = 1 };
+
+typedef struct mvf_solver_opts {
+  int32_t method;       /* MVF_KRYLOV_* */
+  int32_t precision;    /* MVF_PREC_* */
+  int32_t matrix;       /* MVF_MATRIX_*: C64REF = the complex64-rounded matrix the reference hands MUMPS */
+  int32_t precond;      /* MVF_PC_* */
+  double tol;           /* FP64 true relative residual ||b-Ax||/||b|| per polarization */
+  int32_t maxit;        /* total Krylov iterations */
+  int32_t check_every;  /* iterations between convergence checks (host sync) */
+  double inner_tol;     /* MIXED: FP32 residual drop before an FP64 refinement step */
+  int32_t use_graph;    /* capture the iteration block in a HIP graph */
+  int32_t pad_;
+} mvf_solver_opts;
+
+typedef struct mvf_solve_info {
+  int32_t iterations;      /* Krylov iterations performed (max over columns) */
+  int32_t refinements;     /* FP64 refinement steps (MIXED) */
+  int32_t converged;       /* 1 if every column met tol */
+  int32_t fallbacks;       /* restarts after breakdown */
+  double true_resid[64];   /* final FP64 ||b-Ax||/||b|| per column (column = freq*2 + pol) */
+  double seconds;          /* wall time of the solve on the device stream */
+  /* SpMV kernels timed with HIP events on the handle's stream
+   * [0] FP64 operator, [1] FP32 operator (MIXED inner iterations) */
+  int64_t spmv_launches[2];
+  double spmv_ms[2];       /* summed kernel time */
+  double spmv_bytes[2];    /* algorithmic bytes of one launch: nnz*(K+M+col) + nne*(rowptr + ncol*(x+y)) */
+  int32_t spmv_ncol;       /* columns per launch (last batch) */
+  int32_t pad_;
+} mvf_solve_info;
+
+void mvf_default_solver_opts(mvf_solver_opts *o);
+
+/* zmumps(job=6)-shaped: rhs_inout is the host (nne, 2) complex RHS as
+ * global_vfem filled it; it is overwritten by the solution, as MUMPS does. */
+int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_solve_info *info);
+
+/* Device-resident path: RHS integrated on the device for each of nfreq
+ * frequencies (processed in that order, as the reference's frequency loop
+ * would), all 2*nfreq columns solved together by one batched Krylov sweep
+ * sharing K and M.  Solutions stay on the device; fetch with
+ * mvf_get_solution.  nfreq <= 8. */
+int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, const mvf_solver_opts *o,
+                          mvf_solve_info *info);
+/* column-major (nne, 2*nfreq) complex solution of the last solve_frequencies */
+int mvf_get_solution(mvf_handle *h, double *x_out, int32_t ncols);
+/* device RHS (nne, 2*nfreq) of the last solve_frequencies, for checks */
+int mvf_get_rhs(mvf_handle *h, double *b_out, int32_t ncols);
+
+/* Timing of the last setup phases, seconds: [0] numbering, [1] pattern,
+ * [2] element integration (K, M), [3] assembly. */
+int mvf_setup_times(const mvf_handle *h, double *t4);
+
+/* Re-integrates every element's K and M from the resident lattice and model
+ * and re-assembles them in place (the element loop of global_vfem,
+ * MoVFEM_3DMT.f90:193-211, without any host transfer).  precision 0: FP64
+ * integration; 1: FP32 integration (K and M rounded from FP32 element sums).
+ * integrate_ms (optional) receives the element kernel's time from HIP events. */
+int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOVFEM_GPU_H */

@@ -1,0 +1,100 @@
+"""ctypes bindings of libmovfem_gpu.so (include/movfem_gpu.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libmovfem_gpu.so"
+
+MVF_EMU_GPML_PREV_FLAGS = 1
+MVF_EMU_UPDATE_SIGMA = 2
+MVF_EMU_ALL = 3
+MVF_KRYLOV_COCG = 0
+MVF_KRYLOV_BICGSTAB = 1
+MVF_PREC_FP64 = 0
+MVF_PREC_MIXED = 1
+MVF_MATRIX_C64REF = 0
+MVF_MATRIX_FP64 = 1
+MVF_PC_NONE = 0
+MVF_PC_JACOBI = 1
+
+ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR", -4: "MVF_E_INDEX",
+          -5: "MVF_E_UNSUPPORTED", -6: "MVF_E_NOCONV", -7: "MVF_E_BREAKDOWN", -8: "MVF_E_MEMORY"}
+
+# every entry point of include/movfem_gpu.h
+EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne", "mvf_set_frequency",
+           "mvf_integrate", "mvf_global_vfem", "mvf_remove_zeros", "mvf_default_solver_opts", "mvf_solve",
+           "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble")
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nodes_per_elem", C.c_int32),
+                ("xp", C.POINTER(C.c_double)), ("yp", C.POINTER(C.c_double)), ("zp", C.POINTER(C.c_double))]
+
+
+class Model(C.Structure):
+    _fields_ = [("sigma", C.POINTER(C.c_double)), ("mu", C.POINTER(C.c_double))]
+
+
+class BC(C.Structure):
+    _fields_ = [("dirichlet", C.c_int32), ("inimod", C.c_int32), ("gpml_sch", C.c_int32), ("nextd", C.c_int32),
+                ("nzl_top", C.c_int32), ("pad_", C.c_int32), ("a0", C.c_double), ("b0", C.c_double),
+                ("nn", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("emulate", C.c_int32), ("device", C.c_int32), ("verbose", C.c_int32), ("pad_", C.c_int32)]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("matrix", C.c_int32), ("precond", C.c_int32),
+                ("tol", C.c_double), ("maxit", C.c_int32), ("check_every", C.c_int32), ("inner_tol", C.c_double),
+                ("use_graph", C.c_int32), ("pad_", C.c_int32)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
+                ("fallbacks", C.c_int32), ("true_resid", C.c_double * 64), ("seconds", C.c_double),
+                ("spmv_launches", C.c_int64 * 2), ("spmv_ms", C.c_double * 2), ("spmv_bytes", C.c_double * 2),
+                ("spmv_ncol", C.c_int32), ("pad_", C.c_int32)]
+
+
+_L = None
+
+
+def lib():
+    """Load the HIP library; fails loudly when it has not been built."""
+    global _L
+    if _L is not None:
+        return _L
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m movfem_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+    dp, ip, i64p = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    vp = C.c_void_p
+    L.mvf_init.restype = C.c_int
+    L.mvf_init.argtypes = [C.POINTER(Grid), C.POINTER(Model), C.POINTER(BC), C.POINTER(Options), C.POINTER(vp)]
+    L.mvf_free.restype = None
+    L.mvf_free.argtypes = [vp]
+    L.mvf_last_error.restype = C.c_char_p
+    L.mvf_last_error.argtypes = [vp]
+    L.mvf_sizes.argtypes = [vp, ip, i64p, i64p]
+    L.mvf_get_gne.argtypes = [vp, ip]
+    L.mvf_set_frequency.argtypes = [vp, C.c_double]
+    L.mvf_integrate.argtypes = [vp, C.c_int64, C.c_int64, dp, dp, dp]
+    L.mvf_global_vfem.argtypes = [vp, ip, ip, dp, dp, C.c_int64, C.c_int32, C.c_int32]
+    L.mvf_remove_zeros.argtypes = [C.c_int64, ip, ip, dp, i64p]
+    L.mvf_default_solver_opts.restype = None
+    L.mvf_default_solver_opts.argtypes = [C.POINTER(SolverOpts)]
+    L.mvf_solve.argtypes = [vp, dp, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
+    L.mvf_solve_frequencies.argtypes = [vp, dp, C.c_int32, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
+    L.mvf_get_solution.argtypes = [vp, dp, C.c_int32]
+    L.mvf_get_rhs.argtypes = [vp, dp, C.c_int32]
+    L.mvf_setup_times.argtypes = [vp, dp]
+    L.mvf_assemble.argtypes = [vp, C.c_int32, dp]
+    for name in EXPORTS:
+        if name not in ("mvf_free", "mvf_last_error", "mvf_default_solver_opts"):
+            getattr(L, name).restype = C.c_int
+    _L = L
+    return L

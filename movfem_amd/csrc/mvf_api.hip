@@ -1,0 +1,579 @@
+// C ABI of the MI355X element-integration, assembly and solve path
+// (declared in include/movfem_gpu.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "mvf_internal.h"
+
+namespace mvf {
+int solve_device(mvf_handle *h, const double *omegas, int ncol, int ncol_pad, const mvf_solver_opts *o,
+                 const double *b, double *x, mvf_solve_info *info);
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// the flags element 0 uses at assembly call index c (reference: in_pml is 0
+// before the first get_pml, and holds the last element's flags afterwards)
+static int first_flags_for(const mvf_handle *h, int64_t call) {
+  if (!h->gp.on) return 0;
+  return call >= 1 ? h->flags_late : h->flags_first;
+}
+
+// element 0's contribution switches from the first-call flags to the carried
+// ones once the first assembly is done: K += dK0, M += dM0 on its pairs
+__global__ void k_apply_carry(const int32_t *gne0, int me, const int64_t *rowptr, const int32_t *col,
+                              const double *dK, const double *dM, double *K, double *M, float *K32, float *M32,
+                              double sign) {
+  int np = me * (me + 1) / 2;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    int i = 0, rem = p;
+    while (rem >= me - i) {
+      rem -= me - i;
+      i++;
+    }
+    int j = i + rem;
+    int gi = gne0[i], gj = gne0[j];
+    if (gi < 0 || gj < 0) continue;
+    for (int s = 0; s < (i == j ? 1 : 2); s++) {
+      int r = (s == 0 ? gi : gj) - 1, c = (s == 0 ? gj : gi) - 1;
+      int64_t lo = rowptr[r], hi = rowptr[r + 1] - 1, pos = -1;
+      while (lo <= hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (col[mid] == c) {
+          pos = mid;
+          break;
+        }
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid - 1;
+      }
+      if (pos < 0) continue;
+      K[pos] += sign * dK[p];
+      M[pos] += sign * dM[p];
+      K32[pos] = (float)K[pos];
+      M32[pos] = (float)M[pos];
+    }
+  }
+}
+
+static int ensure_call_state(mvf_handle *h, int64_t call) {
+  if (!h->carry_nonzero) return 0;
+  int want = call >= 1 ? 1 : 0;
+  if (want == h->carry_applied) return 0;
+  double sign = want ? 1.0 : -1.0;
+  hipLaunchKernelGGL(k_apply_carry, dim3(1), dim3(256), 0, h->stream, h->d_gne, h->et.ne, h->d_rowptr, h->d_col,
+                     h->d_dK0, h->d_dM0, h->d_K, h->d_M, h->d_K32, h->d_M32, sign);
+  MVF_HIP(h, hipGetLastError());
+  h->carry_applied = want;
+  return 0;
+}
+
+static void inv6(const double *a, double *b, double &d) {
+  d = a[0] * (a[3] * a[5] - a[4] * a[4]) + a[1] * (a[2] * a[4] - a[1] * a[5]) + a[2] * (a[1] * a[4] - a[3] * a[2]);
+  b[0] = (a[3] * a[5] - a[4] * a[4]) / d;
+  b[1] = (a[2] * a[4] - a[1] * a[5]) / d;
+  b[2] = (a[1] * a[4] - a[2] * a[3]) / d;
+  b[3] = (a[0] * a[5] - a[2] * a[2]) / d;
+  b[4] = (a[2] * a[1] - a[0] * a[4]) / d;
+  b[5] = (a[3] * a[0] - a[1] * a[1]) / d;
+}
+
+}  // namespace mvf
+
+using namespace mvf;
+
+extern "C" {
+
+void mvf_default_solver_opts(mvf_solver_opts *o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->method = MVF_KRYLOV_COCG;
+  o->precision = MVF_PREC_FP64;
+  o->matrix = MVF_MATRIX_C64REF;
+  o->precond = MVF_PC_JACOBI;
+  o->tol = 1e-8;
+  o->maxit = 100000;
+  o->check_every = 25;
+  o->inner_tol = 1e-2;
+  o->use_graph = 0;
+}
+
+const char *mvf_last_error(const mvf_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+void mvf_free(mvf_handle *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  void *ptrs[] = {h->d_xp,  h->d_yp,  h->d_zp,   h->d_sig,   h->d_inmu,     h->d_dmu,      h->d_tab,
+                  h->d_gne, h->d_rowptr, h->d_col, h->d_diagpos, h->d_seg, h->d_contrib, h->d_rseg,
+                  h->d_rcontrib, h->d_K, h->d_M, h->d_K32, h->d_M32, h->d_Ke, h->d_Me, h->d_dK0, h->d_dM0,
+                  h->d_x,  h->d_b};
+  for (void *p : ptrs) dfree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, const mvf_options *opts,
+             mvf_handle **out) {
+  if (!out) return MVF_E_ARG;
+  *out = nullptr;
+  if (!grid || !model || !bc || !grid->xp || !grid->yp || !grid->zp || !model->sigma || !model->mu) return MVF_E_ARG;
+  if (grid->nodes_per_elem != 8 && grid->nodes_per_elem != 20 && grid->nodes_per_elem != 27) return MVF_E_ARG;
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return MVF_E_ARG;
+  mvf_handle *h = new (std::nothrow) mvf_handle();
+  if (!h) return MVF_E_MEMORY;
+  *out = h;
+  h->device = opts ? opts->device : 0;
+  h->emulate = opts ? opts->emulate : MVF_EMU_ALL;
+  h->verbose = opts ? opts->verbose : 0;
+  h->bc = *bc;
+  if (bc->dirichlet && bc->inimod != 1)
+    return set_err(h, MVF_E_UNSUPPORTED, "Dirichlet boundary model inimod=%d is not implemented (only 1)", bc->inimod);
+  MVF_HIP(h, hipSetDevice(h->device));
+  // a blocking stream: synchronous runtime calls on the null stream stay ordered after its work
+  MVF_HIP(h, hipStreamCreate(&h->stream));
+  h->nodes = grid->nodes_per_elem;
+  h->et = elem_type(h->nodes);
+  Geom &g = h->geo;
+  g.nx = grid->nx; g.ny = grid->ny; g.nz = grid->nz;
+  g.nord = h->et.nord;
+  int o = g.nord - 1;
+  g.nnx = g.nx * o + 1; g.nny = g.ny * o + 1; g.nnz = g.nz * o + 1;
+  g.nyz = (int64_t)g.nny * g.nnz;
+  g.npt = (int64_t)g.nnx * g.nyz;
+  g.ne = (int64_t)g.nx * g.ny * g.nz;
+  if ((int64_t)h->et.ne * h->et.ne * g.ne / h->et.ne > INT64_MAX / 64) return set_err(h, MVF_E_ARG, "grid too large");
+  // GPML zone state (init_gpml, boundary_conds.f90:51-70)
+  Gpml &gp = h->gp;
+  gp.on = bc->dirichlet ? 0 : 1;
+  gp.sch = bc->gpml_sch;
+  gp.a0 = bc->a0;
+  gp.nn = bc->nn;
+  int nel[3] = {g.nx, g.ny, g.nz};
+  for (int c = 0; c < 3; c++) {
+    gp.el_a[c][0] = bc->nextd;
+    gp.el_a[c][1] = nel[c] + 1 - bc->nextd;
+    gp.el_b[c][0] = 1;
+    gp.el_b[c][1] = nel[c];
+  }
+  int nd = bc->nextd * o;
+  if (gp.on && (bc->nextd < 1 || nd + 1 > g.nnx || nd + 1 > g.nny || nd + 1 > g.nnz))
+    return set_err(h, MVF_E_ARG, "nextd=%d does not fit the grid", bc->nextd);
+  if (gp.on) {
+    gp.a[0][0] = grid->xp[nd]; gp.a[0][1] = grid->xp[g.nnx - nd - 1];
+    gp.b[0][0] = grid->xp[0]; gp.b[0][1] = grid->xp[g.nnx - 1];
+    gp.a[1][0] = grid->yp[nd]; gp.a[1][1] = grid->yp[g.nny - nd - 1];
+    gp.b[1][0] = grid->yp[0]; gp.b[1][1] = grid->yp[g.nny - 1];
+    int64_t id2 = (int64_t)(g.nnx - 1) * g.nyz + (int64_t)(g.nny - 1) * g.nnz + g.nnz - (int64_t)bc->nzl_top * o - 1;
+    if (id2 < 0 || id2 >= g.npt) return set_err(h, MVF_E_ARG, "nzl_top=%d does not fit the grid", bc->nzl_top);
+    gp.a[2][0] = grid->zp[nd]; gp.a[2][1] = grid->zp[id2];
+    gp.b[2][0] = grid->zp[0]; gp.b[2][1] = grid->zp[g.npt - 1];
+    h->flags_first = 0;
+    h->flags_late = zone_flags(gp, g.nx, g.ny, g.nz);
+  }
+  double t0 = now();
+  // lattice and materials
+  MVF_TRY(dalloc(h, &h->d_xp, g.nnx));
+  MVF_TRY(dalloc(h, &h->d_yp, g.nny));
+  MVF_TRY(dalloc(h, &h->d_zp, g.npt));
+  MVF_TRY(dalloc(h, &h->d_sig, 12 * g.npt));
+  MVF_TRY(dalloc(h, &h->d_inmu, 6 * g.npt));
+  MVF_TRY(dalloc(h, &h->d_dmu, 6 * g.npt));
+  MVF_HIP(h, hipMemcpy(h->d_xp, grid->xp, 8 * g.nnx, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(h->d_yp, grid->yp, 8 * g.nny, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(h->d_zp, grid->zp, 8 * g.npt, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(h->d_sig, model->sigma, 96 * g.npt, hipMemcpyHostToDevice));
+  {
+    // per-node inverse permeability and contrast (pinv_geomodel, pdelta_model,
+    // problem.f90:257-335); a singular tensor stops the reference (:260-271)
+    std::vector<double> inmu(6 * g.npt), dmu(6 * g.npt);
+    const double mu0 = 4 * MVF_PI * 1.e-7;
+    for (int64_t n = 0; n < g.npt; n++) {
+      const double *m = model->mu + 6 * n;
+      double d;
+      inv6(m, &inmu[6 * n], d);
+      if (d == 0.0) return set_err(h, MVF_E_SINGULAR, "singular mu tensor at node %lld", (long long)n + 1);
+      const double *s = model->sigma + 12 * n;
+      // complex determinant of the sigma tensor
+      auto cm = [](double ar, double ai, double br, double bi, double &r, double &i) {
+        r = ar * br - ai * bi;
+        i = ar * bi + ai * br;
+      };
+      double p1r, p1i, p2r, p2i, tr, ti, sr = 0, si = 0;
+      // a1*(a4*a6 - a5*a5)
+      cm(s[6], s[7], s[10], s[11], p1r, p1i); cm(s[8], s[9], s[8], s[9], p2r, p2i);
+      cm(s[0], s[1], p1r - p2r, p1i - p2i, tr, ti); sr += tr; si += ti;
+      // a2*(a3*a5 - a2*a6)
+      cm(s[4], s[5], s[8], s[9], p1r, p1i); cm(s[2], s[3], s[10], s[11], p2r, p2i);
+      cm(s[2], s[3], p1r - p2r, p1i - p2i, tr, ti); sr += tr; si += ti;
+      // a3*(a2*a5 - a4*a3)
+      cm(s[2], s[3], s[8], s[9], p1r, p1i); cm(s[6], s[7], s[4], s[5], p2r, p2i);
+      cm(s[4], s[5], p1r - p2r, p1i - p2i, tr, ti); sr += tr; si += ti;
+      if (sr == 0.0 && si == 0.0)
+        return set_err(h, MVF_E_SINGULAR, "singular sigma tensor at node %lld", (long long)n + 1);
+      for (int c = 0; c < 6; c++) dmu[6 * n + c] = m[c];
+      dmu[6 * n + 0] -= mu0; dmu[6 * n + 3] -= mu0; dmu[6 * n + 5] -= mu0;
+    }
+    MVF_HIP(h, hipMemcpy(h->d_inmu, inmu.data(), 48 * g.npt, hipMemcpyHostToDevice));
+    MVF_HIP(h, hipMemcpy(h->d_dmu, dmu.data(), 48 * g.npt, hipMemcpyHostToDevice));
+  }
+  {
+    BasisTab *tab = new BasisTab();
+    memset(tab, 0, sizeof(BasisTab));
+    fill_basis(h->nodes, tab);
+    MVF_TRY(dalloc(h, &h->d_tab, 1));
+    MVF_HIP(h, hipMemcpy(h->d_tab, tab, sizeof(BasisTab), hipMemcpyHostToDevice));
+    delete tab;
+  }
+  double t1 = now();
+  MVF_TRY(setup_numbering(h));
+  double t2 = now();
+  MVF_TRY(setup_pattern(h));
+  double t3 = now();
+  int64_t np = (int64_t)h->et.ne * (h->et.ne + 1) / 2;
+  MVF_TRY(dalloc(h, &h->d_Ke, np * g.ne));
+  MVF_TRY(dalloc(h, &h->d_Me, np * g.ne));
+  int ff = first_flags_for(h, 0);
+  MVF_TRY(integrate_elements(h, 0, g.ne, ff, h->d_Ke, h->d_Me));
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  double t4 = now();
+  MVF_TRY(assemble_operators(h));
+  // element 0 under the carried flags (see MVF_EMU_GPML_PREV_FLAGS)
+  if (gp.on && (h->emulate & MVF_EMU_GPML_PREV_FLAGS) && h->flags_late != h->flags_first) {
+    MVF_TRY(dalloc(h, &h->d_dK0, np));
+    MVF_TRY(dalloc(h, &h->d_dM0, np));
+    MVF_TRY(integrate_elements(h, 0, 1, h->flags_late, h->d_dK0, h->d_dM0));
+    std::vector<double> a(np), b(np), c(np), d(np);
+    MVF_HIP(h, hipMemcpy(a.data(), h->d_dK0, 8 * np, hipMemcpyDeviceToHost));
+    MVF_HIP(h, hipMemcpy(b.data(), h->d_dM0, 8 * np, hipMemcpyDeviceToHost));
+    MVF_HIP(h, hipMemcpy(c.data(), h->d_Ke, 8 * np, hipMemcpyDeviceToHost));
+    MVF_HIP(h, hipMemcpy(d.data(), h->d_Me, 8 * np, hipMemcpyDeviceToHost));
+    bool nz = false;
+    for (int64_t p = 0; p < np; p++) {
+      a[p] -= c[p];
+      b[p] -= d[p];
+      nz |= (a[p] != 0.0 || b[p] != 0.0);
+    }
+    MVF_HIP(h, hipMemcpy(h->d_dK0, a.data(), 8 * np, hipMemcpyHostToDevice));
+    MVF_HIP(h, hipMemcpy(h->d_dM0, b.data(), 8 * np, hipMemcpyHostToDevice));
+    h->carry_nonzero = nz ? 1 : 0;
+  }
+  h->carry_applied = 0;
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  double t5 = now();
+  dfree(h->d_Ke); h->d_Ke = nullptr;
+  dfree(h->d_Me); h->d_Me = nullptr;
+  h->t_setup[0] = t2 - t1;
+  h->t_setup[1] = t3 - t2;
+  h->t_setup[2] = t4 - t3;
+  h->t_setup[3] = t5 - t4;
+  (void)t0;
+  return 0;
+}
+
+int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms) {
+  if (!h || (precision != 0 && precision != 1)) return h ? set_err(h, MVF_E_ARG, "precision must be 0 or 1") : MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  const int64_t ne = h->geo.ne, np = (int64_t)h->et.ne * (h->et.ne + 1) / 2;
+  // element 0 takes the flags of the current call state directly
+  const int ff = first_flags_for(h, h->calls);
+  hipEvent_t ev[2];
+  MVF_HIP(h, hipEventCreate(&ev[0]));
+  MVF_HIP(h, hipEventCreate(&ev[1]));
+  int rc = 0;
+  if (precision == 0) {
+    MVF_TRY(dalloc(h, &h->d_Ke, np * ne));
+    MVF_TRY(dalloc(h, &h->d_Me, np * ne));
+    MVF_HIP(h, hipEventRecord(ev[0], h->stream));
+    rc = integrate_elements(h, 0, ne, ff, h->d_Ke, h->d_Me);
+    (void)hipEventRecord(ev[1], h->stream);
+    if (!rc) rc = assemble_operators(h);
+    dfree(h->d_Ke); dfree(h->d_Me);
+    h->d_Ke = h->d_Me = nullptr;
+  } else {
+    float *k32, *m32;
+    MVF_TRY(dalloc(h, &k32, np * ne));
+    MVF_TRY(dalloc(h, &m32, np * ne));
+    MVF_HIP(h, hipEventRecord(ev[0], h->stream));
+    rc = integrate_elements32(h, 0, ne, ff, k32, m32);
+    (void)hipEventRecord(ev[1], h->stream);
+    if (!rc) rc = assemble_operators32(h, k32, m32);
+    dfree(k32); dfree(m32);
+  }
+  if (!rc) {
+    MVF_HIP(h, hipStreamSynchronize(h->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (integrate_ms) *integrate_ms = ms;
+    h->carry_applied = (h->carry_nonzero && h->calls >= 1) ? 1 : 0;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return rc;
+}
+
+int mvf_sizes(const mvf_handle *h, int32_t *nne, int64_t *nnze, int64_t *nnz_triu) {
+  if (!h) return MVF_E_ARG;
+  if (nne) *nne = h->nne;
+  if (nnze) *nnze = h->nnz;
+  if (nnz_triu) *nnz_triu = h->nnz_triu;
+  return 0;
+}
+
+int mvf_setup_times(const mvf_handle *h, double *t4) {
+  if (!h || !t4) return MVF_E_ARG;
+  for (int i = 0; i < 4; i++) t4[i] = h->t_setup[i];
+  return 0;
+}
+
+int mvf_get_gne(mvf_handle *h, int32_t *gne_out) {
+  if (!h || !gne_out) return MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  MVF_HIP(h, hipMemcpy(gne_out, h->d_gne, sizeof(int32_t) * h->geo.ne * h->et.ne, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mvf_set_frequency(mvf_handle *h, double freq_hz) {
+  if (!h || !(freq_hz > 0)) return MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  h->freq = freq_hz;
+  h->omega = 2.0 * MVF_PI * freq_hz;
+  h->have_freq = 1;
+  MVF_TRY(update_sigma(h, h->omega));
+  return 0;
+}
+
+int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me, double *be) {
+  if (!h || e0 < 0 || e1 > h->geo.ne || e1 < e0) return MVF_E_ARG;
+  if (be && !h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede element RHS");
+  MVF_HIP(h, hipSetDevice(h->device));
+  int64_t n = e1 - e0;
+  if (n == 0) return 0;
+  int64_t np = (int64_t)h->et.ne * (h->et.ne + 1) / 2;
+  int ff = first_flags_for(h, h->calls);
+  if (Ke || Me) {
+    double *dK, *dM;
+    MVF_TRY(dalloc(h, &dK, np * n));
+    MVF_TRY(dalloc(h, &dM, np * n));
+    int rc = integrate_elements(h, e0, e1, ff, dK, dM);
+    if (!rc && Ke)
+      rc = hipMemcpyAsync(Ke, dK, 8 * np * n, hipMemcpyDeviceToHost, h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
+    if (!rc && Me)
+      rc = hipMemcpyAsync(Me, dM, 8 * np * n, hipMemcpyDeviceToHost, h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
+    if (!rc) rc = hipStreamSynchronize(h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
+    dfree(dK);
+    dfree(dM);
+    if (rc) return rc;
+  }
+  if (be) {
+    double *d;
+    MVF_TRY(dalloc(h, &d, (size_t)n * h->et.ne * 4));
+    int rc = element_rhs_range(h, h->omega, ff, e0, e1, d);
+    if (!rc)
+      rc = hipMemcpyAsync(be, d, 8 * (size_t)n * h->et.ne * 4, hipMemcpyDeviceToHost, h->stream) == hipSuccess
+               ? 0
+               : MVF_E_DEVICE;
+    if (!rc) rc = hipStreamSynchronize(h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
+    dfree(d);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// RHS of the current frequency into column pair col0 of an [nne][ncol] device array
+static int build_rhs(mvf_handle *h, double omega, int64_t call, double *d_b, int ncol, int col0) {
+  double *be;
+  MVF_TRY(dalloc(h, &be, (size_t)h->geo.ne * h->et.ne * 4));
+  int rc = element_rhs_range(h, omega, first_flags_for(h, call), 0, h->geo.ne, be);
+  if (!rc) rc = assemble_rhs(h, be, d_b, ncol, col0);
+  dfree(be);
+  return rc;
+}
+
+int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze, int32_t nne,
+                    int32_t ndir) {
+  if (!h || !ia || !ja || !a || !b) return MVF_E_ARG;
+  if (nnze != h->nnz || nne != h->nne || ndir != 2)
+    return set_err(h, MVF_E_ARG, "size mismatch: nnze %lld/%lld nne %d/%d ndir %d", (long long)nnze,
+                   (long long)h->nnz, nne, h->nne, ndir);
+  if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede mvf_global_vfem");
+  MVF_HIP(h, hipSetDevice(h->device));
+  MVF_TRY(ensure_call_state(h, h->calls));
+  double *db;
+  MVF_TRY(dalloc(h, &db, (size_t)nne * 4));
+  MVF_TRY(build_rhs(h, h->omega, h->calls, db, 2, 0));
+  std::vector<int64_t> rp(nne + 1);
+  std::vector<int32_t> col(h->nnz);
+  std::vector<double> K(h->nnz), M(h->nnz), bb((size_t)nne * 4);
+  MVF_HIP(h, hipMemcpyAsync(rp.data(), h->d_rowptr, 8 * (nne + 1), hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(col.data(), h->d_col, 4 * h->nnz, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(K.data(), h->d_K, 8 * h->nnz, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(M.data(), h->d_M, 8 * h->nnz, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(bb.data(), db, 32 * (size_t)nne, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  dfree(db);
+  const double w32 = (double)(float)h->omega;
+  for (int32_t r = 0; r < nne; r++)
+    for (int64_t p = rp[r]; p < rp[r + 1]; p++) {
+      ia[p] = r + 1;
+      ja[p] = col[p] + 1;
+      // ga_sort_sparse leaves the lower-slot value of (col,row) at the
+      // row-major position of (row,col): upper entries carry the symmetric
+      // value, lower ones the never-assigned zero; all rounded to complex64
+      if (col[p] >= r) {
+        a[2 * p] = (double)(float)K[p];
+        a[2 * p + 1] = (double)(float)(w32 * M[p]);
+      } else {
+        a[2 * p] = 0.0;
+        a[2 * p + 1] = 0.0;
+      }
+    }
+  for (int32_t r = 0; r < nne; r++)
+    for (int d = 0; d < 2; d++) {
+      b[2 * ((size_t)d * nne + r)] = bb[4 * (size_t)r + 2 * d];
+      b[2 * ((size_t)d * nne + r) + 1] = bb[4 * (size_t)r + 2 * d + 1];
+    }
+  h->calls++;
+  return 0;
+}
+
+int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_out) {
+  if (n < 0 || (n > 0 && (!ia || !ja || !a)) || !n_out) return MVF_E_ARG;
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (a[2 * i] == 0.0 && a[2 * i + 1] == 0.0) continue;
+    ia[k] = ia[i];
+    ja[k] = ja[i];
+    a[2 * k] = a[2 * i];
+    a[2 * k + 1] = a[2 * i + 1];
+    k++;
+  }
+  *n_out = k;
+  return 0;
+}
+
+static int pad_cols(int n) { return n <= 2 ? 2 : (n <= 4 ? 4 : (n <= 8 ? 8 : (n <= 16 ? 16 : -1))); }
+
+int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_solve_info *info) {
+  if (!h || !rhs_inout) return MVF_E_ARG;
+  if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede mvf_solve");
+  mvf_solver_opts od;
+  mvf_default_solver_opts(&od);
+  if (!o) o = &od;
+  mvf_solve_info dummy;
+  if (!info) info = &dummy;
+  memset(info, 0, sizeof(*info));
+  MVF_HIP(h, hipSetDevice(h->device));
+  int32_t nne = h->nne;
+  std::vector<double> hb((size_t)nne * 4);
+  for (int32_t r = 0; r < nne; r++)
+    for (int d = 0; d < 2; d++) {
+      hb[4 * (size_t)r + 2 * d] = rhs_inout[2 * ((size_t)d * nne + r)];
+      hb[4 * (size_t)r + 2 * d + 1] = rhs_inout[2 * ((size_t)d * nne + r) + 1];
+    }
+  double *db, *dx;
+  MVF_TRY(dalloc(h, &db, (size_t)nne * 4));
+  MVF_TRY(dalloc(h, &dx, (size_t)nne * 4));
+  MVF_HIP(h, hipMemcpyAsync(db, hb.data(), 32 * (size_t)nne, hipMemcpyHostToDevice, h->stream));
+  MVF_HIP(h, hipMemsetAsync(dx, 0, 32 * (size_t)nne, h->stream));
+  double om[2] = {h->omega, h->omega};
+  double t0 = now();
+  int rc = solve_device(h, om, 2, 2, o, db, dx, info);
+  (void)hipStreamSynchronize(h->stream);
+  info->seconds = now() - t0;
+  if (hipMemcpy(hb.data(), dx, 32 * (size_t)nne, hipMemcpyDeviceToHost) != hipSuccess && !rc) rc = MVF_E_DEVICE;
+  dfree(db);
+  dfree(dx);
+  for (int32_t r = 0; r < nne; r++)
+    for (int d = 0; d < 2; d++) {
+      rhs_inout[2 * ((size_t)d * nne + r)] = hb[4 * (size_t)r + 2 * d];
+      rhs_inout[2 * ((size_t)d * nne + r) + 1] = hb[4 * (size_t)r + 2 * d + 1];
+    }
+  return rc;
+}
+
+int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, const mvf_solver_opts *o,
+                          mvf_solve_info *info) {
+  if (!h || !freqs_hz || nfreq < 1 || nfreq > 8) return h ? set_err(h, MVF_E_ARG, "nfreq must be 1..8") : MVF_E_ARG;
+  mvf_solver_opts od;
+  mvf_default_solver_opts(&od);
+  if (!o) o = &od;
+  mvf_solve_info dummy;
+  if (!info) info = &dummy;
+  memset(info, 0, sizeof(*info));
+  MVF_HIP(h, hipSetDevice(h->device));
+  int32_t nne = h->nne;
+  int ncol = 2 * nfreq;
+  size_t need = (size_t)nne * ncol * 2;
+  if (h->xb_cap < need) {
+    dfree(h->d_x);
+    dfree(h->d_b);
+    h->d_x = h->d_b = nullptr;
+    MVF_TRY(dalloc(h, &h->d_x, need));
+    MVF_TRY(dalloc(h, &h->d_b, need));
+    h->xb_cap = need;
+  }
+  h->ncols_last = ncol;
+  double t0 = now();
+  // batches: the first assembly call (if still pending and the carried GPML
+  // flags change element 0) is solved alone, the rest together
+  int f = 0;
+  int rc = 0;
+  while (f < nfreq && !rc) {
+    int nb = nfreq - f;
+    if (h->carry_nonzero && h->calls == 0) nb = 1;
+    MVF_TRY(ensure_call_state(h, h->calls));
+    int pc = pad_cols(2 * nb);
+    double *bb, *xb;
+    MVF_TRY(dalloc(h, &bb, (size_t)nne * pc * 2));
+    MVF_TRY(dalloc(h, &xb, (size_t)nne * pc * 2));
+    MVF_HIP(h, hipMemsetAsync(bb, 0, 16 * (size_t)nne * pc, h->stream));
+    MVF_HIP(h, hipMemsetAsync(xb, 0, 16 * (size_t)nne * pc, h->stream));
+    double om[16] = {0};
+    for (int q = 0; q < nb; q++) {
+      MVF_TRY(mvf_set_frequency(h, freqs_hz[f + q]));
+      MVF_TRY(build_rhs(h, h->omega, h->calls + q, bb, pc, 2 * q));
+      om[2 * q] = om[2 * q + 1] = h->omega;
+    }
+    mvf_solve_info bi;
+    memset(&bi, 0, sizeof(bi));
+    rc = solve_device(h, om, 2 * nb, pc, o, bb, xb, &bi);
+    info->iterations += bi.iterations;
+    info->refinements += bi.refinements;
+    info->fallbacks += bi.fallbacks;
+    for (int c = 0; c < 2 * nb && 2 * f + c < 64; c++) info->true_resid[2 * f + c] = bi.true_resid[c];
+    MVF_HIP(h, hipMemcpy2DAsync(h->d_x + 2 * (size_t)(2 * f), 16 * (size_t)ncol, xb, 16 * (size_t)pc, 16 * (size_t)(2 * nb),
+                                nne, hipMemcpyDeviceToDevice, h->stream));
+    MVF_HIP(h, hipMemcpy2DAsync(h->d_b + 2 * (size_t)(2 * f), 16 * (size_t)ncol, bb, 16 * (size_t)pc, 16 * (size_t)(2 * nb),
+                                nne, hipMemcpyDeviceToDevice, h->stream));
+    MVF_HIP(h, hipStreamSynchronize(h->stream));
+    dfree(bb);
+    dfree(xb);
+    h->calls += nb;
+    f += nb;
+  }
+  info->converged = rc == 0;
+  info->seconds = now() - t0;
+  return rc;
+}
+
+static int fetch_cols(mvf_handle *h, const double *d, double *out, int32_t ncols) {
+  if (!h || !out || !d || ncols != h->ncols_last) return h ? set_err(h, MVF_E_ARG, "column count mismatch") : MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  int32_t nne = h->nne;
+  std::vector<double> t((size_t)nne * ncols * 2);
+  MVF_HIP(h, hipMemcpy(t.data(), d, t.size() * 8, hipMemcpyDeviceToHost));
+  for (int32_t r = 0; r < nne; r++)
+    for (int c = 0; c < ncols; c++) {
+      out[2 * ((size_t)c * nne + r)] = t[2 * ((size_t)r * ncols + c)];
+      out[2 * ((size_t)c * nne + r) + 1] = t[2 * ((size_t)r * ncols + c) + 1];
+    }
+  return 0;
+}
+
+int mvf_get_solution(mvf_handle *h, double *x_out, int32_t ncols) { return fetch_cols(h, h ? h->d_x : nullptr, x_out, ncols); }
+int mvf_get_rhs(mvf_handle *h, double *b_out, int32_t ncols) { return fetch_cols(h, h ? h->d_b : nullptr, b_out, ncols); }
+
+}  // extern "C"

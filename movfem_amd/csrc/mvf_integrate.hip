@@ -1,0 +1,448 @@
+// Element integration on the device.
+//
+// k_elem_km: frequency-independent element matrices.  The reference forms
+//   alocal(i,j) = sum_g w_g (f1 + i fl32(omega) f2)        (integration.f90:76-86)
+// with f1 the real part of the curl-curl integrand and f2 the real part of the
+// mass integrand (:154-238).  Both parts are real and frequency independent,
+// so one pass yields K_ij = sum_g w_g f1 and M_ij = sum_g w_g f2 and
+// alocal = K + i fl32(omega) M for every frequency.
+//
+// Under GPML the reference weights each of the 36 curl products by a product
+// of the (real) stretch factors h (:171-188).  That weight table factors as
+// W = g g^T with g(a,1), g(a,2) = sqrt of h1h3/h2, h1h2/h3, h2h3/h1 cyclically,
+// so f1 = chat_i^T m chat_j with chat(a) = g(a,1) curl1(a) - g(a,2) curl2(a);
+// f2 gains the factor h1 h2 h3 (:228-232).
+//
+// One workgroup per element: node data, Jacobians and interpolated tensors at
+// the Gauss points are computed once (the reference recomputes the Jacobian
+// for every basis call, v_fem.f90:475-476,516-517), the per-(point, DOF)
+// vectors chat, ve, m1*chat, m2*ve are staged in LDS for a chunk of points, and
+// each thread accumulates its upper-packed (i <= j) pairs over the points.
+//
+// k_elem_rhs: the per-frequency element RHS blocal (integration.f90:88-106)
+// with the secondary-field source of p_source (problem.f90:91-128), the air
+// primary field of p_pfields (:340-358) and pset_pmodel's float32 constants.
+#include "mvf_internal.h"
+
+namespace mvf {
+
+__device__ inline void elem_index(const Geom &g, int64_t e, int &i, int &j, int &k) {
+  i = (int)(e / ((int64_t)g.ny * g.nz)) + 1;
+  j = (int)((e / g.nz) % g.ny) + 1;
+  k = (int)(e % g.nz) + 1;
+}
+
+// GPML flags used for element e (MVF_EMU_GPML_PREV_FLAGS reproduces that the
+// reference's int_elem_params reads the flags get_pml left for the previous
+// element, MoVFEM_3DMT.f90:203-206)
+__device__ inline int elem_flags(const Geom &g, const Gpml &gp, int prev, int first_flags, int64_t e) {
+  if (!gp.on) return 0;
+  int i, j, k;
+  if (!prev) {
+    elem_index(g, e, i, j, k);
+    return zone_flags(gp, i, j, k);
+  }
+  if (e == 0) return first_flags;
+  elem_index(g, e - 1, i, j, k);
+  return zone_flags(gp, i, j, k);
+}
+
+__device__ inline void load_node(const BasisTab *tab, const Geom &g, int i, int j, int k, int n, int64_t &id,
+                                 int &ix, int &jy) {
+  int o = g.nord - 1;
+  int i1 = (tab->nref[n][0] + 1) * o / 2, j1 = (tab->nref[n][1] + 1) * o / 2, k1 = (tab->nref[n][2] + 1) * o / 2;
+  int64_t eno = (int64_t)(i - 1) * g.nyz * o + (int64_t)(j - 1) * g.nnz * o + (int64_t)(k - 1) * o;
+  id = eno + (int64_t)i1 * g.nyz + (int64_t)j1 * g.nnz + k1;
+  ix = (i - 1) * o + i1;
+  jy = (j - 1) * o + j1;
+}
+
+// Jacobian at point g from node coordinates; returns det, writes J^-1 / |det|
+template <int NN>
+__device__ inline double jacobian(const BasisTab *tab, int gpt, const double (*xs)[3], double *Ji) {
+  double J[3][3];
+  for (int m = 0; m < 3; m++)
+    for (int n = 0; n < 3; n++) {
+      double acc = 0.0;
+      for (int l = 0; l < NN; l++) acc += tab->dN[gpt][m][l] * xs[l][n];
+      J[m][n] = acc;
+    }
+  double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) + J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  double ad = fabs(det);
+  Ji[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / ad;
+  Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / ad;
+  Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / ad;
+  Ji[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / ad;
+  Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / ad;
+  Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / ad;
+  Ji[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / ad;
+  Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / ad;
+  Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / ad;
+  return det;
+}
+
+__device__ inline void sym6_mul(const double *m, const double *v, double *o) {
+  o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  o[1] = m[1] * v[0] + m[3] * v[1] + m[4] * v[2];
+  o[2] = m[2] * v[0] + m[4] * v[1] + m[5] * v[2];
+}
+
+// jacobian in compute type T (J^-1 / |det|); returns det
+template <typename T, int NN>
+__device__ inline T jacobian_t(const BasisTab *tab, int gpt, const T (*xs)[3], T *Ji) {
+  T J[3][3];
+  for (int m = 0; m < 3; m++)
+    for (int n = 0; n < 3; n++) {
+      T acc = 0;
+      for (int l = 0; l < NN; l++) acc += (T)tab->dN[gpt][m][l] * xs[l][n];
+      J[m][n] = acc;
+    }
+  T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) + J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  T ad = det < 0 ? -det : det;
+  Ji[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / ad;
+  Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / ad;
+  Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / ad;
+  Ji[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / ad;
+  Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / ad;
+  Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / ad;
+  Ji[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / ad;
+  Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / ad;
+  Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / ad;
+  return det;
+}
+
+template <typename T>
+__device__ inline void sym6_mul_t(const T *m, const T *v, T *o) {
+  o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  o[1] = m[1] * v[0] + m[3] * v[1] + m[4] * v[2];
+  o[2] = m[2] * v[0] + m[4] * v[1] + m[5] * v[2];
+}
+
+// T = double: the FP64 path; T = float: FP32 element integration (the GPML
+// stretch, its square-root weights and the element position stay FP64)
+template <typename T, int NN, int NE, int NG, int GC>
+__global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ tab, const double *__restrict__ xp,
+                                                 const double *__restrict__ yp, const double *__restrict__ zp,
+                                                 const double *__restrict__ inmu, const double *__restrict__ sig,
+                                                 Geom g, Gpml gp, int prev, int first_flags, int64_t e0,
+                                                 T *__restrict__ Kout, T *__restrict__ Mout, int *bad) {
+  constexpr int NP = NE * (NE + 1) / 2;
+  constexpr int PPT = (NP + 255) / 256;
+  const int64_t e = e0 + blockIdx.x;
+  const int t = threadIdx.x;
+  __shared__ double xs64[NN][3];
+  __shared__ T xs[NN][3];
+  __shared__ T nm[NN][12];
+  __shared__ T sJi[NG][9];
+  __shared__ T sm1[NG][6], sm2[NG][6], sgf[NG][6];
+  __shared__ T sv[GC][NE][12];
+  int i, j, k;
+  elem_index(g, e, i, j, k);
+  if (t < NN) {
+    int64_t id;
+    int ix, jy;
+    load_node(tab, g, i, j, k, t, id, ix, jy);
+    xs64[t][0] = xp[ix];
+    xs64[t][1] = yp[jy];
+    xs64[t][2] = zp[id];
+    for (int c = 0; c < 3; c++) xs[t][c] = (T)xs64[t][c];
+    for (int c = 0; c < 6; c++) {
+      nm[t][c] = (T)inmu[id * 6 + c];
+      nm[t][6 + c] = (T)sig[2 * (id * 6 + c)];
+    }
+  }
+  __syncthreads();
+  if (t < NG) {
+    T Ji[9];
+    T det = jacobian_t<T, NN>(tab, t, xs, Ji);
+    if (det == (T)0) atomicExch(bad, 1);
+    T wgt = det * (T)tab->w[t];
+    T m1[6], m2[6];
+    for (int c = 0; c < 6; c++) {
+      T a1 = 0, a2 = 0;
+      for (int n = 0; n < NN; n++) {
+        a1 += (T)tab->N[t][n] * nm[n][c];
+        a2 += (T)tab->N[t][n] * nm[n][6 + c];
+      }
+      m1[c] = a1;
+      m2[c] = a2;
+    }
+    double h[3] = {1.0, 1.0, 1.0};
+    if (gp.on) {
+      int fl = elem_flags(g, gp, prev, first_flags, e);
+      double r[3] = {0, 0, 0};
+      for (int n = 0; n < NN; n++)
+        for (int c = 0; c < 3; c++) r[c] += tab->N[t][n] * xs64[n][c];
+      for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, r[c]);
+    }
+    double h3 = h[0] * h[1] * h[2];
+    double q0 = sqrt(h[0] * h[2] / h[1]), q1 = sqrt(h[0] * h[1] / h[2]), q2 = sqrt(h[1] * h[2] / h[0]);
+    sgf[t][0] = (T)q0; sgf[t][1] = (T)q1;
+    sgf[t][2] = (T)q1; sgf[t][3] = (T)q2;
+    sgf[t][4] = (T)q2; sgf[t][5] = (T)q0;
+    for (int c = 0; c < 6; c++) {
+      sm1[t][c] = wgt * m1[c];
+      sm2[t][c] = (T)((double)wgt * h3) * m2[c];
+    }
+    for (int c = 0; c < 9; c++) sJi[t][c] = Ji[c];
+  }
+  __syncthreads();
+  T kacc[PPT], macc[PPT];
+  int pa[PPT], pb[PPT];
+#pragma unroll
+  for (int s = 0; s < PPT; s++) {
+    kacc[s] = macc[s] = 0;
+    int p = t + s * 256;
+    pa[s] = p < NP ? tab->pi[p] : 0;
+    pb[s] = p < NP ? tab->pj[p] : 0;
+  }
+  for (int g0 = 0; g0 < NG; g0 += GC) {
+    const int gcount = (NG - g0) < GC ? (NG - g0) : GC;
+    for (int q = t; q < gcount * NE; q += 256) {
+      int gl = q / NE, ed = q - gl * NE, gg = g0 + gl;
+      const T *Ji = sJi[gg];
+      int d = tab->dir[ed];
+      T v[3] = {Ji[0 * 3 + d], Ji[1 * 3 + d], Ji[2 * 3 + d]};
+      T dp0 = (T)tab->dphi[gg][ed][0], dp1 = (T)tab->dphi[gg][ed][1], dp2 = (T)tab->dphi[gg][ed][2];
+      T dn[3];
+      for (int a = 0; a < 3; a++) dn[a] = Ji[a * 3 + 0] * dp0 + Ji[a * 3 + 1] * dp1 + Ji[a * 3 + 2] * dp2;
+      const T *gf = sgf[gg];
+      T ch[3] = {gf[0] * (dn[1] * v[2]) - gf[1] * (dn[2] * v[1]), gf[2] * (dn[2] * v[0]) - gf[3] * (dn[0] * v[2]),
+                 gf[4] * (dn[0] * v[1]) - gf[5] * (dn[1] * v[0])};
+      T ph = (T)tab->phi[gg][ed];
+      T ve[3] = {ph * v[0], ph * v[1], ph * v[2]};
+      T tk[3], tm[3];
+      sym6_mul_t<T>(sm1[gg], ch, tk);
+      sym6_mul_t<T>(sm2[gg], ve, tm);
+      T *o = sv[gl][ed];
+      o[0] = ch[0]; o[1] = ch[1]; o[2] = ch[2];
+      o[3] = ve[0]; o[4] = ve[1]; o[5] = ve[2];
+      o[6] = tk[0]; o[7] = tk[1]; o[8] = tk[2];
+      o[9] = tm[0]; o[10] = tm[1]; o[11] = tm[2];
+    }
+    __syncthreads();
+    for (int gl = 0; gl < gcount; gl++) {
+#pragma unroll
+      for (int s = 0; s < PPT; s++) {
+        int p = t + s * 256;
+        if (p < NP) {
+          const T *A = sv[gl][pa[s]];
+          const T *B = sv[gl][pb[s]];
+          kacc[s] += A[0] * B[6] + A[1] * B[7] + A[2] * B[8];
+          macc[s] += A[3] * B[9] + A[4] * B[10] + A[5] * B[11];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t base = (e - e0) * (int64_t)NP;
+#pragma unroll
+  for (int s = 0; s < PPT; s++) {
+    int p = t + s * 256;
+    if (p < NP) {
+      Kout[base + p] = kacc[s];
+      Mout[base + p] = macc[s];
+    }
+  }
+}
+
+// per-frequency element RHS: be[e][im][d] complex, d = polarization
+template <int NN, int NE, int NG>
+__global__ __launch_bounds__(64) void k_elem_rhs(const BasisTab *__restrict__ tab, const double *__restrict__ xp,
+                                                 const double *__restrict__ yp, const double *__restrict__ zp,
+                                                 const double *__restrict__ sig, const double *__restrict__ inmu,
+                                                 const double *__restrict__ dmu, Geom g, Gpml gp, int prev,
+                                                 int first_flags, double omega, int64_t e0,
+                                                 double *__restrict__ be) {
+  const int64_t e = e0 + blockIdx.x;
+  const int t = threadIdx.x;
+  __shared__ double xs[NN][3];
+  __shared__ double zz[NN];                 // float32(omega*b0*z)
+  __shared__ double ds[NN][12];             // complex delta sigma (6)
+  __shared__ double mh[NN][2][3];           // (mu^-1 dmu) Hp per polarization (real)
+  __shared__ double sJi[NG][9];
+  __shared__ double swb[NG][2][6];          // w h1h2h3 src, complex 3 per polarization
+  int i, j, k;
+  elem_index(g, e, i, j, k);
+  const double w32 = (double)(float)omega;
+  const double psig = (double)(float)(omega * MVF_EPS0);
+  const double hh = (double)(float)MVF_B0 / (4 * MVF_PI * 1.e-7);
+  if (t < NN) {
+    int64_t id;
+    int ix, jy;
+    load_node(tab, g, i, j, k, t, id, ix, jy);
+    xs[t][0] = xp[ix];
+    xs[t][1] = yp[jy];
+    xs[t][2] = zp[id];
+    zz[t] = (double)(float)(omega * MVF_B0 * zp[id]);
+    for (int c = 0; c < 6; c++) {
+      ds[t][2 * c] = sig[2 * (id * 6 + c)];
+      ds[t][2 * c + 1] = sig[2 * (id * 6 + c) + 1];
+    }
+    ds[t][1] -= psig; ds[t][7] -= psig; ds[t][11] -= psig;
+    const double *im = inmu + id * 6, *dm = dmu + id * 6;
+    double M[3][3] = {{im[0] * dm[0] + im[1] * dm[1] + im[2] * dm[2], im[0] * dm[1] + im[1] * dm[3] + im[2] * dm[4],
+                       im[0] * dm[2] + im[1] * dm[4] + im[2] * dm[5]},
+                      {im[1] * dm[0] + im[3] * dm[1] + im[4] * dm[2], im[1] * dm[1] + im[3] * dm[3] + im[4] * dm[4],
+                       im[1] * dm[2] + im[3] * dm[4] + im[4] * dm[5]},
+                      {im[2] * dm[0] + im[4] * dm[1] + im[5] * dm[2], im[2] * dm[1] + im[4] * dm[3] + im[5] * dm[4],
+                       im[2] * dm[2] + im[4] * dm[4] + im[5] * dm[5]}};
+    // pol 0: Hp = (0, hh, 0); pol 1: Hp = (hh, 0, 0)
+    for (int a = 0; a < 3; a++) {
+      mh[t][0][a] = M[a][1] * hh;
+      mh[t][1][a] = M[a][0] * hh;
+    }
+  }
+  __syncthreads();
+  if (t < NG) {
+    double Ji[9];
+    double det = jacobian<NN>(tab, t, xs, Ji);
+    double wgt = det * tab->w[t];
+    double h[3] = {1.0, 1.0, 1.0};
+    if (gp.on) {
+      int fl = elem_flags(g, gp, prev, first_flags, e);
+      double r[3] = {0, 0, 0};
+      for (int n = 0; n < NN; n++)
+        for (int c = 0; c < 3; c++) r[c] += tab->N[t][n] * xs[n][c];
+      for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, r[c]);
+    }
+    double wh = wgt * (h[0] * h[1] * h[2]);
+    for (int d = 0; d < 2; d++) {
+      double sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};  // dmpf + pcrl
+      for (int n = 0; n < NN; n++) {
+        double N = tab->N[t][n];
+        // dsigma . Ep: Ep has a single imaginary component (-i zz on x for pol 0, +i zz on y for pol 1)
+        int c0 = d == 0 ? 0 : 1;
+        double ep_im = d == 0 ? -zz[n] : zz[n];
+        static const int T6c[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        for (int a = 0; a < 3; a++) {
+          int q = T6c[a][c0];
+          double dr = ds[n][2 * q], di = ds[n][2 * q + 1];
+          // (dr + i di) * (i ep_im)
+          sr[a] += N * (-di * ep_im);
+          si[a] += N * (dr * ep_im);
+        }
+        // grad N x (mu^-1 dmu Hp)
+        double dn[3];
+        for (int a = 0; a < 3; a++)
+          dn[a] = Ji[a * 3 + 0] * tab->dN[t][0][n] + Ji[a * 3 + 1] * tab->dN[t][1][n] + Ji[a * 3 + 2] * tab->dN[t][2][n];
+        const double *v = mh[n][d];
+        sr[0] += v[2] * dn[1] - v[1] * dn[2];
+        sr[1] += v[0] * dn[2] - v[2] * dn[0];
+        sr[2] += v[1] * dn[0] - v[0] * dn[1];
+      }
+      // times -i w32: (a + i b)(-i w) = b w - i a w
+      for (int a = 0; a < 3; a++) {
+        swb[t][d][2 * a] = wh * (si[a] * w32);
+        swb[t][d][2 * a + 1] = wh * (-sr[a] * w32);
+      }
+    }
+    for (int c = 0; c < 9; c++) sJi[t][c] = Ji[c];
+  }
+  __syncthreads();
+  for (int ed = t; ed < NE; ed += 64) {
+    int d = tab->dir[ed];
+    double b[4] = {0, 0, 0, 0};
+    for (int gg = 0; gg < NG; gg++) {
+      double ph = tab->phi[gg][ed];
+      double v0 = ph * sJi[gg][0 * 3 + d], v1 = ph * sJi[gg][1 * 3 + d], v2 = ph * sJi[gg][2 * 3 + d];
+      for (int pol = 0; pol < 2; pol++) {
+        const double *s = swb[gg][pol];
+        b[2 * pol] += v0 * s[0] + v1 * s[2] + v2 * s[4];
+        b[2 * pol + 1] += v0 * s[1] + v1 * s[3] + v2 * s[5];
+      }
+    }
+    double *o = be + ((e - e0) * NE + ed) * 4;
+    o[0] = b[0]; o[1] = b[1]; o[2] = b[2]; o[3] = b[3];
+  }
+}
+
+template <typename T, int NN, int NE, int NG, int GC>
+static int launch_km(mvf_handle *h, int64_t e0, int64_t n, int first_flags, T *dK, T *dM, int *bad) {
+  int prev = (h->emulate & MVF_EMU_GPML_PREV_FLAGS) ? 1 : 0;
+  const int64_t chunk = 1 << 20;
+  for (int64_t s = 0; s < n; s += chunk) {
+    int64_t c = (n - s) < chunk ? (n - s) : chunk;
+    constexpr int64_t NP = NE * (NE + 1) / 2;
+    hipLaunchKernelGGL((k_elem_km<T, NN, NE, NG, GC>), dim3((unsigned)c), dim3(256), 0, h->stream, h->d_tab, h->d_xp,
+                       h->d_yp, h->d_zp, h->d_inmu, h->d_sig, h->geo, h->gp, prev, first_flags, e0 + s, dK + s * NP,
+                       dM + s * NP, bad);
+  }
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int integrate_t(mvf_handle *h, int64_t e0, int64_t e1, int first_flags, T *dK, T *dM) {
+  int *bad;
+  MVF_TRY(dalloc(h, &bad, 1));
+  MVF_HIP(h, hipMemsetAsync(bad, 0, sizeof(int), h->stream));
+  int64_t n = e1 - e0;
+  int rc = 0;
+  if (n > 0) {
+    if (h->nodes == 8) rc = launch_km<T, 8, 12, 8, 8>(h, e0, n, first_flags, dK, dM, bad);
+    else if (h->nodes == 20) rc = launch_km<T, 20, 36, 27, 9>(h, e0, n, first_flags, dK, dM, bad);
+    else rc = launch_km<T, 27, 54, 27, 6>(h, e0, n, first_flags, dK, dM, bad);
+  }
+  int hb = 0;
+  if (!rc) {
+    MVF_HIP(h, hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipStreamSynchronize(h->stream));
+  }
+  dfree(bad);
+  if (rc) return rc;
+  if (hb) return set_err(h, MVF_E_SINGULAR, "zero Jacobian determinant in an element (degenerate coordinates)");
+  return 0;
+}
+
+int integrate_elements(mvf_handle *h, int64_t e0, int64_t e1, int first_flags, double *dK, double *dM) {
+  return integrate_t<double>(h, e0, e1, first_flags, dK, dM);
+}
+
+int integrate_elements32(mvf_handle *h, int64_t e0, int64_t e1, int first_flags, float *dK, float *dM) {
+  return integrate_t<float>(h, e0, e1, first_flags, dK, dM);
+}
+
+int element_rhs_range(mvf_handle *h, double omega, int first_flags, int64_t e0, int64_t e1, double *d_be) {
+  int prev = (h->emulate & MVF_EMU_GPML_PREV_FLAGS) ? 1 : 0;
+  const int64_t chunk = 1 << 22;
+  const int me = h->et.ne;
+  for (int64_t s = e0; s < e1; s += chunk) {
+    int64_t c = (e1 - s) < chunk ? (e1 - s) : chunk;
+    double *out = d_be + (s - e0) * me * 4;
+    if (h->nodes == 8)
+      hipLaunchKernelGGL((k_elem_rhs<8, 12, 8>), dim3((unsigned)c), dim3(64), 0, h->stream, h->d_tab, h->d_xp, h->d_yp,
+                         h->d_zp, h->d_sig, h->d_inmu, h->d_dmu, h->geo, h->gp, prev, first_flags, omega, s, out);
+    else if (h->nodes == 20)
+      hipLaunchKernelGGL((k_elem_rhs<20, 36, 27>), dim3((unsigned)c), dim3(64), 0, h->stream, h->d_tab, h->d_xp,
+                         h->d_yp, h->d_zp, h->d_sig, h->d_inmu, h->d_dmu, h->geo, h->gp, prev, first_flags, omega, s, out);
+    else
+      hipLaunchKernelGGL((k_elem_rhs<27, 54, 27>), dim3((unsigned)c), dim3(64), 0, h->stream, h->d_tab, h->d_xp,
+                         h->d_yp, h->d_zp, h->d_sig, h->d_inmu, h->d_dmu, h->geo, h->gp, prev, first_flags, omega, s, out);
+  }
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+// update_sigma (geometry.f90:144-153): linear entries [0, min(npt+30, 6 npt))
+// whose imaginary part is non-zero get imag = float32(eps*omega).  Without
+// MVF_EMU_UPDATE_SIGMA every entry is refreshed.
+__global__ void k_update_sigma(double *sig, int64_t lim, double im) {
+  int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= lim) return;
+  if (sig[2 * L + 1] != 0.0) sig[2 * L + 1] = im;
+}
+
+int update_sigma(mvf_handle *h, double omega) {
+  int64_t npt = h->geo.npt;
+  int64_t lim = (h->emulate & MVF_EMU_UPDATE_SIGMA) ? npt + 30 : 6 * npt;
+  if (lim > 6 * npt) lim = 6 * npt;
+  double im = (double)(float)(MVF_EPS0 * omega);
+  hipLaunchKernelGGL(k_update_sigma, dim3(grid1d(lim, 256)), dim3(256), 0, h->stream, h->d_sig, lim, im);
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+}  // namespace mvf

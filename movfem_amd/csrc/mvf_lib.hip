@@ -1,0 +1,5 @@
+// Single translation unit for libmovfem_gpu.so.
+#include "mvf_setup.hip"
+#include "mvf_integrate.hip"
+#include "mvf_solver.hip"
+#include "mvf_api.hip"

@@ -1,0 +1,200 @@
+"""Host-side mirror of the reference's interfaces for the hot path.
+
+The reference drives the path from its frequency loop (MoVFEM_3DMT.f90:62-143):
+
+    initialize_vfem()                       -> GpuVfem(model)
+    update_omega / update_sigma / pset_pmodel -> set_frequency(i)
+    int_elem_params + alocal/blocal         -> integrate(e0, e1), alocal(e), blocal(e)
+    global_vfem(irn, jcn, a, rhs)           -> global_vfem()
+    find_zeros / rem_zeros                  -> remove_zeros(ia, ja, a)
+    zmumps(job=6)                           -> solve(rhs, ...)
+
+Every call goes to libmovfem_gpu.so; there is no CPU fallback.  Errors raise
+MvfError carrying the library's code and message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .model import MTModel
+
+
+class MvfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{L.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacobi", tol=1e-8, maxit=100000,
+                check_every=25, inner_tol=1e-2) -> L.SolverOpts:
+    o = L.SolverOpts()
+    L.lib().mvf_default_solver_opts(C.byref(o))
+    o.method = {"cocg": L.MVF_KRYLOV_COCG, "bicgstab": L.MVF_KRYLOV_BICGSTAB}[method]
+    o.precision = {"fp64": L.MVF_PREC_FP64, "mixed": L.MVF_PREC_MIXED}[precision]
+    o.matrix = {"c64ref": L.MVF_MATRIX_C64REF, "fp64": L.MVF_MATRIX_FP64}[matrix]
+    o.precond = {"jacobi": L.MVF_PC_JACOBI, "none": L.MVF_PC_NONE}[precond]
+    o.tol = tol
+    o.maxit = maxit
+    o.check_every = check_every
+    o.inner_tol = inner_tol
+    return o
+
+
+def info_dict(info: L.SolveInfo, ncol: int) -> dict:
+    return {"iterations": info.iterations, "refinements": info.refinements, "converged": bool(info.converged),
+            "fallbacks": info.fallbacks, "true_resid": [info.true_resid[i] for i in range(min(ncol, 64))],
+            "seconds": info.seconds, "spmv_launches": list(info.spmv_launches), "spmv_ms": list(info.spmv_ms),
+            "spmv_bytes": list(info.spmv_bytes), "spmv_ncol": info.spmv_ncol}
+
+
+class GpuVfem:
+    """One model resident on one GPU (the handle owns every device buffer)."""
+
+    def __init__(self, m: MTModel, emulate: int = L.MVF_EMU_ALL, device: int = 0):
+        m.validate()
+        self.m = m
+        self._lib = L.lib()
+        self._keep = [np.ascontiguousarray(m.xp, np.float64), np.ascontiguousarray(m.yp, np.float64),
+                      np.ascontiguousarray(m.zp, np.float64),
+                      np.ascontiguousarray(m.sigma.astype(np.complex128)).view(np.float64),
+                      np.ascontiguousarray(m.mu, np.float64)]
+        xp, yp, zp, sg, mu = self._keep
+        self._grid = L.Grid(m.nx, m.ny, m.nz, m.nodes_per_elem, _dp(xp), _dp(yp), _dp(zp))
+        self._model = L.Model(_dp(sg), _dp(mu))
+        self._bc = L.BC(int(m.dirichlet), m.inimod, m.gpml_sch, m.nextd, m.nzl_top, 0, m.a0, m.b0, m.nn)
+        self._opts = L.Options(emulate, device, 0, 0)
+        self.h = C.c_void_p()
+        rc = self._lib.mvf_init(C.byref(self._grid), C.byref(self._model), C.byref(self._bc), C.byref(self._opts),
+                                C.byref(self.h))
+        if rc:
+            msg = self._lib.mvf_last_error(self.h).decode() if self.h else ""
+            self.close()
+            raise MvfError(rc, msg)
+        nne = C.c_int32()
+        nnze = C.c_int64()
+        ntriu = C.c_int64()
+        self._check(self._lib.mvf_sizes(self.h, C.byref(nne), C.byref(nnze), C.byref(ntriu)))
+        self.nne, self.nnze, self.nnz_triu = nne.value, nnze.value, ntriu.value
+        self.me = m.edges_per_elem
+        self.omega = None
+
+    def _check(self, rc):
+        if rc:
+            raise MvfError(rc, self._lib.mvf_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self._lib.mvf_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup_times(self):
+        t = np.zeros(4)
+        self._check(self._lib.mvf_setup_times(self.h, _dp(t)))
+        return {"numbering": t[0], "pattern": t[1], "integration": t[2], "assembly": t[3]}
+
+    def assemble(self, precision: str = "fp64") -> float:
+        """Re-integrate and re-assemble K, M from the resident model; returns the
+        element kernel's time in ms."""
+        ms = np.zeros(1)
+        self._check(self._lib.mvf_assemble(self.h, {"fp64": 0, "fp32": 1}[precision], _dp(ms)))
+        return float(ms[0])
+
+    def gne(self) -> np.ndarray:
+        out = np.empty(self.m.ne * self.me, np.int32)
+        self._check(self._lib.mvf_get_gne(self.h, _ip(out)))
+        return out.reshape(self.m.ne, self.me)
+
+    def set_frequency(self, i: int):
+        """update_omega(i+1), update_sigma, pset_pmodel for frequency index i."""
+        self.set_frequency_hz(float(self.m.freqs[i]))
+
+    def set_frequency_hz(self, f: float):
+        self._check(self._lib.mvf_set_frequency(self.h, float(f)))
+        self.omega = 2.0 * 3.1415926535897932384626433 * f
+
+    def integrate(self, e0: int, e1: int, want_b: bool = True):
+        """K, M upper-packed (n, E(E+1)/2) and b (n, E, 2) for elements [e0, e1)."""
+        n = e1 - e0
+        np_ = self.me * (self.me + 1) // 2
+        K = np.empty((n, np_))
+        M = np.empty((n, np_))
+        b = np.empty((n, self.me, 2), np.complex128) if want_b else None
+        bp = _dp(b.view(np.float64)) if want_b else None
+        self._check(self._lib.mvf_integrate(self.h, e0, e1, _dp(K), _dp(M), bp))
+        return K, M, b
+
+    def alocal(self, K_packed, M_packed, omega=None):
+        """Full (E, E) element matrix K + i fl32(omega) M from packed rows."""
+        w = float(np.float32(self.omega if omega is None else omega))
+        me = self.me
+        iu = np.triu_indices(me)
+        A = np.zeros((me, me), np.complex128)
+        A[iu] = K_packed + 1j * (w * M_packed)
+        A = A + np.triu(A, 1).T
+        return A
+
+    def global_vfem(self):
+        """(ia, ja, a, b) exactly as the reference's global_vfem leaves them."""
+        ia = np.empty(self.nnze, np.int32)
+        ja = np.empty(self.nnze, np.int32)
+        a = np.empty(self.nnze, np.complex128)
+        b = np.empty(2 * self.nne, np.complex128)
+        self._check(self._lib.mvf_global_vfem(self.h, _ip(ia), _ip(ja), _dp(a.view(np.float64)),
+                                               _dp(b.view(np.float64)), self.nnze, self.nne, 2))
+        return ia, ja, a, b
+
+    def remove_zeros(self, ia, ja, a):
+        ia, ja = ia.copy(), ja.copy()
+        a = np.ascontiguousarray(a, np.complex128).copy()
+        n = C.c_int64()
+        self._check(self._lib.mvf_remove_zeros(len(ia), _ip(ia), _ip(ja), _dp(a.view(np.float64)), C.byref(n)))
+        return ia[:n.value], ja[:n.value], a[:n.value]
+
+    def solve(self, rhs, **kw):
+        """zmumps(job=6)-shaped: rhs (2*nne complex, polarization-major) -> solution, info."""
+        x = np.ascontiguousarray(rhs, np.complex128).copy()
+        o = solver_opts(**kw)
+        info = L.SolveInfo()
+        rc = self._lib.mvf_solve(self.h, _dp(x.view(np.float64)), C.byref(o), C.byref(info))
+        d = info_dict(info, 2)
+        if rc:
+            raise MvfError(rc, self._lib.mvf_last_error(self.h).decode() + f" {d}")
+        return x, d
+
+    def solve_frequencies(self, freqs, **kw):
+        """Device RHS for each frequency (in order) and one batched solve."""
+        f = np.ascontiguousarray(freqs, np.float64)
+        o = solver_opts(**kw)
+        info = L.SolveInfo()
+        rc = self._lib.mvf_solve_frequencies(self.h, _dp(f), len(f), C.byref(o), C.byref(info))
+        d = info_dict(info, 2 * len(f))
+        if rc:
+            raise MvfError(rc, self._lib.mvf_last_error(self.h).decode() + f" {d}")
+        return d
+
+    def solution(self, ncols):
+        x = np.empty(self.nne * ncols, np.complex128)
+        self._check(self._lib.mvf_get_solution(self.h, _dp(x.view(np.float64)), ncols))
+        return x.reshape(ncols, self.nne)
+
+    def rhs(self, ncols):
+        x = np.empty(self.nne * ncols, np.complex128)
+        self._check(self._lib.mvf_get_rhs(self.h, _dp(x.view(np.float64)), ncols))
+        return x.reshape(ncols, self.nne)

@@ -1,0 +1,61 @@
+"""Writes golden vectors from the reference build (oracle/_ref, compiled from
+the reference's own Fortran modules by oracle/Makefile).
+
+Run in the container that holds the reference sources:
+    make -C oracle ref && python -m tests.golden.make_golden
+Each fixture stores the model inputs (lattice, sigma, mu, boundary settings)
+and the reference's outputs: gne, nzindx, alocal/blocal of every element and
+global_vfem (ia, ja, a, b) for the first two frequencies, plus g_sigma after
+update_sigma for frequency 2.
+"""
+from __future__ import annotations
+
+import pathlib
+
+import numpy as np
+
+from movfem_amd import model as M
+from oracle.refrun import run_reference
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+
+def golden_cases():
+    return {
+        "g8_dirichlet_topo": M.random_small(8, 2, 2, 3, seed=31, topo=250.0, nz_air=1, freqs=(0.5, 7.0)),
+        "g8_gpml_mu": M.random_small(8, 3, 2, 3, seed=32, dirichlet=False, nextd=1, mu_contrast=True,
+                                     freqs=(2.0, 0.05)),
+        "g20_gpml_tensor": M.random_small(20, 2, 2, 2, seed=33, dirichlet=False, nextd=1, full_tensor=True,
+                                          topo=120.0, freqs=(1.0, 10.0)),
+        "g27_dirichlet_aniso": M.random_small(27, 2, 2, 2, seed=34, anisotropic=True, nz_air=1, freqs=(3.0, 30.0)),
+    }
+
+
+def main():
+    for name, m in golden_cases().items():
+        els = [(i, j, k) for i in range(1, m.nx + 1) for j in range(1, m.ny + 1) for k in range(1, m.nz + 1)]
+        r = run_reference(m, elements=els, vfem_freqs=[1, 2], sigma_after=[])
+        # sigma after update_sigma at frequency 2 (separate process: state order)
+        r2 = run_reference(m, sigma_after=[2])
+        out = dict(nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp, sigma=m.sigma, mu=m.mu,
+                   freqs=m.freqs, dirichlet=int(m.dirichlet), inimod=m.inimod, gpml_sch=m.gpml_sch, a0=m.a0,
+                   b0=m.b0, nn=m.nn, nextd=m.nextd, nzl_top=m.nzl_top, ztop=m.ztop,
+                   gne=r["gne"], nzindx=r["nzindx"], nne=r["nne"], nnze=r["nnze"], aloc=r["aloc"], bloc=r["bloc"],
+                   sigma_2=r2["sigma_2"])
+        for fi in (1, 2):
+            for k in ("ia", "ja", "a", "b"):
+                out[f"{k}_{fi}"] = r[f"{k}_{fi}"]
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(name, "nne", int(r["nne"]), "nnze", int(r["nnze"]))
+
+
+def load_model(z) -> M.MTModel:
+    m = M.MTModel(int(z["nx"]), int(z["ny"]), int(z["nz"]), int(z["mn"]), z["xp"], z["yp"], z["zp"], z["sigma"],
+                  z["mu"], z["freqs"], dirichlet=bool(z["dirichlet"]), inimod=int(z["inimod"]),
+                  gpml_sch=int(z["gpml_sch"]), a0=float(z["a0"]), b0=float(z["b0"]), nn=float(z["nn"]),
+                  nextd=int(z["nextd"]), nzl_top=int(z["nzl_top"]), ztop=float(z["ztop"]))
+    return m
+
+
+if __name__ == "__main__":
+    main()
