@@ -1,0 +1,132 @@
+! movfem_gpu_mod.f90 -- ISO_C_BINDING interface to libmovfem_gpu.so
+! (include/movfem_gpu.h).  A Fortran caller keeps the reference's call shape:
+!   call global_vfem(irn, jcn, a, rhs)   (MoVFEM_3DMT.f90:82)
+! becomes
+!   ierr = mvf_global_vfem(h, irn, jcn, a, rhs, nnze, nne, 2)
+! and zmumps(job=6) (MoVFEM_3DMT.f90:113-115) becomes mvf_solve(h, rhs, opts, info).
+module movfem_gpu
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: MVF_EMU_ALL = 3
+  integer(c_int), parameter :: MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1
+  integer(c_int), parameter :: MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1
+  integer(c_int), parameter :: MVF_MATRIX_C64REF = 0, MVF_MATRIX_FP64 = 1
+  integer(c_int), parameter :: MVF_PC_NONE = 0, MVF_PC_JACOBI = 1
+
+  type, bind(C) :: mvf_grid
+    integer(c_int32_t) :: nx, ny, nz, nodes_per_elem
+    type(c_ptr) :: xp, yp, zp
+  end type mvf_grid
+
+  type, bind(C) :: mvf_model
+    type(c_ptr) :: sigma, mu
+  end type mvf_model
+
+  type, bind(C) :: mvf_bc
+    integer(c_int32_t) :: dirichlet, inimod, gpml_sch, nextd, nzl_top, pad_
+    real(c_double) :: a0, b0, nn
+  end type mvf_bc
+
+  type, bind(C) :: mvf_options
+    integer(c_int32_t) :: emulate, device, verbose, pad_
+  end type mvf_options
+
+  type, bind(C) :: mvf_solver_opts
+    integer(c_int32_t) :: method, precision, matrix, precond
+    real(c_double) :: tol
+    integer(c_int32_t) :: maxit, check_every
+    real(c_double) :: inner_tol
+    integer(c_int32_t) :: use_graph, pad_
+  end type mvf_solver_opts
+
+  type, bind(C) :: mvf_solve_info
+    integer(c_int32_t) :: iterations, refinements, converged, fallbacks
+    real(c_double) :: true_resid(64)
+    real(c_double) :: seconds
+    integer(c_int64_t) :: spmv_launches(2)
+    real(c_double) :: spmv_ms(2), spmv_bytes(2)
+    integer(c_int32_t) :: spmv_ncol, pad_
+  end type mvf_solve_info
+
+  interface
+    integer(c_int) function mvf_init(grid, model, bc, opts, h) bind(C, name='mvf_init')
+      import :: c_int, c_ptr, mvf_grid, mvf_model, mvf_bc, mvf_options
+      type(mvf_grid), intent(in) :: grid
+      type(mvf_model), intent(in) :: model
+      type(mvf_bc), intent(in) :: bc
+      type(mvf_options), intent(in) :: opts
+      type(c_ptr), intent(out) :: h
+    end function mvf_init
+
+    subroutine mvf_free(h) bind(C, name='mvf_free')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end subroutine mvf_free
+
+    type(c_ptr) function mvf_last_error(h) bind(C, name='mvf_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end function mvf_last_error
+
+    integer(c_int) function mvf_sizes(h, nne, nnze, nnz_triu) bind(C, name='mvf_sizes')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: nne
+      integer(c_int64_t), intent(out) :: nnze, nnz_triu
+    end function mvf_sizes
+
+    integer(c_int) function mvf_set_frequency(h, freq) bind(C, name='mvf_set_frequency')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: freq
+    end function mvf_set_frequency
+
+    integer(c_int) function mvf_global_vfem(h, ia, ja, a, b, nnze, nne, ndir) bind(C, name='mvf_global_vfem')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double_complex
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: ia(*), ja(*)
+      complex(c_double_complex), intent(out) :: a(*), b(*)
+      integer(c_int64_t), value :: nnze
+      integer(c_int32_t), value :: nne, ndir
+    end function mvf_global_vfem
+
+    integer(c_int) function mvf_remove_zeros(n, ia, ja, a, n_out) bind(C, name='mvf_remove_zeros')
+      import :: c_int, c_int32_t, c_int64_t, c_double_complex
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), intent(inout) :: ia(*), ja(*)
+      complex(c_double_complex), intent(inout) :: a(*)
+      integer(c_int64_t), intent(out) :: n_out
+    end function mvf_remove_zeros
+
+    subroutine mvf_default_solver_opts(o) bind(C, name='mvf_default_solver_opts')
+      import :: mvf_solver_opts
+      type(mvf_solver_opts), intent(out) :: o
+    end subroutine mvf_default_solver_opts
+
+    integer(c_int) function mvf_solve(h, rhs, o, info) bind(C, name='mvf_solve')
+      import :: c_int, c_ptr, c_double_complex, mvf_solver_opts, mvf_solve_info
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: rhs(*)
+      type(mvf_solver_opts), intent(in) :: o
+      type(mvf_solve_info), intent(out) :: info
+    end function mvf_solve
+  end interface
+
+contains
+
+  ! the message of the last error on a handle, as a Fortran string
+  function mvf_error_string(h) result(s)
+    type(c_ptr), intent(in) :: h
+    character(len=512) :: s
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    s = ' '
+    call c_f_pointer(mvf_last_error(h), p, [512])
+    do i = 1, 512
+      if (p(i) == c_null_char) exit
+      s(i:i) = p(i)
+    end do
+  end function mvf_error_string
+
+end module movfem_gpu

@@ -979,6 +979,201 @@ int orc_global_vfem(orc_state *s, int32_t *ia, int32_t *ja, double *a, double *b
   return 0;
 }
 
+/* ------------------------------ readout ---------------------------------- */
+/* node_solution (solution.f90:18-49) with get_elem_sol/get_er/get_hr
+ * (:207-438) for pe_sch = 1: total E at every lattice node (secondary from the
+ * edge solution x plus the interpolated primary field), then H =
+ * -i fl32(1/omega) mu^-1 curl E from the element's nodal E; the first element
+ * (reference order) visiting a node sets its value.  x: complex 2*nne,
+ * polarization-major.  esol/hsol: complex (2*npt, 3), polarization-major. */
+int orc_node_solution(orc_state *s, const double *x, double *esol, double *hsol) {
+  if (!s->gne) orc_build_gne(s);
+  if (s->dirichlet && s->inimod != 1) return -4;
+  int o = s->nord - 1, mn = s->mn, me = s->me, nord = s->nord;
+  int64_t npt = s->npt;
+  char *ve = (char *)calloc((size_t)2 * npt, 1), *vh = (char *)calloc((size_t)2 * npt, 1);
+  memset(esol, 0, sizeof(double) * 2 * 3 * 2 * npt);
+  memset(hsol, 0, sizeof(double) * 2 * 3 * 2 * npt);
+  double as[3] = {-1.0, 0.0, 1.0};
+  if (nord == 2) as[1] = 1.0;
+  orc_ws *w = (orc_ws *)malloc(sizeof(orc_ws));
+  const zc mi = CMPLX(0.0, (double)(float)(-1.0 / s->omega));
+  for (int ie = 1; ie <= s->nx; ie++)
+    for (int je = 1; je <= s->ny; je++)
+      for (int ke = 1; ke <= s->nz; ke++) {
+        int64_t ide = ((int64_t)(ie - 1) * s->ny + (je - 1)) * s->nz + (ke - 1);
+        int64_t eno = (int64_t)(ie - 1) * s->nyz * o + (int64_t)(je - 1) * s->nnz * o + (int64_t)(ke - 1) * o;
+        /* nf_get_r + p_elem_fields: node coordinates and primary fields */
+        int rc = orc_elem_params(s, w, ie, je, ke);
+        if (rc) {
+          free(ve); free(vh); free(w);
+          return rc;
+        }
+        const int32_t *g = s->gne + ide * me;
+        for (int d = 0; d < 2; d++) {
+          for (int i1 = 0; i1 < nord; i1++)
+            for (int j1 = 0; j1 < nord; j1++)
+              for (int k1 = 0; k1 < nord; k1++) {
+                int64_t id = eno + (int64_t)i1 * s->nyz + (int64_t)j1 * s->nnz + k1;
+                int64_t idd = id + d * npt;
+                if (ve[idd]) continue;
+                double r[3] = {as[i1], as[j1], as[k1]};
+                double J[3][3], Ji[3][3], det;
+                orc_jac(s, w->re, r, J, Ji, &det);
+                zc ef[3] = {0, 0, 0}, ep[3] = {0, 0, 0};
+                for (int im = 0; im < me; im++) {
+                  zc f = 0.0; /* f_boundary is zero for inimod 1 and GPML */
+                  if (g[im] >= 0) f = CMPLX(x[2 * ((int64_t)g[im] - 1 + d * (int64_t)s->nne)],
+                                            x[2 * ((int64_t)g[im] - 1 + d * (int64_t)s->nne) + 1]);
+                  int node = s->ed[im][0], dir = s->ed[im][1] - 1;
+                  double phi = mix_ln(s, node, dir, r);
+                  for (int c = 0; c < 3; c++) ef[c] += f * (phi * Ji[c][dir]);
+                }
+                for (int n = 0; n < mn; n++) {
+                  double N = nf_ln(s, n + 1, r);
+                  for (int c = 0; c < 3; c++) ep[c] += w->ep[d][n][c] * N;
+                }
+                for (int c = 0; c < 3; c++) {
+                  zc v = ef[c] + ep[c];
+                  esol[2 * (idd * 3 + c)] = creal(v);
+                  esol[2 * (idd * 3 + c) + 1] = cimag(v);
+                }
+                ve[idd] = 1;
+              }
+          for (int i1 = 0; i1 < nord; i1++)
+            for (int j1 = 0; j1 < nord; j1++)
+              for (int k1 = 0; k1 < nord; k1++) {
+                int64_t id = eno + (int64_t)i1 * s->nyz + (int64_t)j1 * s->nnz + k1;
+                int64_t idd = id + d * npt;
+                if (vh[idd]) continue;
+                double r[3] = {as[i1], as[j1], as[k1]};
+                double J[3][3], Ji[3][3], det;
+                orc_jac(s, w->re, r, J, Ji, &det);
+                zc ef[3] = {0, 0, 0};
+                for (int n = 0; n < mn; n++) {
+                  int64_t nd = w->idx[n] + d * npt;
+                  zc f2[3];
+                  for (int c = 0; c < 3; c++) f2[c] = CMPLX(esol[2 * (nd * 3 + c)], esol[2 * (nd * 3 + c) + 1]);
+                  double dn[3];
+                  for (int a = 0; a < 3; a++) {
+                    double acc = 0.0;
+                    for (int b = 0; b < 3; b++) acc += Ji[a][b] * nf_dln(s, b, n + 1, r);
+                    dn[a] = acc;
+                  }
+                  ef[0] += (f2[2] * dn[1] - f2[1] * dn[2]);
+                  ef[1] += (f2[0] * dn[2] - f2[2] * dn[0]);
+                  ef[2] += (f2[1] * dn[0] - f2[0] * dn[1]);
+                }
+                double m1[6];
+                for (int c = 0; c < 6; c++) {
+                  double a1 = 0.0;
+                  for (int n = 0; n < mn; n++) a1 += nf_ln(s, n + 1, r) * w->inmu[n][c];
+                  m1[c] = a1;
+                }
+                zc md[6];
+                for (int c = 0; c < 6; c++) md[c] = mi * m1[c];
+                zc hv[3] = {md[0] * ef[0] + md[1] * ef[1] + md[2] * ef[2], md[1] * ef[0] + md[3] * ef[1] + md[4] * ef[2],
+                            md[2] * ef[0] + md[4] * ef[1] + md[5] * ef[2]};
+                for (int c = 0; c < 3; c++) {
+                  hsol[2 * (idd * 3 + c)] = creal(hv[c]);
+                  hsol[2 * (idd * 3 + c) + 1] = cimag(hv[c]);
+                }
+                vh[idd] = 1;
+              }
+        }
+      }
+  free(ve); free(vh); free(w);
+  return 0;
+}
+
+/* get_impedance + inv_matrix + get_res_phase (solution.f90:456-505):
+ * e, h = (x pol1, x pol2, y pol1, y pol2) */
+void orc_impedance(const double *e_, const double *h_, double omega, double *z_, double *rho, double *phi) {
+  zc e[4], h[4], ih[4], z[4];
+  for (int i = 0; i < 4; i++) {
+    e[i] = CMPLX(e_[2 * i], e_[2 * i + 1]);
+    h[i] = CMPLX(h_[2 * i], h_[2 * i + 1]);
+  }
+  zc det = h[0] * h[3] - h[1] * h[2];
+  ih[0] = h[3] / det;
+  ih[1] = -h[1] / det;
+  ih[2] = -h[2] / det;
+  ih[3] = h[0] / det;
+  z[0] = ih[0] * e[0] + ih[2] * e[1];
+  z[1] = ih[1] * e[0] + ih[3] * e[1];
+  z[2] = ih[0] * e[2] + ih[2] * e[3];
+  z[3] = ih[1] * e[2] + ih[3] * e[3];
+  double mu0 = 4 * ORC_PI * 1.e-7;
+  for (int i = 0; i < 4; i++) {
+    z_[2 * i] = creal(z[i]);
+    z_[2 * i + 1] = cimag(z[i]);
+    rho[i] = (1.0 / (omega * mu0)) * (creal(z[i]) * creal(z[i]) + cimag(z[i]) * cimag(z[i]));
+    if (rho[i] < 1e-2) {
+      rho[i] = 0.0;
+      phi[i] = 0.0;
+    } else {
+      phi[i] = atan(cimag(z[i]) / creal(z[i])) * (180.0 / ORC_PI);
+    }
+  }
+}
+
+/* rx_data with get_rx_element and interpolate_rx (receiver_data.f90:111-253):
+ * bilinear interpolation of the nodal E and H on the top face of element
+ * (ie, je, ke_top) containing (x, y), then impedance, rho_a and phase.
+ * ie runs over nextd+1 .. nx-nextd (the reference's inner region).
+ * out_e/out_h: complex (nrx, 2 pol, 3); out_z complex (nrx, 4); rho/phi (nrx, 4). */
+int orc_receivers(orc_state *s, const double *esol, const double *hsol, int ke_top, int nrx, const double *rx,
+                  const double *ry, double *out_e, double *out_h, double *out_z, double *rho, double *phi) {
+  int o = s->nord - 1;
+  for (int ir = 0; ir < nrx; ir++) {
+    int ie = -1, je = -1;
+    for (int a = s->nextd + 1; a <= s->nx - s->nextd; a++)
+      if (rx[ir] >= s->xp[(a - 1) * o] && rx[ir] < s->xp[a * o]) ie = a;
+    for (int b = s->nextd + 1; b <= s->ny - s->nextd; b++)
+      if (ry[ir] >= s->yp[(b - 1) * o] && ry[ir] < s->yp[b * o]) je = b;
+    if (ie < 0 || je < 0) return -5;
+    int ke = ke_top;
+    int64_t eno = (int64_t)(ie - 1) * o * s->nyz + (int64_t)(je - 1) * o * s->nnz + (int64_t)(ke - 1) * o;
+    double x1 = s->xp[(ie - 1) * o], x2 = s->xp[(ie - 1) * o + o];
+    double y1 = s->yp[(je - 1) * o], y2 = s->yp[(je - 1) * o + o];
+    int64_t id11 = eno + o, id12 = eno + (int64_t)o * s->nnz + o, id21 = eno + (int64_t)o * s->nyz + o,
+            id22 = eno + (int64_t)o * s->nyz + (int64_t)o * s->nnz + o;
+    double x = rx[ir], y = ry[ir];
+    double ax = (x2 - x) / (x2 - x1), bx = (x - x1) / (x2 - x1);
+    double ay = (y2 - y) / (y2 - y1), by = (y - y1) / (y2 - y1);
+    double e4[8], h4[8];
+    for (int d = 0; d < 2; d++) {
+      for (int c = 0; c < 3; c++) {
+        for (int q = 0; q < 2; q++) {
+          const double *S[2] = {esol, hsol};
+          for (int t = 0; t < 2; t++) {
+            const double *F = S[t];
+            double f11 = F[2 * ((id11 + d * s->npt) * 3 + c) + q], f12 = F[2 * ((id12 + d * s->npt) * 3 + c) + q];
+            double f21 = F[2 * ((id21 + d * s->npt) * 3 + c) + q], f22 = F[2 * ((id22 + d * s->npt) * 3 + c) + q];
+            double fx1 = ax * f11 + bx * f21, fx2 = ax * f12 + bx * f22;
+            double v = ay * fx1 + by * fx2;
+            double *O = t == 0 ? out_e : out_h;
+            O[2 * ((ir * 2 + d) * 3 + c) + q] = v;
+          }
+        }
+      }
+    }
+    /* assign_rxmatrix: (x pol1, x pol2, y pol1, y pol2) */
+    for (int q = 0; q < 2; q++) {
+      e4[0 + q] = out_e[2 * ((ir * 2 + 0) * 3 + 0) + q];
+      e4[2 + q] = out_e[2 * ((ir * 2 + 1) * 3 + 0) + q];
+      e4[4 + q] = out_e[2 * ((ir * 2 + 0) * 3 + 1) + q];
+      e4[6 + q] = out_e[2 * ((ir * 2 + 1) * 3 + 1) + q];
+      h4[0 + q] = out_h[2 * ((ir * 2 + 0) * 3 + 0) + q];
+      h4[2 + q] = out_h[2 * ((ir * 2 + 1) * 3 + 0) + q];
+      h4[4 + q] = out_h[2 * ((ir * 2 + 0) * 3 + 1) + q];
+      h4[6 + q] = out_h[2 * ((ir * 2 + 1) * 3 + 1) + q];
+    }
+    orc_impedance(e4, h4, s->omega, out_z + 8 * ir, rho + 4 * ir, phi + 4 * ir);
+  }
+  return 0;
+}
+
 /* find_zeros + rem_zeros (global_assembly.f90:123-150): compacts in place,
  * returns the new length */
 int64_t orc_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a) {

@@ -55,6 +55,12 @@ def lib():
         L.orc_global_vfem.argtypes = [C.c_void_p, ip, ip, dp, dp]
         L.orc_remove_zeros.restype = C.c_int64
         L.orc_remove_zeros.argtypes = [C.c_int64, ip, ip, dp]
+        L.orc_node_solution.restype = C.c_int
+        L.orc_node_solution.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_receivers.restype = C.c_int
+        L.orc_receivers.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_impedance.restype = None
+        L.orc_impedance.argtypes = [dp, dp, C.c_double, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -162,6 +168,34 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_global_vfem failed ({rc})")
         return ia, ja, a.view(np.complex128), b.view(np.complex128)
+
+    def node_solution(self, x):
+        """esol, hsol (2, npt, 3) complex: total E and H at every lattice node for
+        the solution x (2*nne, polarization-major) at the current frequency."""
+        npt = self.m.npt
+        xv = np.ascontiguousarray(np.asarray(x, np.complex128).ravel()).view(np.float64)
+        es = np.empty(2 * npt * 3 * 2)
+        hs = np.empty(2 * npt * 3 * 2)
+        rc = lib().orc_node_solution(self.h, _dp(xv), _dp(es), _dp(hs))
+        if rc:
+            raise RuntimeError(f"orc_node_solution failed ({rc})")
+        return es.view(np.complex128).reshape(2, npt, 3), hs.view(np.complex128).reshape(2, npt, 3)
+
+    def receivers(self, esol, hsol, ke_top, rx, ry):
+        """E, H (nrx, 2, 3), Z (nrx, 4), rho_a, phase (nrx, 4) at receivers."""
+        rx = np.ascontiguousarray(rx, np.float64)
+        ry = np.ascontiguousarray(ry, np.float64)
+        n = len(rx)
+        es = np.ascontiguousarray(esol, np.complex128).view(np.float64)
+        hs = np.ascontiguousarray(hsol, np.complex128).view(np.float64)
+        oe, oh, oz = np.empty(n * 12), np.empty(n * 12), np.empty(n * 8)
+        rho, phi = np.empty(n * 4), np.empty(n * 4)
+        rc = lib().orc_receivers(self.h, _dp(es), _dp(hs), ke_top, n, _dp(rx), _dp(ry), _dp(oe), _dp(oh), _dp(oz),
+                                 _dp(rho), _dp(phi))
+        if rc:
+            raise RuntimeError(f"orc_receivers failed ({rc})")
+        return (oe.view(np.complex128).reshape(n, 2, 3), oh.view(np.complex128).reshape(n, 2, 3),
+                oz.view(np.complex128).reshape(n, 4), rho.reshape(n, 4), phi.reshape(n, 4))
 
     @staticmethod
     def remove_zeros(ia, ja, a):
