@@ -35,6 +35,10 @@ FP64_PEAK_TFS = 78.6
 FP32_PEAK_TFS = 157.3
 # algorithmic flops per element matrix (K, M upper and b), canonical counts
 ELEM_FLOPS = {8: 21768, 20: 376731, 27: 744795}
+# HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs, scaled by the same counters on a 1 GiB copy
+# (profiles/k_spmv_halfspace8.md)
+TRAFFIC_MEASURED = {("halfspace8", "k_spmv<float>"): 11.98e6}
 
 
 def workload(name):
@@ -54,20 +58,25 @@ def workload(name):
     return m, desc
 
 
-def cpu_baseline(nodes, full_ne, full_nne, budget_s=25.0):
+def cpu_baseline(m_full, budget_s=25.0):
     """Reference path on the host: the reference's own modules (oracle/_ref) for
     element integration + assembly + sort of one frequency, SciPy SuperLU
-    (complex128, stand-in for MUMPS 4.10) for the 2-RHS solve, on a reduced
-    model of the same family; scaled linearly to the full model."""
+    (complex128, stand-in for MUMPS 4.10) for the 2-RHS solve.  A workload
+    small enough to finish in the budget is timed whole; a larger one on a
+    reduced model of the same family, scaled linearly to the full model."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
 
     from movfem_amd import model as M
     from oracle.oracle import Oracle, ref_available
 
-    n = 10 if nodes == 20 else 16
-    m = M.anisotropic_blocks(nx=n, ny=n, nz_earth=max(2, (3 * n) // 4 - 2), nz_air=2, h=500.0, nodes=nodes,
-                             freqs=(1e-3,), block=4, seed=2, nextd=2)
+    nodes = m_full.nodes_per_elem
+    if m_full.ne <= 10000:
+        m = m_full
+    else:
+        n = 10 if nodes == 20 else 16
+        m = M.anisotropic_blocks(nx=n, ny=n, nz_earth=max(2, (3 * n) // 4 - 2), nz_air=2, h=500.0, nodes=nodes,
+                                 freqs=(1e-3,), block=4, seed=2, nextd=2)
     kind = "reference" if ref_available() else "port"
     t0 = time.perf_counter()
     if kind == "reference":
@@ -93,14 +102,19 @@ def cpu_baseline(nodes, full_ne, full_nne, budget_s=25.0):
     for d in range(2):
         lu.solve(b[d * nne:(d + 1) * nne])
     t_solve = time.perf_counter() - t1
-    scale_e = full_ne / m.ne
-    scale_n = full_nne / nne
-    t_full = t_asm * scale_e + t_solve * scale_n
-    return {"value": 1.0 / t_full, "unit": "solves/s", "cores": 1, "kind": kind,
-            "sample": (f"{m.nx}x{m.ny}x{m.nz} {nodes}-node model of the same family ({m.ne} elements, {nne} DOFs): "
-                       f"{'reference modules' if kind == 'reference' else 'C restatement'} element loop + assembly + "
-                       f"sort {t_asm:.2f}s, SciPy SuperLU (MUMPS stand-in) factor+2 solves {t_solve:.2f}s; scaled "
-                       f"linearly to {full_ne} elements / {full_nne} DOFs (linear scaling flatters the direct solve)"),
+    what = f"{'reference modules' if kind == 'reference' else 'C restatement'}"
+    if m is m_full:
+        t_full = t_asm + t_solve
+        sample = (f"the whole workload ({m.ne} elements, {nne} DOFs), one frequency: {what} element loop + assembly + "
+                  f"sort {t_asm:.2f}s, SciPy SuperLU (MUMPS stand-in) factor + 2 solves {t_solve:.2f}s")
+    else:
+        full_ne, full_nne = m_full.ne, m_full.nne_hint
+        t_full = t_asm * (full_ne / m.ne) + t_solve * (full_nne / nne)
+        sample = (f"{m.nx}x{m.ny}x{m.nz} {nodes}-node model of the same family ({m.ne} elements, {nne} DOFs): {what} "
+                  f"element loop + assembly + sort {t_asm:.2f}s, SciPy SuperLU (MUMPS stand-in) factor + 2 solves "
+                  f"{t_solve:.2f}s; scaled linearly to {full_ne} elements / {full_nne} DOFs (linear scaling flatters "
+                  f"the direct solve)")
+    return {"value": 1.0 / t_full, "unit": "solves/s", "cores": 1, "kind": kind, "sample": sample,
             "wall_s": time.perf_counter() - t0}
 
 
@@ -109,11 +123,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="aniso20")
+    ap.add_argument("--workload", default="halfspace8")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--maxit", type=int, default=60000)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+
+        n = args.copy_probe_mib << 20
+        x = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -148,7 +175,8 @@ def main():
     achieved = last["spmv_bytes"][k] / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": "k_spmv<float>" if k == 1 else "k_spmv<double>",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "avg_launch_ms": avg_ms, "launches": last["spmv_launches"][k],
+                "traffic": TRAFFIC_MEASURED.get((args.workload, "k_spmv<float>" if k == 1 else "k_spmv<double>")),
+                "avg_launch_ms": avg_ms, "launches": last["spmv_launches"][k],
                 "bytes_per_launch": last["spmv_bytes"][k], "columns": last["spmv_ncol"]}
     # element kernel (FP64 and FP32 integration)
     e64 = float(np.median(elem_ms))
@@ -180,7 +208,8 @@ def main():
         out["plain_gpu_iterations"] = its
         out["speedup_vs_plain"] = value / out["plain_gpu_solves_per_s"]
     if not args.no_cpu:
-        cb = cpu_baseline(m.nodes_per_elem, m.ne, g.nne)
+        m.nne_hint = g.nne
+        cb = cpu_baseline(m)
         out["cpu_baseline"] = {k2: cb[k2] for k2 in ("value", "unit", "cores", "kind", "sample")}
         out["speedup_vs_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)

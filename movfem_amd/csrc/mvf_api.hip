@@ -543,6 +543,16 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
     info->iterations += bi.iterations;
     info->refinements += bi.refinements;
     info->fallbacks += bi.fallbacks;
+    // SpMV timing is reported for the widest batch (bytes are per launch and
+    // depend on the column count, so batches of different widths do not mix)
+    if (bi.spmv_ncol >= info->spmv_ncol) {
+      for (int p = 0; p < 2; p++) {
+        info->spmv_launches[p] = bi.spmv_launches[p];
+        info->spmv_ms[p] = bi.spmv_ms[p];
+        info->spmv_bytes[p] = bi.spmv_bytes[p];
+      }
+      info->spmv_ncol = bi.spmv_ncol;
+    }
     for (int c = 0; c < 2 * nb && 2 * f + c < 64; c++) info->true_resid[2 * f + c] = bi.true_resid[c];
     MVF_HIP(h, hipMemcpy2DAsync(h->d_x + 2 * (size_t)(2 * f), 16 * (size_t)ncol, xb, 16 * (size_t)pc, 16 * (size_t)(2 * nb),
                                 nne, hipMemcpyDeviceToDevice, h->stream));
