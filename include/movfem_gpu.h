@@ -131,10 +131,15 @@ int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *
 int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_out);
 
 /* ---- solve (replaces zmumps(job=6), MoVFEM_3DMT.f90:113-115) --------------- */
-enum { MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1 };
+enum { MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1, MVF_KRYLOV_GMRES = 2 };
 enum { MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1 };  /* MIXED: FP32 inner Krylov, FP64 refinement */
 enum { MVF_MATRIX_C64REF = 0, MVF_MATRIX_FP64 = 1 };/* which operator the residual is judged on */
-enum { MVF_PC_NONE = 0, MVF_PC_JACOBI = 1 };
+/* MVF_PC_DIRECT: multifrontal LU (FP64, nested dissection of the element box)
+ * of the operator at the batch's first frequency, with a small imaginary
+ * shift on the diagonal of massless (all-air) DOFs; the outer method is then
+ * right-preconditioned GMRES on the unshifted operator whatever `method` says.
+ * It stands where zmumps(job=6) stands (MoVFEM_3DMT.f90:113-115). */
+enum { MVF_PC_NONE = 0, MVF_PC_JACOBI = 1, MVF_PC_DIRECT = 2 };
 
 typedef struct mvf_solver_opts {
   int32_t method;       /* MVF_KRYLOV_* */
@@ -146,7 +151,7 @@ typedef struct mvf_solver_opts {
   int32_t check_every;  /* iterations between convergence checks (host sync) */
   double inner_tol;     /* MIXED: FP32 residual drop before an FP64 refinement step */
   int32_t use_graph;    /* capture the iteration block in a HIP graph */
-  int32_t pad_;
+  int32_t gmres_m;      /* GMRES restart length (MVF_PC_DIRECT), 0 = 30 */
 } mvf_solver_opts;
 
 typedef struct mvf_solve_info {
@@ -163,6 +168,14 @@ typedef struct mvf_solve_info {
   double spmv_bytes[2];    /* algorithmic bytes of one launch: nnz*(K+M+col) + nne*(rowptr + ncol*(x+y)) */
   int32_t spmv_ncol;       /* columns per launch (last batch) */
   int32_t pad_;
+  /* MVF_PC_DIRECT */
+  double factor_seconds;   /* numeric factorizations performed by this call, wall time */
+  double factor_flops;     /* algorithmic real flops of one factorization */
+  double factor_bytes;     /* bytes of the stored factor */
+  int32_t factorizations;  /* factorizations performed by the handle so far */
+  int32_t static_pivots;   /* pivots replaced because the owned block was singular */
+  int32_t precond_applies; /* triangular-solve sweeps performed by this call */
+  int32_t pad2_;
 } mvf_solve_info;
 
 void mvf_default_solver_opts(mvf_solver_opts *o);
@@ -193,6 +206,15 @@ int mvf_setup_times(const mvf_handle *h, double *t4);
  * integration; 1: FP32 integration (K and M rounded from FP32 element sums).
  * integrate_ms (optional) receives the element kernel's time from HIP events. */
 int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms);
+
+/* Host-only symbolic analysis of the sparse direct factorization for a
+ * numbering (gne, ne x me, 1-based, <= 0 removed) and the full CSR pattern
+ * (0-based rowptr[nne+1], col): no device use.  stats[10] = fronts, tree
+ * depth, max owned DOFs, max update DOFs, max front, factor entries, real
+ * flops of one factorization, sum of owned DOFs, seconds, update entries.
+ * Diagnostic; no counterpart in the reference (whose ordering is MUMPS's). */
+int mvf_direct_plan_stats(int32_t nx, int32_t ny, int32_t nz, int32_t me, const int32_t *gne, int64_t nne,
+                          const int64_t *rowptr, const int32_t *col, double *stats);
 
 #ifdef __cplusplus
 }

@@ -12,12 +12,14 @@ MVF_EMU_UPDATE_SIGMA = 2
 MVF_EMU_ALL = 3
 MVF_KRYLOV_COCG = 0
 MVF_KRYLOV_BICGSTAB = 1
+MVF_KRYLOV_GMRES = 2
 MVF_PREC_FP64 = 0
 MVF_PREC_MIXED = 1
 MVF_MATRIX_C64REF = 0
 MVF_MATRIX_FP64 = 1
 MVF_PC_NONE = 0
 MVF_PC_JACOBI = 1
+MVF_PC_DIRECT = 2
 
 ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR", -4: "MVF_E_INDEX",
           -5: "MVF_E_UNSUPPORTED", -6: "MVF_E_NOCONV", -7: "MVF_E_BREAKDOWN", -8: "MVF_E_MEMORY"}
@@ -25,7 +27,8 @@ ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR"
 # every entry point of include/movfem_gpu.h
 EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne", "mvf_set_frequency",
            "mvf_integrate", "mvf_global_vfem", "mvf_remove_zeros", "mvf_default_solver_opts", "mvf_solve",
-           "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble")
+           "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
+           "mvf_direct_plan_stats")
 
 
 class Grid(C.Structure):
@@ -50,14 +53,17 @@ class Options(C.Structure):
 class SolverOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("matrix", C.c_int32), ("precond", C.c_int32),
                 ("tol", C.c_double), ("maxit", C.c_int32), ("check_every", C.c_int32), ("inner_tol", C.c_double),
-                ("use_graph", C.c_int32), ("pad_", C.c_int32)]
+                ("use_graph", C.c_int32), ("gmres_m", C.c_int32)]
 
 
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
                 ("fallbacks", C.c_int32), ("true_resid", C.c_double * 64), ("seconds", C.c_double),
                 ("spmv_launches", C.c_int64 * 2), ("spmv_ms", C.c_double * 2), ("spmv_bytes", C.c_double * 2),
-                ("spmv_ncol", C.c_int32), ("pad_", C.c_int32)]
+                ("spmv_ncol", C.c_int32), ("pad_", C.c_int32),
+                ("factor_seconds", C.c_double), ("factor_flops", C.c_double), ("factor_bytes", C.c_double),
+                ("factorizations", C.c_int32), ("static_pivots", C.c_int32), ("precond_applies", C.c_int32),
+                ("pad2_", C.c_int32)]
 
 
 _L = None
@@ -93,6 +99,7 @@ def lib():
     L.mvf_get_rhs.argtypes = [vp, dp, C.c_int32]
     L.mvf_setup_times.argtypes = [vp, dp]
     L.mvf_assemble.argtypes = [vp, C.c_int32, dp]
+    L.mvf_direct_plan_stats.argtypes = [C.c_int32] * 4 + [ip, C.c_int64, i64p, ip, dp]
     for name in EXPORTS:
         if name not in ("mvf_free", "mvf_last_error", "mvf_default_solver_opts"):
             getattr(L, name).restype = C.c_int

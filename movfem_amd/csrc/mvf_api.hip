@@ -67,6 +67,7 @@ static int ensure_call_state(mvf_handle *h, int64_t call) {
   hipLaunchKernelGGL(k_apply_carry, dim3(1), dim3(256), 0, h->stream, h->d_gne, h->et.ne, h->d_rowptr, h->d_col,
                      h->d_dK0, h->d_dM0, h->d_K, h->d_M, h->d_K32, h->d_M32, sign);
   MVF_HIP(h, hipGetLastError());
+  direct_invalidate(h);
   h->carry_applied = want;
   return 0;
 }
@@ -111,6 +112,7 @@ void mvf_free(mvf_handle *h) {
                   h->d_rcontrib, h->d_K, h->d_M, h->d_K32, h->d_M32, h->d_Ke, h->d_Me, h->d_dK0, h->d_dM0,
                   h->d_x,  h->d_b};
   for (void *p : ptrs) dfree(p);
+  direct_free(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -302,6 +304,7 @@ int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms) {
     if (!rc) rc = assemble_operators32(h, k32, m32);
     dfree(k32); dfree(m32);
   }
+  direct_invalidate(h);
   if (!rc) {
     MVF_HIP(h, hipStreamSynchronize(h->stream));
     float ms = 0.f;
@@ -524,6 +527,8 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
   while (f < nfreq && !rc) {
     int nb = nfreq - f;
     if (h->carry_nonzero && h->calls == 0) nb = 1;
+    // the direct preconditioner is factored for one frequency: one frequency per batch
+    if (o->precond == MVF_PC_DIRECT) nb = 1;
     MVF_TRY(ensure_call_state(h, h->calls));
     int pc = pad_cols(2 * nb);
     double *bb, *xb;
@@ -540,6 +545,12 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
     mvf_solve_info bi;
     memset(&bi, 0, sizeof(bi));
     rc = solve_device(h, om, 2 * nb, pc, o, bb, xb, &bi);
+    info->factor_seconds += bi.factor_seconds;
+    info->factor_flops = bi.factor_flops;
+    info->factor_bytes = bi.factor_bytes;
+    info->factorizations = bi.factorizations;
+    info->static_pivots = std::max(info->static_pivots, bi.static_pivots);
+    info->precond_applies += bi.precond_applies;
     info->iterations += bi.iterations;
     info->refinements += bi.refinements;
     info->fallbacks += bi.fallbacks;

@@ -1,0 +1,1513 @@
+// Sparse direct factorization on the device (the role zmumps plays in the
+// reference, MoVFEM_3DMT.f90:37-55,113-115) used as the preconditioner of an
+// FP64 GMRES on the true operator.
+//
+// Ordering: geometric nested dissection of the element box.  The box is
+// bisected along its longest side until a leaf holds at most kLeafMax
+// elements; every DOF is eliminated at the lowest common ancestor of the
+// leaves of the elements that contain it, which is a valid elimination tree
+// for any matrix assembled from those elements.
+//
+// Fronts: node t eliminates its n1 owned DOFs; its front also holds the n2
+// DOFs of ancestors that couple to its subtree.  The front is kept row-major
+// and split in two bands with leading dimension m = n1 + n2:
+//   top band    T (n1 x m) = [F11 | F12]  lives in the persistent factor store
+//   bottom band B (n2 x m) = [F21 | F22]  lives in a per-level pool
+// Factorization of a front:
+//   P F11 = L U (partial pivoting restricted to the owned rows), applied to
+//   the whole top band: T = [L\U | L^-1 P F12];  then Y = U^-1 (L^-1 P F12) =
+//   F11^-1 F12 overwrites F12, and the Schur complement S = F22 - F21 Y (lower
+//   triangle, S is symmetric because A is) is extend-added into the parent.
+// The factor of a node is its top band [L\U | Y] plus the row permutation.
+// Solve with the factor (block elimination):
+//   forward  y1 = F11^-1 b1,  b2 -= Y^T b1     (F21 F11^-1 = Y^T, F11 = F11^T)
+//   backward x1 = y1 - Y x2
+//
+// The matrix factored is A_pc = A + i*eps*|K_ii| on the diagonal of DOFs with
+// no mass (all-air DOFs), eps = kRegEps * omega * min_i(M_ii/|K_ii|): the air
+// gradient null space of A would otherwise produce zero pivots.  GMRES runs
+// on A itself, so the regularization only changes the preconditioner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "mvf_internal.h"
+
+namespace mvf {
+
+constexpr int kLeafMax = 16;       // elements per leaf box
+constexpr int kNB = 32;            // panel width / triangular block size
+constexpr double kRegEps = 1e-4;   // air regularization relative to the smallest omega*M/K
+constexpr int kTile = 64;          // GEMM tile edge
+
+struct DNode {
+  int64_t n1, n2, m;
+  int64_t fac;   // offset (complex entries) of T in the factor store
+  int64_t pool;  // offset of B in the level pool
+  int64_t idx;   // offset of the front index list (m DOF ids, 0-based)
+  int64_t rel;   // offset of the parent-relative map of the update DOFs (n2 entries)
+  int64_t pos0;  // elimination position of the first owned DOF (also pivot offset)
+  int64_t sv;    // offset (rows) of the node's solve work vector (m rows)
+  int32_t child[2];
+  int32_t parent;
+  int32_t depth;
+};
+
+struct DirectPlan {
+  int T = 0;
+  int maxdepth = 0;
+  std::vector<DNode> nodes;
+  std::vector<std::vector<int32_t>> levels;
+  std::vector<int64_t> lev_off, lev_pool, lev_fac0, lev_fac1, lev_maxn1, lev_maxn2, lev_maxm;
+  int64_t fac_entries = 0, sv_rows = 0, nidx = 0, nrel = 0;
+  double flops = 0.0;
+  double minMK = 1.0, maxK = 0.0, maxM = 0.0;
+  int64_t nne = 0;
+  // device
+  DNode *d_nodes = nullptr;
+  int32_t *d_lev = nullptr, *d_idx = nullptr, *d_fpos = nullptr, *d_rel = nullptr, *d_dst = nullptr;
+  int32_t *d_piv = nullptr, *d_perm = nullptr, *d_pos = nullptr;
+  double2 *d_fac = nullptr;
+  int *d_npert = nullptr;
+  double2 *d_sv = nullptr, *d_zb = nullptr;
+  int sv_cols = 0;
+  // state
+  int factored = 0;
+  double omega = -1.0;
+  int c64 = 0;
+  int npert = 0;
+  double t_symbolic = 0.0, t_factor = 0.0;
+  int64_t factorizations = 0;
+};
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------ symbolic ------
+struct Box {
+  int x0, x1, y0, y1, z0, z1;
+};
+
+struct TreeBuild {
+  std::vector<Box> box;
+  std::vector<int> parent, depth, first;
+  std::vector<std::array<int, 2>> child;
+};
+
+static int tree_rec(TreeBuild &tb, Box b, int depth) {
+  int ex = b.x1 - b.x0, ey = b.y1 - b.y0, ez = b.z1 - b.z0;
+  int64_t ne = (int64_t)ex * ey * ez;
+  int mx = std::max(ex, std::max(ey, ez));
+  std::array<int, 2> ch = {-1, -1};
+  int first = -1;
+  if (ne > kLeafMax && mx >= 2) {
+    int ax = (ex == mx) ? 0 : (ey == mx ? 1 : 2);
+    Box b1 = b, b2 = b;
+    if (ax == 0) { int mid = b.x0 + ex / 2; b1.x1 = mid; b2.x0 = mid; }
+    else if (ax == 1) { int mid = b.y0 + ey / 2; b1.y1 = mid; b2.y0 = mid; }
+    else { int mid = b.z0 + ez / 2; b1.z1 = mid; b2.z0 = mid; }
+    ch[0] = tree_rec(tb, b1, depth + 1);
+    ch[1] = tree_rec(tb, b2, depth + 1);
+    first = tb.first[ch[0]];
+  }
+  int id = (int)tb.box.size();
+  tb.box.push_back(b);
+  tb.parent.push_back(-1);
+  tb.depth.push_back(depth);
+  tb.first.push_back(first < 0 ? id : first);
+  tb.child.push_back(ch);
+  if (ch[0] >= 0) tb.parent[ch[0]] = tb.parent[ch[1]] = id;
+  return id;
+}
+
+struct PlanArrays {
+  std::vector<int32_t> idx, rel, pos, dst;
+};
+
+// Builds the plan from the numbering (gne, ne x me, 1-based, <= 0 removed) and
+// the full CSR pattern.  Host arrays.
+static int build_plan_host(DirectPlan &P, int nx, int ny, int nz, int me, const int32_t *gne, int64_t nne,
+                           const int64_t *rowptr, const int32_t *col, PlanArrays &out, std::string &err) {
+  TreeBuild tb;
+  tree_rec(tb, Box{0, nx, 0, ny, 0, nz}, 0);
+  const int T = (int)tb.box.size();
+  P.T = T;
+  P.nne = nne;
+  int64_t ne = (int64_t)nx * ny * nz;
+  // leaf of each element
+  std::vector<int32_t> leaf(ne, -1);
+  for (int t = 0; t < T; t++) {
+    if (tb.child[t][0] >= 0) continue;
+    const Box &b = tb.box[t];
+    for (int i = b.x0; i < b.x1; i++)
+      for (int j = b.y0; j < b.y1; j++)
+        for (int k = b.z0; k < b.z1; k++) leaf[((int64_t)i * ny + j) * nz + k] = t;
+  }
+  // owner = lowest common ancestor of the leaves of the DOF's elements
+  std::vector<int32_t> own(nne, -1);
+  for (int64_t e = 0; e < ne; e++) {
+    int l = leaf[e];
+    for (int q = 0; q < me; q++) {
+      int32_t g = gne[e * me + q];
+      if (g <= 0) continue;
+      if (g > nne) {
+        err = "DOF number out of range";
+        return MVF_E_INDEX;
+      }
+      int32_t &o = own[g - 1];
+      if (o < 0) {
+        o = l;
+        continue;
+      }
+      int a = o, b = l;
+      while (a != b) {
+        if (tb.depth[a] > tb.depth[b]) a = tb.parent[a];
+        else if (tb.depth[b] > tb.depth[a]) b = tb.parent[b];
+        else { a = tb.parent[a]; b = tb.parent[b]; }
+      }
+      o = a;
+    }
+  }
+  for (int64_t d = 0; d < nne; d++)
+    if (own[d] < 0) {
+      err = "a DOF belongs to no element";
+      return MVF_E_INDEX;
+    }
+  // owned lists in node (postorder) order, DOF-ascending inside a node
+  std::vector<int64_t> optr(T + 1, 0);
+  for (int64_t d = 0; d < nne; d++) optr[own[d] + 1]++;
+  for (int t = 0; t < T; t++) optr[t + 1] += optr[t];
+  std::vector<int32_t> owned(nne), pos(nne);
+  {
+    std::vector<int64_t> fill(optr.begin(), optr.end() - 1);
+    for (int64_t d = 0; d < nne; d++) {
+      int64_t p = fill[own[d]]++;
+      owned[p] = (int32_t)d;
+      pos[d] = (int32_t)p;
+    }
+  }
+  auto is_anc = [&](int a, int t) { return a > t && tb.first[a] <= t; };
+  // update sets, bottom-up
+  std::vector<std::vector<int32_t>> upd(T);
+  std::vector<int32_t> mark(nne, -1);
+  for (int t = 0; t < T; t++) {
+    std::vector<int32_t> L;
+    for (int s = 0; s < 2; s++) {
+      int c = tb.child[t][s];
+      if (c < 0) continue;
+      for (int32_t d : upd[c])
+        if (own[d] != t && mark[d] != t) {
+          mark[d] = t;
+          L.push_back(d);
+        }
+    }
+    for (int64_t p = optr[t]; p < optr[t + 1]; p++) {
+      int32_t d = owned[p];
+      for (int64_t q = rowptr[d]; q < rowptr[d + 1]; q++) {
+        int32_t k = col[q];
+        int a = own[k];
+        if (a != t && is_anc(a, t) && mark[k] != t) {
+          mark[k] = t;
+          L.push_back(k);
+        }
+      }
+    }
+    std::sort(L.begin(), L.end(), [&](int32_t a, int32_t b) { return pos[a] < pos[b]; });
+    upd[t] = std::move(L);
+    // children's update sets are no longer needed once consumed, but rel maps
+    // need them: keep
+  }
+  // node table; factor offsets are laid out level by level (deepest first)
+  P.nodes.assign(T, DNode{});
+  int maxd = 0;
+  for (int t = 0; t < T; t++) maxd = std::max(maxd, tb.depth[t]);
+  P.maxdepth = maxd;
+  P.levels.assign(maxd + 1, {});
+  for (int t = 0; t < T; t++) P.levels[tb.depth[t]].push_back(t);
+  int64_t idxoff = 0, reloff = 0, svoff = 0;
+  for (int t = 0; t < T; t++) {
+    DNode &n = P.nodes[t];
+    n.n1 = optr[t + 1] - optr[t];
+    n.n2 = (int64_t)upd[t].size();
+    n.m = n.n1 + n.n2;
+    n.idx = idxoff;
+    idxoff += n.m;
+    n.rel = reloff;
+    reloff += n.n2;
+    n.pos0 = optr[t];
+    n.sv = svoff;
+    svoff += n.m;
+    n.child[0] = tb.child[t][0];
+    n.child[1] = tb.child[t][1];
+    n.parent = tb.parent[t];
+    n.depth = tb.depth[t];
+    if (n.m > (int64_t)INT32_MAX / 2) {
+      err = "front too large";
+      return MVF_E_UNSUPPORTED;
+    }
+    double a = (double)n.n1, b = (double)n.n2, m = (double)n.m;
+    // complex FMAs: wide LU, Y = U^-1 Z, lower Schur update; 8 real flops each
+    P.flops += 8.0 * (a * a * m / 2.0 - a * a * a / 6.0 + a * a * b / 2.0 + a * b * b / 2.0);
+  }
+  P.nidx = idxoff;
+  P.nrel = reloff;
+  P.sv_rows = svoff;
+  P.lev_off.assign(maxd + 2, 0);
+  P.lev_pool.assign(maxd + 1, 0);
+  P.lev_fac0.assign(maxd + 1, 0);
+  P.lev_fac1.assign(maxd + 1, 0);
+  P.lev_maxn1.assign(maxd + 1, 0);
+  P.lev_maxn2.assign(maxd + 1, 0);
+  P.lev_maxm.assign(maxd + 1, 0);
+  int64_t fac = 0, lo = 0;
+  for (int d = maxd; d >= 0; d--) {
+    P.lev_fac0[d] = fac;
+    int64_t pool = 0;
+    for (int t : P.levels[d]) {
+      DNode &n = P.nodes[t];
+      n.fac = fac;
+      fac += n.n1 * n.m;
+      n.pool = pool;
+      pool += n.n2 * n.m;
+      P.lev_maxn1[d] = std::max(P.lev_maxn1[d], n.n1);
+      P.lev_maxn2[d] = std::max(P.lev_maxn2[d], n.n2);
+      P.lev_maxm[d] = std::max(P.lev_maxm[d], n.m);
+    }
+    P.lev_fac1[d] = fac;
+    P.lev_pool[d] = pool;
+  }
+  for (int d = 0; d <= maxd; d++) {
+    P.lev_off[d] = lo;
+    lo += (int64_t)P.levels[d].size();
+  }
+  P.lev_off[maxd + 1] = lo;
+  P.fac_entries = fac;
+  // front index lists, relative maps, assembly destinations
+  std::vector<int32_t> &idx = out.idx, &rel = out.rel, &dst = out.dst;
+  idx.assign(P.nidx, 0);
+  rel.assign(P.nrel, 0);
+  for (int t = 0; t < T; t++) {
+    const DNode &n = P.nodes[t];
+    for (int64_t j = 0; j < n.n1; j++) idx[n.idx + j] = owned[optr[t] + j];
+    for (int64_t j = 0; j < n.n2; j++) idx[n.idx + n.n1 + j] = upd[t][j];
+  }
+  std::vector<int32_t> lstamp(nne, -1), lval(nne, 0);
+  dst.assign(rowptr[nne], -1);
+  for (int t = 0; t < T; t++) {
+    const DNode &n = P.nodes[t];
+    for (int64_t j = 0; j < n.m; j++) {
+      int32_t d = idx[n.idx + j];
+      lstamp[d] = t;
+      lval[d] = (int32_t)j;
+    }
+    for (int s = 0; s < 2; s++) {
+      int c = n.child[s];
+      if (c < 0) continue;
+      const DNode &cn = P.nodes[c];
+      for (int64_t k = 0; k < cn.n2; k++) {
+        int32_t d = idx[cn.idx + cn.n1 + k];
+        if (lstamp[d] != t) {
+          err = "update DOF missing from the parent front";
+          return MVF_E_INDEX;
+        }
+        rel[cn.rel + k] = lval[d];
+      }
+    }
+    for (int64_t j = 0; j < n.n1; j++) {
+      int32_t d = idx[n.idx + j];
+      for (int64_t q = rowptr[d]; q < rowptr[d + 1]; q++) {
+        int32_t k = col[q];
+        int a = own[k];
+        if (a == t || is_anc(a, t)) {
+          if (lstamp[k] != t) {
+            err = "coupled DOF missing from the front";
+            return MVF_E_INDEX;
+          }
+          int32_t r = lval[k];
+          dst[q] = r >= j ? r : -1;
+        }
+      }
+    }
+  }
+  // check: relative maps increase (lower triangles map to lower triangles)
+  for (int t = 0; t < T; t++) {
+    const DNode &n = P.nodes[t];
+    for (int64_t k = 1; k < n.n2; k++)
+      if (n.parent >= 0 && rel[n.rel + k] <= rel[n.rel + k - 1]) {
+        err = "relative map not increasing";
+        return MVF_E_INDEX;
+      }
+  }
+  out.pos.swap(pos);
+  return 0;
+}
+
+
+// ------------------------------------------------------------- device ------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// c -= a * b
+__device__ __forceinline__ void cfms(double2 &c, double2 a, double2 b) {
+  c.x -= a.x * b.x - a.y * b.y;
+  c.y -= a.x * b.y + a.y * b.x;
+}
+// c += a * b
+__device__ __forceinline__ void cfma(double2 &c, double2 a, double2 b) {
+  c.x += a.x * b.x - a.y * b.y;
+  c.y += a.x * b.y + a.y * b.x;
+}
+// c += conj(a) * b
+__device__ __forceinline__ void cfmac(double2 &c, double2 a, double2 b) {
+  c.x += a.x * b.x + a.y * b.y;
+  c.y += a.x * b.y - a.y * b.x;
+}
+__device__ __forceinline__ double2 cinv(double2 a) {
+  const double d = a.x * a.x + a.y * a.y;
+  return make_double2(a.x / d, -a.y / d);
+}
+
+__global__ void k_fill2(double2 *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+
+// original entries: column j of the front = owned DOF d; every CSR entry (d, k)
+// whose destination row r = dst[p] >= j lands in F(r, j)
+__global__ __launch_bounds__(256) void k_fa_assemble(const DNode *nodes, const int32_t *lev, const int32_t *idx,
+                                                     const int64_t *rowptr, const int32_t *col, const int32_t *dst,
+                                                     const double *K, const double *M, double w, int c64, double reg,
+                                                     double2 *fac, double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t j = (int64_t)blockIdx.x * 4 + wv; j < nd.n1; j += (int64_t)gridDim.x * 4) {
+    const int32_t d = idx[nd.idx + j];
+    const int64_t p1 = rowptr[d + 1];
+    for (int64_t p = rowptr[d] + lane; p < p1; p += 64) {
+      const int32_t r = dst[p];
+      if (r < 0) continue;
+      double vr = K[p], vi = w * M[p];
+      if (c64) {
+        vr = (double)(float)vr;
+        vi = (double)(float)vi;
+      }
+      if (col[p] == d && M[p] == 0.0) vi += reg * fabs(vr);
+      double2 *q = r < nd.n1 ? fac + nd.fac + (int64_t)r * nd.m + j : pool + nd.pool + (int64_t)(r - nd.n1) * nd.m + j;
+      q->x += vr;
+      q->y += vi;
+    }
+  }
+}
+
+// extend-add of child `slot`'s Schur complement (lower triangle of its F22)
+__global__ __launch_bounds__(256) void k_fa_extend(const DNode *nodes, const int32_t *lev, const int32_t *rel, int slot,
+                                                   double2 *fac, double2 *pool, const double2 *cpool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int c = slot ? nd.child[1] : nd.child[0];
+  if (c < 0) return;
+  const DNode cn = nodes[c];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < cn.n2; i += (int64_t)gridDim.x * 4) {
+    const int64_t pi = rel[cn.rel + i];
+    const double2 *src = cpool + cn.pool + i * cn.m + cn.n1;
+    double2 *drow = pi < nd.n1 ? fac + nd.fac + pi * nd.m : pool + nd.pool + (pi - nd.n1) * nd.m;
+    for (int64_t j = lane; j <= i; j += 64) {
+      const int32_t pj = rel[cn.rel + j];
+      const double2 v = src[j];
+      drow[pj].x += v.x;
+      drow[pj].y += v.y;
+    }
+  }
+}
+
+// upper part of the top band from the lower parts: T(i, j) = F(j, i), j > i
+__global__ __launch_bounds__(256) void k_fa_sym(const DNode *nodes, const int32_t *lev, double2 *fac,
+                                                const double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t tj_n = (nd.m + 31) / 32, ti_n = (nd.n1 + 31) / 32;
+  __shared__ double2 s[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  double2 *T = fac + nd.fac;
+  const double2 *B = pool + nd.pool;
+  for (int64_t tile = blockIdx.x; tile < ti_n * tj_n; tile += gridDim.x) {
+    const int64_t ti = tile / tj_n, tj = tile % tj_n;
+    if (tj < ti) continue;
+    const int64_t i0 = ti * 32, j0 = tj * 32;
+    for (int jj = ty; jj < 32; jj += 8) {
+      const int64_t j = j0 + jj, i = i0 + tx;
+      double2 v = make_double2(0.0, 0.0);
+      if (j < nd.m && i < nd.n1) v = j < nd.n1 ? T[j * nd.m + i] : B[(j - nd.n1) * nd.m + i];
+      s[jj][tx] = v;
+    }
+    __syncthreads();
+    for (int ii = ty; ii < 32; ii += 8) {
+      const int64_t i = i0 + ii, j = j0 + tx;
+      if (i < nd.n1 && j < nd.m && j > i) T[i * nd.m + j] = s[tx][ii];
+    }
+    __syncthreads();
+  }
+}
+
+// unblocked LU with partial pivoting of panel columns [k, k+NB) over the owned
+// rows [k, n1); one workgroup per front
+template <int NB>
+__global__ __launch_bounds__(256) void k_fa_panel(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
+                                                  int32_t *piv, int *npert, double big, double tiny) {
+  const DNode nd = nodes[lev[blockIdx.x]];
+  if (k >= nd.n1) return;
+  const int64_t n1 = nd.n1, m = nd.m, kend = (k + NB < n1) ? k + NB : n1;
+  double2 *T = fac + nd.fac;
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  __shared__ double2 prow[NB];
+  const int tid = threadIdx.x;
+  for (int64_t q = k; q < kend; q++) {
+    double best = -1.0;
+    int64_t bi = q;
+    for (int64_t i = q + tid; i < n1; i += 256) {
+      const double2 v = T[i * m + q];
+      const double a = v.x * v.x + v.y * v.y;
+      if (a > best) {
+        best = a;
+        bi = i;
+      }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        const double o = sv[tid + s];
+        const int64_t oi = si[tid + s];
+        if (o > sv[tid] || (o == sv[tid] && oi < si[tid])) {
+          sv[tid] = o;
+          si[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    const int64_t r = si[0];
+    const double bv = sv[0];
+    __syncthreads();
+    if (r != q)
+      for (int64_t c = k + tid; c < kend; c += 256) {
+        const double2 a = T[q * m + c];
+        T[q * m + c] = T[r * m + c];
+        T[r * m + c] = a;
+      }
+    if (tid == 0) piv[nd.pos0 + q] = (int32_t)r;
+    __syncthreads();
+    if (tid == 0 && !(bv > tiny * tiny)) {
+      T[q * m + q] = make_double2(big, 0.0);
+      atomicAdd(npert, 1);
+    }
+    __syncthreads();
+    const double2 inv = cinv(T[q * m + q]);
+    for (int64_t c = q + 1 + tid; c < kend; c += 256) prow[c - k] = T[q * m + c];
+    __syncthreads();
+    for (int64_t i = q + 1 + tid; i < n1; i += 256) {
+      const double2 l = cmul(T[i * m + q], inv);
+      T[i * m + q] = l;
+      for (int64_t c = q + 1; c < kend; c++) cfms(T[i * m + c], l, prow[c - k]);
+    }
+    __syncthreads();
+  }
+}
+
+// the panel's row interchanges applied to the top band outside the panel
+template <int NB>
+__global__ __launch_bounds__(256) void k_fa_swap(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
+                                                 const int32_t *piv) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  if (k >= nd.n1) return;
+  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
+  double2 *T = fac + nd.fac;
+  for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += (int64_t)gridDim.x * 256) {
+    if (j >= k && j < kend) continue;
+    for (int64_t q = k; q < kend; q++) {
+      const int64_t r = piv[nd.pos0 + q];
+      if (r != q) {
+        const double2 a = T[q * m + j];
+        T[q * m + j] = T[r * m + j];
+        T[r * m + j] = a;
+      }
+    }
+  }
+}
+
+// U row block: T[k:kend, kend:m] = L_kk^-1 T[k:kend, kend:m] (unit lower).
+// Each of the 128 threads owns one column, held in LDS.
+template <int NB>
+__global__ __launch_bounds__(128) void k_fa_trsm_u(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  if (k >= nd.n1) return;
+  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
+  if (kend >= m) return;
+  const int w = (int)(kend - k);
+  double2 *T = fac + nd.fac;
+  __shared__ double2 L[NB][NB + 1];
+  __shared__ double2 xs[NB][129];
+  for (int e = threadIdx.x; e < NB * NB; e += 128) {
+    const int r = e / NB, c = e % NB;
+    L[r][c] = (r < w && c < w) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  for (int64_t j0 = kend + (int64_t)blockIdx.x * 128; j0 < m; j0 += (int64_t)gridDim.x * 128) {
+    const int64_t j = j0 + t;
+    if (j < m) {
+      for (int r = 0; r < w; r++) xs[r][t] = T[(k + r) * m + j];
+      for (int r = 1; r < w; r++) {
+        double2 v = xs[r][t];
+        for (int c = 0; c < r; c++) cfms(v, L[r][c], xs[c][t]);
+        xs[r][t] = v;
+      }
+      for (int r = 0; r < w; r++) T[(k + r) * m + j] = xs[r][t];
+    }
+  }
+}
+
+// Y block rows from the bottom: T[r0:r1, n1:m] = U_bb^-1 T[r0:r1, n1:m]
+template <int NB>
+__global__ __launch_bounds__(128) void k_fa_trsm_y(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                   double2 *fac) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t r1 = nd.n1 - step * NB;
+  if (r1 <= 0 || nd.n2 == 0) return;
+  const int64_t r0 = r1 > NB ? r1 - NB : 0, m = nd.m;
+  const int w = (int)(r1 - r0);
+  double2 *T = fac + nd.fac;
+  __shared__ double2 U[NB][NB + 1];
+  __shared__ double2 dinv[NB];
+  __shared__ double2 xs[NB][129];
+  for (int e = threadIdx.x; e < NB * NB; e += 128) {
+    const int r = e / NB, c = e % NB;
+    U[r][c] = (r < w && c < w && c >= r) ? T[(r0 + r) * m + r0 + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
+  __syncthreads();
+  const int t = threadIdx.x;
+  for (int64_t j0 = nd.n1 + (int64_t)blockIdx.x * 128; j0 < m; j0 += (int64_t)gridDim.x * 128) {
+    const int64_t j = j0 + t;
+    if (j < m) {
+      for (int r = 0; r < w; r++) xs[r][t] = T[(r0 + r) * m + j];
+      for (int r = w - 1; r >= 0; r--) {
+        double2 v = xs[r][t];
+        for (int c = r + 1; c < w; c++) cfms(v, U[r][c], xs[c][t]);
+        xs[r][t] = cmul(v, dinv[r]);
+      }
+      for (int r = 0; r < w; r++) T[(r0 + r) * m + j] = xs[r][t];
+    }
+  }
+}
+
+// batched complex GEMM update C -= A B on 64 x 64 tiles, one front per blockIdx.y
+//   MODE 0: LU trailing update of the top band after panel [k0, k1)
+//   MODE 1: Y solve, rows above the solved block rows [k0, k1)
+//   MODE 2: Schur complement F22 -= F21 Y (lower tiles only)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fa_gemm(const DNode *nodes, const int32_t *lev, int64_t step, double2 *fac,
+                                                 double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
+  double2 *T = fac + nd.fac;
+  const double2 *A, *B;
+  double2 *C;
+  int64_t Mr, Nc, Kd;
+  bool lower = false;
+  if (MODE == 0) {
+    const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
+    if (k0 >= n1) return;
+    A = T + k1 * m + k0;
+    B = T + k0 * m + k1;
+    C = T + k1 * m + k1;
+    Mr = n1 - k1;
+    Nc = m - k1;
+    Kd = k1 - k0;
+  } else if (MODE == 1) {
+    const int64_t r1 = n1 - step * kNB;
+    if (r1 <= 0) return;
+    const int64_t r0 = r1 > kNB ? r1 - kNB : 0;
+    A = T + r0;
+    B = T + r0 * m + n1;
+    C = T + n1;
+    Mr = r0;
+    Nc = n2;
+    Kd = r1 - r0;
+  } else {
+    double2 *Bb = pool + nd.pool;
+    A = Bb;
+    B = T + n1;
+    C = Bb + n1;
+    Mr = n2;
+    Nc = n2;
+    Kd = n1;
+    lower = true;
+  }
+  if (Mr <= 0 || Nc <= 0 || Kd <= 0) return;
+  const int64_t tn = (Nc + kTile - 1) / kTile, tm = (Mr + kTile - 1) / kTile;
+  __shared__ double2 As[kTile][17];
+  __shared__ double2 Bs[16][kTile + 1];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  for (int64_t tile = blockIdx.x; tile < tm * tn; tile += gridDim.x) {
+  const int64_t ti = tile / tn, tj = tile % tn;
+  if (lower && tj > ti) continue;
+  const int64_t i0 = ti * kTile, j0 = tj * kTile;
+  double2 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = make_double2(0.0, 0.0);
+  for (int64_t kk = 0; kk < Kd; kk += 16) {
+#pragma unroll
+    for (int e = tid; e < kTile * 16; e += 256) {
+      const int r = e >> 4, q = e & 15;
+      As[r][q] = (i0 + r < Mr && kk + q < Kd) ? A[(i0 + r) * m + kk + q] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int e = tid; e < kTile * 16; e += 256) {
+      const int q = e >> 6, c = e & 63;
+      Bs[q][c] = (kk + q < Kd && j0 + c < Nc) ? B[(kk + q) * m + j0 + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      double2 a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) a[r] = As[ty * 4 + r][q];
+#pragma unroll
+      for (int c = 0; c < 4; c++) b[c] = Bs[q][tx * 4 + c];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) cfma(acc[r][c], a[r], b[c]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int64_t i = i0 + ty * 4 + r;
+    if (i >= Mr) continue;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int64_t j = j0 + tx * 4 + c;
+      if (j < Nc) {
+        double2 &o = C[i * m + j];
+        o.x -= acc[r][c].x;
+        o.y -= acc[r][c].y;
+      }
+    }
+  }
+  }
+}
+
+// row permutation of each front from its interchange sequence
+__global__ void k_fa_perm(const DNode *nodes, const int32_t *lev, int nl, const int32_t *piv, int32_t *perm) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  const DNode nd = nodes[lev[t]];
+  int32_t *p = perm + nd.pos0;
+  for (int64_t q = 0; q < nd.n1; q++) p[q] = (int32_t)q;
+  for (int64_t q = 0; q < nd.n1; q++) {
+    const int64_t r = piv[nd.pos0 + q];
+    const int32_t a = p[q];
+    p[q] = p[r];
+    p[r] = a;
+  }
+}
+
+// ----------------------------------------------------------- solve kernels --
+// vectors: [row][NC] complex; front work vectors W at rows nd.sv .. nd.sv+m
+template <int NC>
+__global__ __launch_bounds__(256) void k_sv_gather(const DNode *nodes, const int32_t *lev, const int32_t *idx,
+                                                   const double2 *b, double2 *W) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nd.m * NC; e += (int64_t)gridDim.x * 256) {
+    const int64_t r = e / NC;
+    const int c = (int)(e % NC);
+    W[(nd.sv + r) * NC + c] = r < nd.n1 ? b[(int64_t)idx[nd.idx + r] * NC + c] : make_double2(0.0, 0.0);
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_sv_extend(const DNode *nodes, const int32_t *lev, const int32_t *rel, int slot,
+                                                   double2 *W) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int ch = slot ? nd.child[1] : nd.child[0];
+  if (ch < 0) return;
+  const DNode cn = nodes[ch];
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < cn.n2 * NC; e += (int64_t)gridDim.x * 256) {
+    const int64_t k = e / NC;
+    const int c = (int)(e % NC);
+    const double2 v = W[(cn.sv + cn.n1 + k) * NC + c];
+    double2 &o = W[(nd.sv + rel[cn.rel + k]) * NC + c];
+    o.x += v.x;
+    o.y += v.y;
+  }
+}
+
+// W2[j] -= sum_i Y[i][j] W1[i]
+template <int NC>
+__global__ __launch_bounds__(256) void k_sv_contrib(const DNode *nodes, const int32_t *lev, const double2 *fac,
+                                                    double2 *W) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  if (nd.n1 == 0) return;
+  const double2 *T = fac + nd.fac;
+  __shared__ double2 s[256][NC];
+  for (int64_t jb = blockIdx.x; jb * 256 < nd.n2; jb += gridDim.x) {
+    const int64_t j = jb * 256 + threadIdx.x;
+    double2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = make_double2(0.0, 0.0);
+    for (int64_t i0 = 0; i0 < nd.n1; i0 += 256) {
+      for (int e = threadIdx.x; e < 256 * NC; e += 256) {
+        const int r = e / NC, c = e % NC;
+        s[r][c] = i0 + r < nd.n1 ? W[(nd.sv + i0 + r) * NC + c] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      if (j < nd.n2) {
+        const int64_t lim = (nd.n1 - i0) < 256 ? nd.n1 - i0 : 256;
+        for (int64_t r = 0; r < lim; r++) {
+          const double2 y = T[(i0 + r) * nd.m + nd.n1 + j];
+#pragma unroll
+          for (int c = 0; c < NC; c++) cfma(acc[c], y, s[r][c]);
+        }
+      }
+      __syncthreads();
+    }
+    if (j < nd.n2)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double2 &o = W[(nd.sv + nd.n1 + j) * NC + c];
+        o.x -= acc[c].x;
+        o.y -= acc[c].y;
+      }
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_sv_perm(const DNode *nodes, const int32_t *lev, const int32_t *perm,
+                                                 const double2 *W, double2 *zb) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nd.n1 * NC; e += (int64_t)gridDim.x * 256) {
+    const int64_t q = e / NC;
+    const int c = (int)(e % NC);
+    zb[(nd.pos0 + q) * NC + c] = W[(nd.sv + perm[nd.pos0 + q]) * NC + c];
+  }
+}
+
+// forward substitution (unit lower L) inside block rows [b*NB, b*NB+NB)
+template <int NC, int NB>
+__global__ __launch_bounds__(64) void k_sv_ldiag(const DNode *nodes, const int32_t *lev, int64_t b, const double2 *fac,
+                                                 double2 *zb) {
+  const DNode nd = nodes[lev[blockIdx.x]];
+  const int64_t b0 = b * NB;
+  if (b0 >= nd.n1) return;
+  const int w = (int)((b0 + NB < nd.n1 ? b0 + NB : nd.n1) - b0);
+  const double2 *T = fac + nd.fac;
+  __shared__ double2 L[NB][NB + 1];
+  __shared__ double2 z[NB][NC];
+  for (int e = threadIdx.x; e < NB * NB; e += 64) {
+    const int r = e / NB, c = e % NB;
+    L[r][c] = (r < w && c < r) ? T[(b0 + r) * nd.m + b0 + c] : make_double2(0.0, 0.0);
+  }
+  for (int e = threadIdx.x; e < NB * NC; e += 64) {
+    const int r = e / NC, c = e % NC;
+    z[r][c] = r < w ? zb[(nd.pos0 + b0 + r) * NC + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const int c = threadIdx.x;
+    for (int r = 1; r < w; r++) {
+      double2 v = z[r][c];
+      for (int q = 0; q < r; q++) cfms(v, L[r][q], z[q][c]);
+      z[r][c] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < w * NC; e += 64) {
+    const int r = e / NC, c = e % NC;
+    zb[(nd.pos0 + b0 + r) * NC + c] = z[r][c];
+  }
+}
+
+// rows below the block: z[r] -= L[r][b0:b1] z[b0:b1]
+template <int NC, int NB>
+__global__ __launch_bounds__(256) void k_sv_lupd(const DNode *nodes, const int32_t *lev, int64_t b, const double2 *fac,
+                                                 double2 *zb) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t b0 = b * NB;
+  if (b0 >= nd.n1) return;
+  const int64_t b1 = b0 + NB < nd.n1 ? b0 + NB : nd.n1;
+  const int w = (int)(b1 - b0);
+  if (b1 + (int64_t)blockIdx.x * 256 >= nd.n1) return;
+  const double2 *T = fac + nd.fac;
+  __shared__ double2 z[NB][NC];
+  for (int e = threadIdx.x; e < NB * NC; e += 256) {
+    const int r = e / NC, c = e % NC;
+    z[r][c] = r < w ? zb[(nd.pos0 + b0 + r) * NC + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int64_t r = b1 + (int64_t)blockIdx.x * 256 + threadIdx.x; r < nd.n1; r += (int64_t)gridDim.x * 256) {
+    double2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = zb[(nd.pos0 + r) * NC + c];
+    const double2 *Lr = T + r * nd.m + b0;
+    for (int q = 0; q < w; q++) {
+      const double2 l = Lr[q];
+#pragma unroll
+      for (int c = 0; c < NC; c++) cfms(acc[c], l, z[q][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) zb[(nd.pos0 + r) * NC + c] = acc[c];
+  }
+}
+
+// back substitution (upper U) inside block rows [r0, r1), r1 = n1 - step*NB
+template <int NC, int NB>
+__global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                 const double2 *fac, double2 *zb) {
+  const DNode nd = nodes[lev[blockIdx.x]];
+  const int64_t r1 = nd.n1 - step * NB;
+  if (r1 <= 0) return;
+  const int64_t r0 = r1 > NB ? r1 - NB : 0;
+  const int w = (int)(r1 - r0);
+  const double2 *T = fac + nd.fac;
+  __shared__ double2 U[NB][NB + 1];
+  __shared__ double2 z[NB][NC];
+  for (int e = threadIdx.x; e < NB * NB; e += 64) {
+    const int r = e / NB, c = e % NB;
+    U[r][c] = (r < w && c < w && c >= r) ? T[(r0 + r) * nd.m + r0 + c] : make_double2(0.0, 0.0);
+  }
+  for (int e = threadIdx.x; e < NB * NC; e += 64) {
+    const int r = e / NC, c = e % NC;
+    z[r][c] = r < w ? zb[(nd.pos0 + r0 + r) * NC + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const int c = threadIdx.x;
+    for (int r = w - 1; r >= 0; r--) {
+      double2 v = z[r][c];
+      for (int q = r + 1; q < w; q++) cfms(v, U[r][q], z[q][c]);
+      z[r][c] = cmul(v, cinv(U[r][r]));
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < w * NC; e += 64) {
+    const int r = e / NC, c = e % NC;
+    zb[(nd.pos0 + r0 + r) * NC + c] = z[r][c];
+  }
+}
+
+// rows above the block: z[r] -= U[r][r0:r1] z[r0:r1]
+template <int NC, int NB>
+__global__ __launch_bounds__(256) void k_sv_uupd(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                 const double2 *fac, double2 *zb) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t r1 = nd.n1 - step * NB;
+  if (r1 <= 0) return;
+  const int64_t r0 = r1 > NB ? r1 - NB : 0;
+  const int w = (int)(r1 - r0);
+  if ((int64_t)blockIdx.x * 256 >= r0) return;
+  const double2 *T = fac + nd.fac;
+  __shared__ double2 z[NB][NC];
+  for (int e = threadIdx.x; e < NB * NC; e += 256) {
+    const int r = e / NC, c = e % NC;
+    z[r][c] = r < w ? zb[(nd.pos0 + r0 + r) * NC + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < r0; r += (int64_t)gridDim.x * 256) {
+    double2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = zb[(nd.pos0 + r) * NC + c];
+    const double2 *Ur = T + r * nd.m + r0;
+    for (int q = 0; q < w; q++) {
+      const double2 u = Ur[q];
+#pragma unroll
+      for (int c = 0; c < NC; c++) cfms(acc[c], u, z[q][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) zb[(nd.pos0 + r) * NC + c] = acc[c];
+  }
+}
+
+// backward: x1[i] = y1[i] - sum_j Y[i][j] x2[j]; one wave per row
+template <int NC>
+__global__ __launch_bounds__(256) void k_sv_back(const DNode *nodes, const int32_t *lev, const int32_t *fpos,
+                                                 const double2 *fac, double2 *zb) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  if (nd.n2 == 0) return;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double2 *T = fac + nd.fac;
+  for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nd.n1; i += (int64_t)gridDim.x * 4) {
+    double2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = make_double2(0.0, 0.0);
+    const double2 *Yr = T + i * nd.m + nd.n1;
+    for (int64_t j = lane; j < nd.n2; j += 64) {
+      const double2 y = Yr[j];
+      const int64_t pj = fpos[nd.idx + nd.n1 + j];
+#pragma unroll
+      for (int c = 0; c < NC; c++) cfma(acc[c], y, zb[pj * NC + c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+      for (int off = 32; off > 0; off >>= 1) {
+        acc[c].x += __shfl_xor(acc[c].x, off, 64);
+        acc[c].y += __shfl_xor(acc[c].y, off, 64);
+      }
+    if (lane < NC) {
+      double2 &o = zb[(nd.pos0 + i) * NC + lane];
+      o.x -= acc[lane].x;
+      o.y -= acc[lane].y;
+    }
+  }
+}
+
+template <int NC>
+__global__ void k_sv_out(int64_t n, const int32_t *pos, const double2 *zb, double2 *x) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * NC; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = e / NC;
+    const int c = (int)(e % NC);
+    x[e] = zb[(int64_t)pos[d] * NC + c];
+  }
+}
+
+// ----------------------------------------------------------- host driver ----
+static unsigned cap_grid(int64_t g) { return (unsigned)(g < 1 ? 1 : (g > 65535 * 16 ? 65535 * 16 : g)); }
+
+void direct_free(mvf_handle *h) {
+  DirectPlan *P = (DirectPlan *)h->direct;
+  if (!P) return;
+  dfree(P->d_nodes); dfree(P->d_lev); dfree(P->d_idx); dfree(P->d_fpos); dfree(P->d_rel); dfree(P->d_dst);
+  dfree(P->d_piv); dfree(P->d_perm); dfree(P->d_pos); dfree(P->d_fac); dfree(P->d_npert); dfree(P->d_sv);
+  dfree(P->d_zb);
+  delete P;
+  h->direct = nullptr;
+}
+
+void direct_invalidate(mvf_handle *h) {
+  if (h->direct) ((DirectPlan *)h->direct)->factored = 0;
+}
+
+static int direct_plan(mvf_handle *h) {
+  if (h->direct) return 0;
+  double t0 = now_s();
+  const int64_t nne = h->nne, nnz = h->nnz, ne = h->geo.ne;
+  const int me = h->et.ne;
+  std::vector<int32_t> gne(ne * me), col(nnz);
+  std::vector<int64_t> rowptr(nne + 1), diagpos(nne);
+  MVF_HIP(h, hipMemcpy(gne.data(), h->d_gne, gne.size() * 4, hipMemcpyDeviceToHost));
+  MVF_HIP(h, hipMemcpy(col.data(), h->d_col, col.size() * 4, hipMemcpyDeviceToHost));
+  MVF_HIP(h, hipMemcpy(rowptr.data(), h->d_rowptr, rowptr.size() * 8, hipMemcpyDeviceToHost));
+  MVF_HIP(h, hipMemcpy(diagpos.data(), h->d_diagpos, diagpos.size() * 8, hipMemcpyDeviceToHost));
+  DirectPlan *P = new DirectPlan();
+  PlanArrays A;
+  std::string err;
+  int rc = build_plan_host(*P, h->geo.nx, h->geo.ny, h->geo.nz, me, gne.data(), nne, rowptr.data(), col.data(), A, err);
+  if (rc) {
+    delete P;
+    return set_err(h, rc, "direct plan: %s", err.c_str());
+  }
+  for (int d = 0; d <= P->maxdepth; d++)
+    if (P->levels[d].size() > 65535) {
+      delete P;
+      return set_err(h, MVF_E_UNSUPPORTED, "direct plan: %zu fronts in one level", P->levels[d].size());
+    }
+  h->direct = P;
+  // smallest M/|K| over DOFs with mass, largest diagonal magnitudes
+  {
+    std::vector<double> K(nnz), M(nnz);
+    MVF_HIP(h, hipMemcpy(K.data(), h->d_K, nnz * 8, hipMemcpyDeviceToHost));
+    MVF_HIP(h, hipMemcpy(M.data(), h->d_M, nnz * 8, hipMemcpyDeviceToHost));
+    double mn = INFINITY;
+    P->maxK = 0.0;
+    P->maxM = 0.0;
+    for (int64_t d = 0; d < nne; d++) {
+      int64_t p = diagpos[d];
+      if (p < 0) continue;
+      double k = fabs(K[p]), m = M[p];
+      P->maxK = std::max(P->maxK, k);
+      P->maxM = std::max(P->maxM, fabs(m));
+      if (m > 0.0 && k > 0.0) mn = std::min(mn, m / k);
+    }
+    P->minMK = std::isfinite(mn) ? mn : 1.0;
+  }
+  std::vector<int32_t> lev(P->lev_off[P->maxdepth + 1]);
+  for (int d = 0; d <= P->maxdepth; d++)
+    std::copy(P->levels[d].begin(), P->levels[d].end(), lev.begin() + P->lev_off[d]);
+  std::vector<int32_t> fpos(P->nidx);
+  for (int64_t i = 0; i < P->nidx; i++) fpos[i] = A.pos[A.idx[i]];
+  MVF_TRY(dalloc(h, &P->d_nodes, P->T));
+  MVF_TRY(dalloc(h, &P->d_lev, lev.size()));
+  MVF_TRY(dalloc(h, &P->d_idx, P->nidx));
+  MVF_TRY(dalloc(h, &P->d_fpos, P->nidx));
+  MVF_TRY(dalloc(h, &P->d_rel, P->nrel));
+  MVF_TRY(dalloc(h, &P->d_dst, nnz));
+  MVF_TRY(dalloc(h, &P->d_pos, nne));
+  MVF_TRY(dalloc(h, &P->d_piv, nne));
+  MVF_TRY(dalloc(h, &P->d_perm, nne));
+  MVF_TRY(dalloc(h, &P->d_npert, 1));
+  MVF_TRY(dalloc(h, &P->d_fac, P->fac_entries));
+  MVF_HIP(h, hipMemcpy(P->d_nodes, P->nodes.data(), sizeof(DNode) * P->T, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(P->d_lev, lev.data(), lev.size() * 4, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(P->d_idx, A.idx.data(), A.idx.size() * 4, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(P->d_fpos, fpos.data(), fpos.size() * 4, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(P->d_rel, A.rel.data(), A.rel.size() * 4, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(P->d_dst, A.dst.data(), A.dst.size() * 4, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(P->d_pos, A.pos.data(), A.pos.size() * 4, hipMemcpyHostToDevice));
+  P->t_symbolic = now_s() - t0;
+  return 0;
+}
+
+// numeric factorization of A_pc at angular frequency w (fl32(omega) as in the operator)
+static int direct_factor(mvf_handle *h, double w, int c64) {
+  DirectPlan *P = (DirectPlan *)h->direct;
+  hipStream_t st = h->stream;
+  double t0 = now_s();
+  const double reg = kRegEps * w * P->minMK;
+  const double big = P->maxK + w * P->maxM;
+  const double tiny = 1e-14 * big;
+  MVF_HIP(h, hipMemsetAsync(P->d_npert, 0, sizeof(int), st));
+  double2 *prev = nullptr;
+  for (int d = P->maxdepth; d >= 0; d--) {
+    const int nl = (int)P->levels[d].size();
+    const int32_t *lev = P->d_lev + P->lev_off[d];
+    double2 *pool = nullptr;
+    MVF_TRY(dalloc(h, &pool, P->lev_pool[d]));
+    const int64_t nfac = P->lev_fac1[d] - P->lev_fac0[d];
+    hipLaunchKernelGGL(k_fill2, dim3(cap_grid((nfac + 255) / 256)), dim3(256), 0, st, P->d_fac + P->lev_fac0[d], nfac);
+    hipLaunchKernelGGL(k_fill2, dim3(cap_grid((P->lev_pool[d] + 255) / 256)), dim3(256), 0, st, pool, P->lev_pool[d]);
+    const int64_t mn1 = P->lev_maxn1[d], mn2 = P->lev_maxn2[d], mm = P->lev_maxm[d];
+    hipLaunchKernelGGL(k_fa_assemble, dim3(cap_grid((mn1 + 3) / 4), nl), dim3(256), 0, st, P->d_nodes, lev, P->d_idx,
+                       h->d_rowptr, h->d_col, P->d_dst, h->d_K, h->d_M, w, c64, reg, P->d_fac, pool);
+    if (d < P->maxdepth) {
+      int64_t cn2 = P->lev_maxn2[d + 1];
+      for (int s = 0; s < 2; s++)
+        hipLaunchKernelGGL(k_fa_extend, dim3(cap_grid((cn2 + 3) / 4), nl), dim3(256), 0, st, P->d_nodes, lev,
+                           P->d_rel, s, P->d_fac, pool, prev);
+    }
+    MVF_HIP(h, hipGetLastError());
+    if (prev) {
+      MVF_HIP(h, hipStreamSynchronize(st));
+      dfree(prev);
+      prev = nullptr;
+    }
+    if (mn1 > 0) {
+      hipLaunchKernelGGL(k_fa_sym, dim3(cap_grid(((mn1 + 31) / 32) * ((mm + 31) / 32)), nl), dim3(256), 0, st,
+                         P->d_nodes, lev, P->d_fac, pool);
+      for (int64_t k = 0; k < mn1; k += kNB) {
+        int64_t gt = 0, gc = 0;
+        for (int t : P->levels[d]) {
+          const DNode &n = P->nodes[t];
+          if (k >= n.n1) continue;
+          const int64_t k1 = std::min(k + kNB, n.n1);
+          gt = std::max(gt, ((n.n1 - k1 + kTile - 1) / kTile) * ((n.m - k1 + kTile - 1) / kTile));
+          gc = std::max(gc, n.m);
+        }
+        hipLaunchKernelGGL((k_fa_panel<kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_piv,
+                           P->d_npert, big, tiny);
+        hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
+                           k, P->d_fac, P->d_piv);
+        hipLaunchKernelGGL((k_fa_trsm_u<kNB>), dim3(cap_grid((gc - k + 127) / 128), nl), dim3(128), 0, st,
+                           P->d_nodes, lev, k, P->d_fac);
+        if (gt > 0)
+          hipLaunchKernelGGL((k_fa_gemm<0>), dim3(cap_grid(gt), nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac,
+                             pool);
+      }
+      if (mn2 > 0) {
+        const int64_t nsteps = (mn1 + kNB - 1) / kNB;
+        for (int64_t s = 0; s < nsteps; s++) {
+          int64_t gt = 0;
+          for (int t : P->levels[d]) {
+            const DNode &n = P->nodes[t];
+            const int64_t r1 = n.n1 - s * kNB;
+            if (r1 <= 0 || n.n2 == 0) continue;
+            const int64_t r0 = std::max<int64_t>(0, r1 - kNB);
+            gt = std::max(gt, ((r0 + kTile - 1) / kTile) * ((n.n2 + kTile - 1) / kTile));
+          }
+          hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + 127) / 128), nl), dim3(128), 0, st, P->d_nodes,
+                             lev, s, P->d_fac);
+          if (gt > 0)
+            hipLaunchKernelGGL((k_fa_gemm<1>), dim3(cap_grid(gt), nl), dim3(256), 0, st, P->d_nodes, lev, s,
+                               P->d_fac, pool);
+        }
+        const int64_t tq = (mn2 + kTile - 1) / kTile;
+        hipLaunchKernelGGL((k_fa_gemm<2>), dim3(cap_grid(tq * tq), nl), dim3(256), 0, st, P->d_nodes, lev, 0,
+                           P->d_fac, pool);
+      }
+      hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
+    }
+    MVF_HIP(h, hipGetLastError());
+    prev = pool;
+    if (h->verbose) {
+      MVF_HIP(h, hipStreamSynchronize(st));
+      double lf = 0.0;
+      for (int t : P->levels[d]) {
+        const DNode &n = P->nodes[t];
+        const double a = (double)n.n1, b = (double)n.n2, mm2 = (double)n.m;
+        lf += 8.0 * (a * a * mm2 / 2.0 - a * a * a / 6.0 + a * a * b / 2.0 + a * b * b / 2.0);
+      }
+      const double tl = now_s() - t0;
+      fprintf(stderr, "[direct] level %2d fronts %6d max n1 %6lld n2 %6lld  %.3e flop  t=%.3f s\n", d, nl,
+              (long long)mn1, (long long)mn2, lf, tl);
+    }
+  }
+  MVF_HIP(h, hipStreamSynchronize(st));
+  dfree(prev);
+  int np = 0;
+  MVF_HIP(h, hipMemcpy(&np, P->d_npert, sizeof(int), hipMemcpyDeviceToHost));
+  P->npert = np;
+  P->factored = 1;
+  P->omega = w;
+  P->c64 = c64;
+  P->factorizations++;
+  P->t_factor = now_s() - t0;
+  return 0;
+}
+
+// out = A_pc^-1 in, NC columns ([nne][NC] complex, device)
+template <int NC>
+static int direct_apply(mvf_handle *h, const double2 *in, double2 *out) {
+  DirectPlan *P = (DirectPlan *)h->direct;
+  hipStream_t st = h->stream;
+  if (P->sv_cols < NC) {
+    dfree(P->d_sv);
+    dfree(P->d_zb);
+    P->d_sv = P->d_zb = nullptr;
+    MVF_TRY(dalloc(h, &P->d_sv, P->sv_rows * NC));
+    MVF_TRY(dalloc(h, &P->d_zb, P->nne * NC));
+    P->sv_cols = NC;
+  }
+  double2 *W = P->d_sv, *zb = P->d_zb;
+  for (int d = P->maxdepth; d >= 0; d--) {
+    const int nl = (int)P->levels[d].size();
+    const int32_t *lev = P->d_lev + P->lev_off[d];
+    const int64_t mn1 = P->lev_maxn1[d], mn2 = P->lev_maxn2[d], mm = P->lev_maxm[d];
+    hipLaunchKernelGGL((k_sv_gather<NC>), dim3(cap_grid((mm * NC + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
+                       P->d_idx, in, W);
+    if (d < P->maxdepth) {
+      const int64_t cn2 = P->lev_maxn2[d + 1];
+      for (int s = 0; s < 2; s++)
+        hipLaunchKernelGGL((k_sv_extend<NC>), dim3(cap_grid((cn2 * NC + 255) / 256), nl), dim3(256), 0, st,
+                           P->d_nodes, lev, P->d_rel, s, W);
+    }
+    if (mn1 == 0) continue;
+    if (mn2 > 0)
+      hipLaunchKernelGGL((k_sv_contrib<NC>), dim3(cap_grid((mn2 + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
+                         P->d_fac, W);
+    hipLaunchKernelGGL((k_sv_perm<NC>), dim3(cap_grid((mn1 * NC + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
+                       P->d_perm, W, zb);
+    const int64_t nb = (mn1 + kNB - 1) / kNB;
+    for (int64_t b = 0; b < nb; b++) {
+      hipLaunchKernelGGL((k_sv_ldiag<NC, kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, b, P->d_fac, zb);
+      if ((b + 1) * kNB < mn1)
+        hipLaunchKernelGGL((k_sv_lupd<NC, kNB>), dim3(cap_grid((mn1 - (b + 1) * kNB + 255) / 256), nl), dim3(256), 0,
+                           st, P->d_nodes, lev, b, P->d_fac, zb);
+    }
+    for (int64_t s = 0; s < nb; s++) {
+      hipLaunchKernelGGL((k_sv_udiag<NC, kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, s, P->d_fac, zb);
+      if ((s + 1) * kNB < mn1)
+        hipLaunchKernelGGL((k_sv_uupd<NC, kNB>), dim3(cap_grid((mn1 - (s + 1) * kNB + 255) / 256), nl), dim3(256), 0,
+                           st, P->d_nodes, lev, s, P->d_fac, zb);
+    }
+  }
+  for (int d = 0; d <= P->maxdepth; d++) {
+    const int nl = (int)P->levels[d].size();
+    const int32_t *lev = P->d_lev + P->lev_off[d];
+    const int64_t mn1 = P->lev_maxn1[d], mn2 = P->lev_maxn2[d];
+    if (mn1 == 0 || mn2 == 0) continue;
+    hipLaunchKernelGGL((k_sv_back<NC>), dim3(cap_grid((mn1 + 3) / 4), nl), dim3(256), 0, st, P->d_nodes, lev,
+                       P->d_fpos, P->d_fac, zb);
+  }
+  hipLaunchKernelGGL((k_sv_out<NC>), dim3(cap_grid((P->nne * NC + 255) / 256)), dim3(256), 0, st, P->nne, P->d_pos, zb,
+                     out);
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------------ GMRES --
+template <int NC>
+__global__ __launch_bounds__(256) void k_gm_dot(int64_t n, const double2 *V, int64_t vstride, const double2 *w,
+                                                double *part) {
+  const double2 *v = V + (int64_t)blockIdx.y * vstride;
+  double acc[1][NC][2];
+#pragma unroll
+  for (int c = 0; c < NC; c++) acc[0][c][0] = acc[0][c][1] = 0.0;
+  for (int64_t r = blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256)
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const double2 a = v[r * NC + c], b = w[r * NC + c];
+      acc[0][c][0] += a.x * b.x + a.y * b.y;
+      acc[0][c][1] += a.x * b.y - a.y * b.x;
+    }
+  block_partials<NC, 1>(acc, part + (int64_t)blockIdx.y * gridDim.x * NC * 2);
+}
+
+template <int NC>
+__global__ void k_gm_red(const double *part, int nblocks, double *out) {
+  reduce_partials<NC, 1>(part + (int64_t)blockIdx.x * nblocks * NC * 2, nblocks, out + (int64_t)blockIdx.x * NC * 2);
+}
+
+// w = (init ? 0 : w) + sign * sum_v coef[v][c] V_v
+template <int NC>
+__global__ void k_gm_axpy(int64_t n, const double2 *V, int64_t vstride, int nv, const double *coef, double2 *w,
+                          double sign, int init) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * NC; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % NC);
+    double2 acc = init ? make_double2(0.0, 0.0) : w[e];
+    for (int v = 0; v < nv; v++) {
+      const double2 cf = make_double2(sign * coef[(v * NC + c) * 2], sign * coef[(v * NC + c) * 2 + 1]);
+      cfma(acc, cf, V[v * vstride + e]);
+    }
+    w[e] = acc;
+  }
+}
+
+template <int NC>
+__global__ void k_gm_add(int64_t n, double2 *x, const double2 *u) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * NC; e += (int64_t)gridDim.x * blockDim.x) {
+    x[e].x += u[e].x;
+    x[e].y += u[e].y;
+  }
+}
+
+struct cplx {
+  double r, i;
+};
+static inline cplx cm(cplx a, cplx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+static inline cplx cconj(cplx a) { return {a.r, -a.i}; }
+static inline double cabs_(cplx a) { return std::hypot(a.r, a.i); }
+
+// Right-preconditioned restarted GMRES with the multifrontal factor of A_pc at
+// the frequency of column 0; classical Gram-Schmidt with one
+// reorthogonalization pass, Hermitian inner products, FP64 throughout.
+template <int NC>
+static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  mvf_handle *h = S.h;
+  const int64_t n = S.n;
+  const int64_t nv = n * NC;  // complex entries per vector
+  const int mres = o->gmres_m > 0 ? std::min(o->gmres_m, 64) : 30;
+  MVF_TRY(direct_plan(h));
+  DirectPlan *P = (DirectPlan *)h->direct;
+  const double w0 = S.w.w[0];
+  if (!P->factored || P->omega != w0 || P->c64 != (int)S.c64) MVF_TRY(direct_factor(h, w0, (int)S.c64));
+  info->factor_seconds += P->t_factor;
+  info->factorizations = (int32_t)P->factorizations;
+  info->static_pivots = P->npert;
+  info->factor_flops = P->flops;
+  info->factor_bytes = (double)P->fac_entries * 16.0;
+  double2 *V, *z, *wv, *r;
+  double *part, *dout, *coef, *dnorm;
+  MVF_TRY(dalloc(h, &V, (size_t)(mres + 1) * nv));
+  MVF_TRY(dalloc(h, &z, (size_t)nv));
+  MVF_TRY(dalloc(h, &wv, (size_t)nv));
+  MVF_TRY(dalloc(h, &r, (size_t)nv));
+  const int nbk = vec_blocks(n, NC);
+  MVF_TRY(dalloc(h, &part, (size_t)(mres + 1) * std::max(nbk, kRedBlocks) * NC * 2 + (size_t)kRedBlocks * 3 * NC * 2));
+  MVF_TRY(dalloc(h, &dout, (size_t)(mres + 1) * NC * 2));
+  MVF_TRY(dalloc(h, &coef, (size_t)(mres + 1) * NC * 2));
+  MVF_TRY(dalloc(h, &dnorm, 4 * NC));
+  hipStream_t st = h->stream;
+  const unsigned gv = cap_grid((nv + 255) / 256);
+  auto dots = [&](int nvec, const double2 *w, std::vector<cplx> &outv) -> int {
+    hipLaunchKernelGGL((k_gm_dot<NC>), dim3(nbk, nvec), dim3(kBS), 0, st, n, V, nv, w, part);
+    hipLaunchKernelGGL((k_gm_red<NC>), dim3(nvec), dim3(1024), 0, st, part, nbk, dout);
+    std::vector<double> hb((size_t)nvec * NC * 2);
+    MVF_HIP(h, hipMemcpyAsync(hb.data(), dout, hb.size() * 8, hipMemcpyDeviceToHost, st));
+    MVF_HIP(h, hipStreamSynchronize(st));
+    outv.resize((size_t)nvec * NC);
+    for (size_t k = 0; k < outv.size(); k++) outv[k] = {hb[2 * k], hb[2 * k + 1]};
+    return 0;
+  };
+  auto put_coef = [&](const std::vector<double> &cf) -> int {
+    MVF_HIP(h, hipMemcpyAsync(coef, cf.data(), cf.size() * 8, hipMemcpyHostToDevice, st));
+    MVF_HIP(h, hipStreamSynchronize(st));
+    return 0;
+  };
+  double res[NC] = {0};
+  int it = 0, restarts = 0, rc = 0;
+  double t_apply = 0.0;
+  int applies = 0;
+  bool done = false;
+  while (!done) {
+    MVF_TRY(true_resid<NC>(S, b, x, (double *)z, part + (size_t)(mres + 1) * std::max(nbk, kRedBlocks) * NC * 2,
+                           dnorm, res, (double *)r));
+    double hn[2 * NC];
+    MVF_HIP(h, hipMemcpy(hn, dnorm, sizeof(hn), hipMemcpyDeviceToHost));
+    bool ok = true;
+    for (int c = 0; c < S.ncol; c++) ok &= res[c] <= o->tol;
+    if (ok) break;
+    if (it >= o->maxit || restarts > 50) {
+      rc = set_err(h, MVF_E_NOCONV, "GMRES with the direct preconditioner did not converge");
+      break;
+    }
+    double bn[NC], beta[NC];
+    std::vector<double> cf((size_t)NC * 2, 0.0);
+    for (int c = 0; c < NC; c++) {
+      beta[c] = hn[2 * c];
+      bn[c] = hn[2 * c + 1];
+      cf[2 * c] = beta[c] > 0 ? 1.0 / beta[c] : 0.0;
+    }
+    MVF_TRY(put_coef(cf));
+    hipLaunchKernelGGL((k_gm_axpy<NC>), dim3(gv), dim3(256), 0, st, n, r, nv, 1, coef, V, 1.0, 1);
+    // Hessenberg columns per RHS column, Givens rotations, rhs g
+    std::vector<std::vector<cplx>> H(NC, std::vector<cplx>((size_t)(mres + 1) * mres, cplx{0, 0}));
+    std::vector<std::vector<double>> gc(NC, std::vector<double>(mres, 0.0));
+    std::vector<std::vector<cplx>> gs(NC, std::vector<cplx>(mres, cplx{0, 0})), g(NC, std::vector<cplx>(mres + 1));
+    for (int c = 0; c < NC; c++) g[c][0] = {beta[c], 0.0};
+    int kdone = 0;
+    for (int j = 0; j < mres && it < o->maxit; j++) {
+      double ta = now_s();
+      MVF_TRY(direct_apply<NC>(h, V + (size_t)j * nv, z));
+      spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, (const double *)z, (double *)wv, nullptr, nullptr);
+      std::vector<cplx> hv;
+      for (int pass = 0; pass < 2; pass++) {
+        MVF_TRY(dots(j + 1, wv, hv));
+        std::vector<double> cf2((size_t)(j + 1) * NC * 2);
+        for (int i = 0; i <= j; i++)
+          for (int c = 0; c < NC; c++) {
+            cplx &Hij = H[c][(size_t)i * mres + j];
+            Hij.r += hv[(size_t)i * NC + c].r;
+            Hij.i += hv[(size_t)i * NC + c].i;
+            cf2[((size_t)i * NC + c) * 2] = hv[(size_t)i * NC + c].r;
+            cf2[((size_t)i * NC + c) * 2 + 1] = hv[(size_t)i * NC + c].i;
+          }
+        MVF_TRY(put_coef(cf2));
+        hipLaunchKernelGGL((k_gm_axpy<NC>), dim3(gv), dim3(256), 0, st, n, V, nv, j + 1, coef, wv, -1.0, 0);
+      }
+      // ||w||^2 = w^H w
+      hipLaunchKernelGGL((k_gm_dot<NC>), dim3(nbk, 1), dim3(kBS), 0, st, n, wv, nv, wv, part);
+      hipLaunchKernelGGL((k_gm_red<NC>), dim3(1), dim3(1024), 0, st, part, nbk, dout);
+      double nn[2 * NC];
+      MVF_HIP(h, hipMemcpyAsync(nn, dout, sizeof(nn), hipMemcpyDeviceToHost, st));
+      MVF_HIP(h, hipStreamSynchronize(st));
+      std::vector<double> cf3((size_t)NC * 2, 0.0);
+      double worst = 0.0;
+      for (int c = 0; c < NC; c++) {
+        const double hnrm = std::sqrt(std::max(nn[2 * c], 0.0));
+        H[c][(size_t)(j + 1) * mres + j] = {hnrm, 0.0};
+        cf3[2 * c] = hnrm > 0 ? 1.0 / hnrm : 0.0;
+        // apply previous rotations to column j
+        for (int i = 0; i < j; i++) {
+          cplx a = H[c][(size_t)i * mres + j], bb = H[c][(size_t)(i + 1) * mres + j];
+          cplx s_ = gs[c][i];
+          double cc = gc[c][i];
+          cplx t = {cc * a.r + cm(s_, bb).r, cc * a.i + cm(s_, bb).i};
+          cplx u = {-cm(cconj(s_), a).r + cc * bb.r, -cm(cconj(s_), a).i + cc * bb.i};
+          H[c][(size_t)i * mres + j] = t;
+          H[c][(size_t)(i + 1) * mres + j] = u;
+        }
+        cplx a = H[c][(size_t)j * mres + j];
+        const double bb = hnrm, aa = cabs_(a), rr = std::hypot(aa, bb);
+        double cc;
+        cplx s_;
+        if (rr == 0.0) {
+          cc = 1.0;
+          s_ = {0.0, 0.0};
+        } else if (aa == 0.0) {
+          cc = 0.0;
+          s_ = {1.0, 0.0};
+          H[c][(size_t)j * mres + j] = {bb, 0.0};
+        } else {
+          cc = aa / rr;
+          cplx ph = {a.r / aa, a.i / aa};
+          s_ = {ph.r * bb / rr, ph.i * bb / rr};
+          H[c][(size_t)j * mres + j] = {ph.r * rr, ph.i * rr};
+        }
+        H[c][(size_t)(j + 1) * mres + j] = {0.0, 0.0};
+        gc[c][j] = cc;
+        gs[c][j] = s_;
+        cplx gj = g[c][j];
+        g[c][j + 1] = {-cm(cconj(s_), gj).r, -cm(cconj(s_), gj).i};
+        g[c][j] = {cc * gj.r, cc * gj.i};
+        if (c < S.ncol && bn[c] > 0) worst = std::max(worst, cabs_(g[c][j + 1]) / bn[c]);
+      }
+      MVF_TRY(put_coef(cf3));
+      hipLaunchKernelGGL((k_gm_axpy<NC>), dim3(gv), dim3(256), 0, st, n, wv, nv, 1, coef, V + (size_t)(j + 1) * nv,
+                         1.0, 1);
+      t_apply += now_s() - ta;
+      applies++;
+      it++;
+      kdone = j + 1;
+      if (worst <= 0.3 * o->tol) break;
+    }
+    // y = R^-1 g per column; u = V y; x += M^-1 u
+    std::vector<double> cy((size_t)kdone * NC * 2, 0.0);
+    for (int c = 0; c < NC; c++) {
+      std::vector<cplx> y(kdone);
+      for (int i = kdone - 1; i >= 0; i--) {
+        cplx v = g[c][i];
+        for (int q = i + 1; q < kdone; q++) {
+          cplx t = cm(H[c][(size_t)i * mres + q], y[q]);
+          v.r -= t.r;
+          v.i -= t.i;
+        }
+        cplx d = H[c][(size_t)i * mres + i];
+        double dd = d.r * d.r + d.i * d.i;
+        y[i] = dd > 0 ? cplx{(v.r * d.r + v.i * d.i) / dd, (v.i * d.r - v.r * d.i) / dd} : cplx{0.0, 0.0};
+      }
+      for (int i = 0; i < kdone; i++) {
+        cy[((size_t)i * NC + c) * 2] = y[i].r;
+        cy[((size_t)i * NC + c) * 2 + 1] = y[i].i;
+      }
+    }
+    if (kdone > 0) {
+      MVF_TRY(put_coef(cy));
+      hipLaunchKernelGGL((k_gm_axpy<NC>), dim3(gv), dim3(256), 0, st, n, V, nv, kdone, coef, wv, 1.0, 1);
+      MVF_TRY(direct_apply<NC>(h, wv, z));
+      hipLaunchKernelGGL((k_gm_add<NC>), dim3(gv), dim3(256), 0, st, n, (double2 *)x, z);
+      applies++;
+    }
+    restarts++;
+  }
+  info->iterations = it;
+  info->refinements = restarts;
+  info->converged = rc == 0;
+  info->precond_applies += applies;
+  for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
+  dfree(V); dfree(z); dfree(wv); dfree(r); dfree(part); dfree(dout); dfree(coef); dfree(dnorm);
+  return rc;
+}
+
+}  // namespace mvf
+
+// Host-only symbolic analysis on caller arrays (no device use): stats[0..9] =
+// fronts, depth, max n1, max n2, max front, factor entries, real flops of one
+// factorization, sum of n1, seconds, update entries.
+extern "C" int mvf_direct_plan_stats(int32_t nx, int32_t ny, int32_t nz, int32_t me, const int32_t *gne, int64_t nne,
+                                     const int64_t *rowptr, const int32_t *col, double *stats) {
+  if (!gne || !rowptr || !col || !stats || nx < 1 || ny < 1 || nz < 1 || me < 1 || nne < 1) return MVF_E_ARG;
+  double t0 = mvf::now_s();
+  mvf::DirectPlan P;
+  mvf::PlanArrays A;
+  std::string err;
+  int rc = mvf::build_plan_host(P, nx, ny, nz, me, gne, nne, rowptr, col, A, err);
+  if (rc) return rc;
+  int64_t mn1 = 0, mn2 = 0, mm = 0, s1 = 0;
+  for (const mvf::DNode &n : P.nodes) {
+    mn1 = std::max(mn1, n.n1);
+    mn2 = std::max(mn2, n.n2);
+    mm = std::max(mm, n.m);
+    s1 += n.n1;
+  }
+  stats[0] = P.T;
+  stats[1] = P.maxdepth;
+  stats[2] = (double)mn1;
+  stats[3] = (double)mn2;
+  stats[4] = (double)mm;
+  stats[5] = (double)P.fac_entries;
+  stats[6] = P.flops;
+  stats[7] = (double)s1;
+  stats[8] = mvf::now_s() - t0;
+  stats[9] = (double)P.nrel;
+  return 0;
+}
+

@@ -110,6 +110,8 @@ struct mvf_handle {
   double *d_x = nullptr, *d_b = nullptr;
   size_t xb_cap = 0;
   double t_setup[4] = {0, 0, 0, 0};
+  // sparse direct factorization state (mvf_direct.hip)
+  void *direct = nullptr;
   // scratch
   std::vector<void *> owned;
 };
@@ -170,5 +172,7 @@ int assemble_operators32(mvf_handle *h, const float *Ke, const float *Me);
 int element_rhs_range(mvf_handle *h, double omega, int first_flags, int64_t e0, int64_t e1, double *d_be);
 int assemble_rhs(mvf_handle *h, const double *d_be, double *d_b, int ncol, int col0);
 int update_sigma(mvf_handle *h, double omega);
+void direct_free(mvf_handle *h);
+void direct_invalidate(mvf_handle *h);
 
 }  // namespace mvf

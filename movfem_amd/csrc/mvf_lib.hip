@@ -2,4 +2,5 @@
 #include "mvf_setup.hip"
 #include "mvf_integrate.hip"
 #include "mvf_solver.hip"
+#include "mvf_direct.hip"
 #include "mvf_api.hip"

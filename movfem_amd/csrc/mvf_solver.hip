@@ -1034,6 +1034,9 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
 }
 
 template <int NC>
+static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info);
+
+template <int NC>
 static int solve_nc(mvf_handle *h, const double *omegas, int ncol, const mvf_solver_opts *o, const double *b,
                     double *x, mvf_solve_info *info) {
   SpmvTimer *tm = new SpmvTimer();
@@ -1041,7 +1044,8 @@ static int solve_nc(mvf_handle *h, const double *omegas, int ncol, const mvf_sol
   Solver S{h, ncol, {}, o->matrix == MVF_MATRIX_C64REF, 16, (int64_t)h->nne, tm};
   for (int c = 0; c < NC; c++) S.w.w[c] = c < ncol ? (double)(float)omegas[c] : 0.0;
   int rc;
-  if (o->precision == MVF_PREC_MIXED) rc = run_cocg_mixed<NC>(S, o, b, x, info);
+  if (o->precond == MVF_PC_DIRECT) rc = run_gmres_direct<NC>(S, o, b, x, info);
+  else if (o->precision == MVF_PREC_MIXED) rc = run_cocg_mixed<NC>(S, o, b, x, info);
   else if (o->method == MVF_KRYLOV_BICGSTAB) rc = run_bicgstab<NC>(S, o, b, x, info);
   else rc = run_cocg64<NC>(S, o, b, x, info);
   tm->destroy();

@@ -9,10 +9,10 @@ module movfem_gpu
   implicit none
 
   integer(c_int), parameter :: MVF_EMU_ALL = 3
-  integer(c_int), parameter :: MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1
+  integer(c_int), parameter :: MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1, MVF_KRYLOV_GMRES = 2
   integer(c_int), parameter :: MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1
   integer(c_int), parameter :: MVF_MATRIX_C64REF = 0, MVF_MATRIX_FP64 = 1
-  integer(c_int), parameter :: MVF_PC_NONE = 0, MVF_PC_JACOBI = 1
+  integer(c_int), parameter :: MVF_PC_NONE = 0, MVF_PC_JACOBI = 1, MVF_PC_DIRECT = 2
 
   type, bind(C) :: mvf_grid
     integer(c_int32_t) :: nx, ny, nz, nodes_per_elem
@@ -37,7 +37,7 @@ module movfem_gpu
     real(c_double) :: tol
     integer(c_int32_t) :: maxit, check_every
     real(c_double) :: inner_tol
-    integer(c_int32_t) :: use_graph, pad_
+    integer(c_int32_t) :: use_graph, gmres_m
   end type mvf_solver_opts
 
   type, bind(C) :: mvf_solve_info
@@ -47,6 +47,8 @@ module movfem_gpu
     integer(c_int64_t) :: spmv_launches(2)
     real(c_double) :: spmv_ms(2), spmv_bytes(2)
     integer(c_int32_t) :: spmv_ncol, pad_
+    real(c_double) :: factor_seconds, factor_flops, factor_bytes
+    integer(c_int32_t) :: factorizations, static_pivots, precond_applies, pad2_
   end type mvf_solve_info
 
   interface

@@ -37,13 +37,14 @@ def _ip(a):
 
 
 def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacobi", tol=1e-8, maxit=100000,
-                check_every=25, inner_tol=1e-2) -> L.SolverOpts:
+                check_every=25, inner_tol=1e-2, gmres_m=0) -> L.SolverOpts:
     o = L.SolverOpts()
     L.lib().mvf_default_solver_opts(C.byref(o))
-    o.method = {"cocg": L.MVF_KRYLOV_COCG, "bicgstab": L.MVF_KRYLOV_BICGSTAB}[method]
+    o.method = {"cocg": L.MVF_KRYLOV_COCG, "bicgstab": L.MVF_KRYLOV_BICGSTAB, "gmres": L.MVF_KRYLOV_GMRES}[method]
     o.precision = {"fp64": L.MVF_PREC_FP64, "mixed": L.MVF_PREC_MIXED}[precision]
     o.matrix = {"c64ref": L.MVF_MATRIX_C64REF, "fp64": L.MVF_MATRIX_FP64}[matrix]
-    o.precond = {"jacobi": L.MVF_PC_JACOBI, "none": L.MVF_PC_NONE}[precond]
+    o.precond = {"jacobi": L.MVF_PC_JACOBI, "none": L.MVF_PC_NONE, "direct": L.MVF_PC_DIRECT}[precond]
+    o.gmres_m = gmres_m
     o.tol = tol
     o.maxit = maxit
     o.check_every = check_every
@@ -55,13 +56,16 @@ def info_dict(info: L.SolveInfo, ncol: int) -> dict:
     return {"iterations": info.iterations, "refinements": info.refinements, "converged": bool(info.converged),
             "fallbacks": info.fallbacks, "true_resid": [info.true_resid[i] for i in range(min(ncol, 64))],
             "seconds": info.seconds, "spmv_launches": list(info.spmv_launches), "spmv_ms": list(info.spmv_ms),
-            "spmv_bytes": list(info.spmv_bytes), "spmv_ncol": info.spmv_ncol}
+            "spmv_bytes": list(info.spmv_bytes), "spmv_ncol": info.spmv_ncol,
+            "factor_seconds": info.factor_seconds, "factor_flops": info.factor_flops,
+            "factor_bytes": info.factor_bytes, "factorizations": info.factorizations,
+            "static_pivots": info.static_pivots, "precond_applies": info.precond_applies}
 
 
 class GpuVfem:
     """One model resident on one GPU (the handle owns every device buffer)."""
 
-    def __init__(self, m: MTModel, emulate: int = L.MVF_EMU_ALL, device: int = 0):
+    def __init__(self, m: MTModel, emulate: int = L.MVF_EMU_ALL, device: int = 0, verbose: int = 0):
         m.validate()
         self.m = m
         self._lib = L.lib()
@@ -73,7 +77,7 @@ class GpuVfem:
         self._grid = L.Grid(m.nx, m.ny, m.nz, m.nodes_per_elem, _dp(xp), _dp(yp), _dp(zp))
         self._model = L.Model(_dp(sg), _dp(mu))
         self._bc = L.BC(int(m.dirichlet), m.inimod, m.gpml_sch, m.nextd, m.nzl_top, 0, m.a0, m.b0, m.nn)
-        self._opts = L.Options(emulate, device, 0, 0)
+        self._opts = L.Options(emulate, device, verbose, 0)
         self.h = C.c_void_p()
         rc = self._lib.mvf_init(C.byref(self._grid), C.byref(self._model), C.byref(self._bc), C.byref(self._opts),
                                 C.byref(self.h))
