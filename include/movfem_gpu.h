@@ -207,6 +207,23 @@ int mvf_setup_times(const mvf_handle *h, double *t4);
  * integrate_ms (optional) receives the element kernel's time from HIP events. */
 int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms);
 
+/* ---- readout (node_solution, solution.f90:18-49; rx_data,
+ *      receiver_data.f90:111-133; get_impedance/get_res_phase, :456-505) -------
+ * x: host solution (nne, 2) complex column-major at the current frequency, as
+ * mvf_solve returns it; or NULL for frequency f (0-based) of the last
+ * mvf_solve_frequencies, read from the device.
+ * esol/hsol (optional): (2, npt, 3) complex, total E and H at every lattice
+ * node per polarization in node-id order; the first element in the
+ * reference's loop order that contains a node sets it (solution.f90:230-238).
+ * Receivers (nrx >= 0 at rx/ry): fields interpolated on the top face of
+ * element layer ke_top (1-based: the reference's top earth layer) inside the
+ * region ie, je in [nextd+1, n-nextd] (get_rx_element); rx_e/rx_h (nrx, 2, 3)
+ * complex, rx_z (nrx, 4) complex [Zxx Zxy Zyx Zyy], rho/phi (nrx, 4), degrees.
+ * A receiver outside that region is MVF_E_ARG.  Outputs may be NULL. */
+int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32_t nrx, const double *rx,
+                const double *ry, double *esol, double *hsol, double *rx_e, double *rx_h, double *rx_z, double *rho,
+                double *phi);
+
 /* Host-only symbolic analysis of the sparse direct factorization for a
  * numbering (gne, ne x me, 1-based, <= 0 removed) and the full CSR pattern
  * (0-based rowptr[nne+1], col): no device use.  stats[10] = fronts, tree

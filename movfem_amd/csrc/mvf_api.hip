@@ -110,7 +110,7 @@ void mvf_free(mvf_handle *h) {
   void *ptrs[] = {h->d_xp,  h->d_yp,  h->d_zp,   h->d_sig,   h->d_inmu,     h->d_dmu,      h->d_tab,
                   h->d_gne, h->d_rowptr, h->d_col, h->d_diagpos, h->d_seg, h->d_contrib, h->d_rseg,
                   h->d_rcontrib, h->d_K, h->d_M, h->d_K32, h->d_M32, h->d_Ke, h->d_Me, h->d_dK0, h->d_dM0,
-                  h->d_x,  h->d_b};
+                  h->d_x,  h->d_b, h->d_ntab};
   for (void *p : ptrs) dfree(p);
   direct_free(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -519,6 +519,7 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
     h->xb_cap = need;
   }
   h->ncols_last = ncol;
+  h->last_freqs.assign(freqs_hz, freqs_hz + nfreq);
   double t0 = now();
   // batches: the first assembly call (if still pending and the carried GPML
   // flags change element 0) is solved alone, the rest together
@@ -596,5 +597,107 @@ static int fetch_cols(mvf_handle *h, const double *d, double *out, int32_t ncols
 
 int mvf_get_solution(mvf_handle *h, double *x_out, int32_t ncols) { return fetch_cols(h, h ? h->d_x : nullptr, x_out, ncols); }
 int mvf_get_rhs(mvf_handle *h, double *b_out, int32_t ncols) { return fetch_cols(h, h ? h->d_b : nullptr, b_out, ncols); }
+
+int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32_t nrx, const double *rx,
+                const double *ry, double *esol, double *hsol, double *rx_e, double *rx_h, double *rx_z, double *rho,
+                double *phi) {
+  if (!h) return MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  const Geom &g = h->geo;
+  const int o = g.nord - 1;
+  double omega;
+  int ncol, col0;
+  double *dx = nullptr;
+  if (x) {
+    if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_readout: no current frequency");
+    omega = h->omega;
+    ncol = 2;
+    col0 = 0;
+    std::vector<double> t((size_t)h->nne * 4);
+    for (int64_t r = 0; r < h->nne; r++)
+      for (int d = 0; d < 2; d++) {
+        t[4 * r + 2 * d] = x[2 * ((size_t)d * h->nne + r)];
+        t[4 * r + 2 * d + 1] = x[2 * ((size_t)d * h->nne + r) + 1];
+      }
+    MVF_TRY(dalloc(h, &dx, t.size()));
+    MVF_HIP(h, hipMemcpy(dx, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    if (!h->d_x || f < 0 || f >= (int)h->last_freqs.size())
+      return set_err(h, MVF_E_ARG, "mvf_readout: frequency %d not in the last batch", f);
+    omega = 2.0 * MVF_PI * h->last_freqs[f];
+    ncol = h->ncols_last;
+    col0 = 2 * f;
+  }
+  if (nrx < 0 || (nrx > 0 && (!rx || !ry))) return set_err(h, MVF_E_ARG, "mvf_readout: bad receiver arguments");
+  if (nrx > 0 && (ke_top < 1 || ke_top > g.nz)) return set_err(h, MVF_E_ARG, "mvf_readout: ke_top=%d", ke_top);
+  // receiver elements: the reference's search over the inner region (get_rx_element)
+  std::vector<double> hxp(g.nnx), hyp(g.nny);
+  std::vector<int32_t> rie(nrx > 0 ? nrx : 1), rje(nrx > 0 ? nrx : 1);
+  if (nrx > 0) {
+    MVF_HIP(h, hipMemcpy(hxp.data(), h->d_xp, 8 * g.nnx, hipMemcpyDeviceToHost));
+    MVF_HIP(h, hipMemcpy(hyp.data(), h->d_yp, 8 * g.nny, hipMemcpyDeviceToHost));
+    const int nd = h->bc.nextd;
+    for (int ir = 0; ir < nrx; ir++) {
+      int ie = -1, je = -1;
+      for (int a = nd + 1; a <= g.nx - nd; a++)
+        if (rx[ir] >= hxp[(a - 1) * o] && rx[ir] < hxp[a * o]) ie = a;
+      for (int b = nd + 1; b <= g.ny - nd; b++)
+        if (ry[ir] >= hyp[(b - 1) * o] && ry[ir] < hyp[b * o]) je = b;
+      if (ie < 0 || je < 0) {
+        dfree(dx);
+        return set_err(h, MVF_E_ARG, "mvf_readout: receiver %d (%g, %g) outside the inner region", ir, rx[ir], ry[ir]);
+      }
+      rie[ir] = ie;
+      rje[ir] = je;
+    }
+  }
+  double *es = nullptr, *hs = nullptr;
+  const size_t nf = (size_t)2 * g.npt * 3 * 2;
+  int rc = dalloc(h, &es, nf);
+  if (!rc) rc = dalloc(h, &hs, nf);
+  if (!rc) rc = node_fields(h, x ? dx : h->d_x, ncol, col0, omega, es, hs);
+  if (!rc && nrx > 0) {
+    double *drx = nullptr, *dry = nullptr, *doe = nullptr, *doh = nullptr, *doz = nullptr, *drho = nullptr,
+           *dphi = nullptr;
+    int32_t *die = nullptr, *dje = nullptr;
+    rc = dalloc(h, &drx, nrx);
+    if (!rc) rc = dalloc(h, &dry, nrx);
+    if (!rc) rc = dalloc(h, &die, nrx);
+    if (!rc) rc = dalloc(h, &dje, nrx);
+    if (!rc) rc = dalloc(h, &doe, (size_t)nrx * 12);
+    if (!rc) rc = dalloc(h, &doh, (size_t)nrx * 12);
+    if (!rc) rc = dalloc(h, &doz, (size_t)nrx * 8);
+    if (!rc) rc = dalloc(h, &drho, (size_t)nrx * 4);
+    if (!rc) rc = dalloc(h, &dphi, (size_t)nrx * 4);
+    if (!rc) {
+      hipError_t e = hipMemcpy(drx, rx, 8 * nrx, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(dry, ry, 8 * nrx, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(die, rie.data(), 4 * nrx, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(dje, rje.data(), 4 * nrx, hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = receivers(h, es, hs, ke_top, nrx, die, dje, drx, dry, omega, doe, doh, doz, drho, dphi);
+    if (!rc) {
+      hipError_t e = hipStreamSynchronize(h->stream);
+      if (e == hipSuccess && rx_e) e = hipMemcpy(rx_e, doe, 8 * (size_t)nrx * 12, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && rx_h) e = hipMemcpy(rx_h, doh, 8 * (size_t)nrx * 12, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && rx_z) e = hipMemcpy(rx_z, doz, 8 * (size_t)nrx * 8, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && rho) e = hipMemcpy(rho, drho, 8 * (size_t)nrx * 4, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && phi) e = hipMemcpy(phi, dphi, 8 * (size_t)nrx * 4, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
+    }
+    dfree(drx); dfree(dry); dfree(die); dfree(dje); dfree(doe); dfree(doh); dfree(doz); dfree(drho); dfree(dphi);
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && esol) e = hipMemcpy(esol, es, nf * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && hsol) e = hipMemcpy(hsol, hs, nf * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
+  }
+  dfree(es);
+  dfree(hs);
+  dfree(dx);
+  return rc;
+}
 
 }  // extern "C"

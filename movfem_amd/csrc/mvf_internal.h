@@ -112,6 +112,9 @@ struct mvf_handle {
   double t_setup[4] = {0, 0, 0, 0};
   // sparse direct factorization state (mvf_direct.hip)
   void *direct = nullptr;
+  // readout: basis at the element's lattice nodes; frequencies of the last batch
+  mvf::BasisTab *d_ntab = nullptr;
+  std::vector<double> last_freqs;
   // scratch
   std::vector<void *> owned;
 };
@@ -173,6 +176,10 @@ int element_rhs_range(mvf_handle *h, double omega, int first_flags, int64_t e0, 
 int assemble_rhs(mvf_handle *h, const double *d_be, double *d_b, int ncol, int col0);
 int update_sigma(mvf_handle *h, double omega);
 void direct_free(mvf_handle *h);
+int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega, double *es, double *hs);
+int receivers(mvf_handle *h, const double *es, const double *hs, int ke_top, int nrx, const int32_t *rie,
+              const int32_t *rje, const double *rx, const double *ry, double omega, double *oe, double *oh, double *oz,
+              double *rho, double *phi);
 void direct_invalidate(mvf_handle *h);
 
 }  // namespace mvf

@@ -3,4 +3,5 @@
 #include "mvf_integrate.hip"
 #include "mvf_solver.hip"
 #include "mvf_direct.hip"
+#include "mvf_readout.hip"
 #include "mvf_api.hip"

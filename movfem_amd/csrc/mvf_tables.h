@@ -192,12 +192,23 @@ struct BasisTab {
   unsigned char pi[1485], pj[1485];  // upper-packed pair p -> (i, j), row-major i <= j
 };
 
+// Basis tabulated at npts given points (w = 0); fill_basis below uses the
+// Gauss rule, the readout uses the element's lattice nodes.
+inline void fill_basis_points(int nodes, const double (*pts)[3], int npts, const double *w, BasisTab *t);
+
 inline void fill_basis(int nodes, BasisTab *t) {
   ElemType et = elem_type(nodes);
-  double w[27];
-  gauss_rule(et.ng1, t->pts, w);
-  for (int g = 0; g < et.ng; g++) {
-    t->w[g] = w[g];
+  double w[27], pts[27][3];
+  gauss_rule(et.ng1, pts, w);
+  fill_basis_points(nodes, pts, et.ng, w, t);
+}
+
+inline void fill_basis_points(int nodes, const double (*pts)[3], int npts, const double *w, BasisTab *t) {
+  ElemType et = elem_type(nodes);
+  for (int g = 0; g < npts; g++)
+    for (int c = 0; c < 3; c++) t->pts[g][c] = pts[g][c];
+  for (int g = 0; g < npts; g++) {
+    t->w[g] = w ? w[g] : 0.0;
     for (int n = 0; n < et.nn; n++) {
       t->N[g][n] = nodal(nodes, n, t->pts[g]);
       for (int d = 0; d < 3; d++) t->dN[g][d][n] = dnodal(nodes, n, d, t->pts[g]);

@@ -202,3 +202,31 @@ class GpuVfem:
         x = np.empty(self.nne * ncols, np.complex128)
         self._check(self._lib.mvf_get_rhs(self.h, _dp(x.view(np.float64)), ncols))
         return x.reshape(ncols, self.nne)
+
+    def readout(self, x=None, f=0, ke_top=0, rx=(), ry=()):
+        """node_solution + rx_data on the device.  x: host solution (2*nne,
+        polarization-major) at the current frequency, or None for frequency f
+        of the last solve_frequencies.  Returns esol, hsol (2, npt, 3) and, with
+        receivers, (E, H (nrx, 2, 3), Z (nrx, 4), rho, phase (nrx, 4))."""
+        npt = self.m.npt
+        es = np.empty(2 * npt * 3, np.complex128)
+        hs = np.empty(2 * npt * 3, np.complex128)
+        rx = np.ascontiguousarray(rx, np.float64)
+        ry = np.ascontiguousarray(ry, np.float64)
+        n = len(rx)
+        oe = np.empty(max(n, 1) * 6, np.complex128)
+        oh = np.empty(max(n, 1) * 6, np.complex128)
+        oz = np.empty(max(n, 1) * 4, np.complex128)
+        rho = np.empty(max(n, 1) * 4)
+        phi = np.empty(max(n, 1) * 4)
+        xv = None if x is None else np.ascontiguousarray(x, np.complex128)
+        self._check(self._lib.mvf_readout(self.h, None if xv is None else _dp(xv.view(np.float64)), f, ke_top, n,
+                                          _dp(rx) if n else None, _dp(ry) if n else None,
+                                          _dp(es.view(np.float64)), _dp(hs.view(np.float64)),
+                                          _dp(oe.view(np.float64)), _dp(oh.view(np.float64)),
+                                          _dp(oz.view(np.float64)), _dp(rho), _dp(phi)))
+        out = (es.reshape(2, npt, 3), hs.reshape(2, npt, 3))
+        if n:
+            out += (oe[:6 * n].reshape(n, 2, 3), oh[:6 * n].reshape(n, 2, 3), oz[:4 * n].reshape(n, 4),
+                    rho[:4 * n].reshape(n, 4), phi[:4 * n].reshape(n, 4))
+        return out

@@ -14,7 +14,7 @@ module mvf_ref_harness
   use kind_param
   use geometry, only: g_nx, g_ny, g_nz, g_nnx, g_nny, g_nnz, g_nyz, g_npt, g_nordx, g_nordy, &
        g_nordz, g_xp, g_yp, g_zp, g_sigma, g_mu, g_nsf, g_nzl, g_nf, g_freq, g_ztop, nextd, &
-       omega, pi, update_omega, update_sigma
+       omega, pi, update_omega, update_sigma, asx, asy, asz
   use n_fem, only: init_n_fem, nf_get_r
   use v_fem, only: init_v_fem
   use problem, only: init_problem, p_elem_fields, pset_pmodel
@@ -23,6 +23,8 @@ module mvf_ref_harness
   use integration, only: init_integration, int_elem_params, alocal, blocal
   use global_assembly, only: ga_init, ga_assemble_nze, ga_sort_sparse, assign_aij, assign_bi, &
        gne, nzindx, nne, nnze, sym
+  use solution, only: node_solution, esol, hsol
+  use receiver_data, only: write_receiver_data
   implicit none
   integer, save :: h_me = 0
 contains
@@ -59,6 +61,13 @@ contains
     g_nf = nf
     allocate(g_freq(nf)); g_freq = freqs(1:nf)
     g_ztop = ztop
+    ! reference coordinates of the element's lattice nodes (what the
+    ! reference's private gqg_nodes sets, geometry.f90:715-740)
+    allocate(asx(g_nordx), asy(g_nordy), asz(g_nordz))
+    do i = 1, g_nordx
+      asx(i) = -1.d0 + dble(i - 1)*2.d0/dble(g_nordx - 1)
+    end do
+    asy = asx; asz = asx
     omega = 2.d0*pi*g_freq(1)
     gpml_sch = sch; a0 = a0_; b0 = b0_; nn = nn_
     ! initialize_vfem order (MoVFEM_3DMT.f90:354-388)
@@ -182,4 +191,33 @@ contains
       call assign_bi(ide, im, (blocal(im) - bda), b)
     end do
   end subroutine local_vfem_h
+
+  ! node_solution (solution.f90:18-49) on the solution x (2*nne, polarization
+  ! major): total E and H at every lattice node, (2*npt, 3) each
+  subroutine ref_node_solution(x, eo, ho) bind(C, name='ref_node_solution')
+    complex(c_double_complex), intent(in) :: x(*)
+    complex(c_double_complex), intent(out) :: eo(2*g_npt, 3), ho(2*g_npt, 3)
+    call node_solution(x(1:2*nne))
+    eo = esol
+    ho = hsol
+  end subroutine ref_node_solution
+
+  ! write_receiver_data(1) (receiver_data.f90:25-67) for receivers (xs, ys);
+  ! it writes RECEIVER.OUT in the working directory.  nzl_a/nzl_b are the
+  ! layer counts of the two top zones that place the receiver element
+  ! (get_rx_element, :187-209).  Must follow ref_node_solution.
+  subroutine ref_receivers(nsr, xs, ys, nzl_a, nzl_b) bind(C, name='ref_receivers')
+    use geometry, only: g_nsr, g_xsr, g_ysr, g_zsr, x0, y0, z0
+    integer(c_int), value :: nsr, nzl_a, nzl_b
+    real(c_double), intent(in) :: xs(*), ys(*)
+    g_nsr = nsr
+    if (allocated(g_xsr)) deallocate(g_xsr, g_ysr, g_zsr)
+    allocate(g_xsr(nsr), g_ysr(nsr), g_zsr(nsr))
+    g_xsr = xs(1:nsr); g_ysr = ys(1:nsr); g_zsr = 0.d0
+    x0 = 0.d0; y0 = 0.d0; z0 = 0.d0
+    g_nzl(g_nsf) = nzl_a
+    g_nzl(g_nsf - 1) = nzl_b
+    g_nf = 1
+    call write_receiver_data(1)
+  end subroutine ref_receivers
 end module mvf_ref_harness

@@ -22,25 +22,59 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 
 
-def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=()):
+def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=None, receivers=None):
+    rx = np.zeros((0, 2)) if receivers is None else np.asarray(receivers["xy"], np.float64)
+    rz = np.zeros(2, np.int32) if receivers is None else np.asarray(receivers["nzl"], np.int32)
     np.savez(path, nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp,
              sigma=m.sigma, mu=m.mu, freqs=m.freqs, dirichlet=int(m.dirichlet), inimod=m.inimod,
              gpml_sch=m.gpml_sch, a0=m.a0, b0=m.b0, nn=m.nn, nextd=m.nextd, nzl_top=m.nzl_top,
              ztop=m.ztop, elements=np.asarray(elements, np.int32).reshape(-1, 3),
-             vfem_freqs=np.asarray(vfem_freqs, np.int32), sigma_after=np.asarray(sigma_after, np.int32))
+             vfem_freqs=np.asarray(vfem_freqs, np.int32), sigma_after=np.asarray(sigma_after, np.int32),
+             readout_x=np.zeros(0, complex) if readout_x is None else np.asarray(readout_x, np.complex128),
+             rx_xy=rx, rx_nzl=rz)
 
 
-def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600):
+def parse_receiver_out(path, nsr):
+    """Fields, Z, rho_a and phase from the reference's RECEIVER.OUT."""
+    lines = pathlib.Path(path).read_text().splitlines()
+    num = []
+    for ln in lines[2 + nsr:]:
+        parts = ln.split()
+        try:
+            num.append([float(p) for p in parts])
+        except ValueError:
+            continue  # section header
+    fields = np.array(num[:nsr])            # x y, then 12 reals per polarization
+    zs = np.array(num[nsr:2 * nsr])         # x y Re/Im of Zxx Zxy Zyx Zyy
+    rp = np.array(num[2 * nsr:3 * nsr])     # x y rho(4) phi(4)
+    e = np.zeros((nsr, 2, 3), complex)
+    hh = np.zeros((nsr, 2, 3), complex)
+    for d in range(2):
+        v = fields[:, 2 + 12 * d: 2 + 12 * (d + 1)]
+        e[:, d, :] = v[:, 0:6:2] + 1j * v[:, 1:6:2]
+        hh[:, d, :] = v[:, 6:12:2] + 1j * v[:, 7:12:2]
+    z = zs[:, 2::2] + 1j * zs[:, 3::2]
+    return {"rx_e": e, "rx_h": hh, "rx_z": z, "rx_rho": rp[:, 2:6], "rx_phi": rp[:, 6:10]}
+
+
+def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, readout_x=None, receivers=None):
     """Returns a dict of reference outputs for model m (fresh subprocess)."""
+    import os
+
     with tempfile.TemporaryDirectory() as td:
         fin, fout = pathlib.Path(td) / "in.npz", pathlib.Path(td) / "out.npz"
-        dump_model(m, fin, elements, vfem_freqs, sigma_after)
+        dump_model(m, fin, elements, vfem_freqs, sigma_after, readout_x, receivers)
+        env = dict(os.environ)
+        env["PYTHONPATH"] = str(HERE.parent) + os.pathsep + env.get("PYTHONPATH", "")
         r = subprocess.run([sys.executable, "-m", "oracle.refrun", str(fin), str(fout)],
-                           cwd=str(HERE.parent), capture_output=True, text=True, timeout=timeout)
+                           cwd=td, env=env, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError("reference run failed:\n" + r.stdout[-2000:] + r.stderr[-2000:])
         with np.load(fout) as z:
-            return {k: z[k] for k in z.files}
+            out = {k: z[k] for k in z.files}
+        if receivers is not None:
+            out.update(parse_receiver_out(pathlib.Path(td) / "RECEIVER.OUT", len(receivers["xy"])))
+        return out
 
 
 def _main(fin, fout):
@@ -57,6 +91,8 @@ def _main(fin, fout):
     L.ref_get_sigma.argtypes = [dp]
     L.ref_element.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
     L.ref_global_vfem.argtypes = [ip, ip, dp, dp]
+    L.ref_node_solution.argtypes = [dp, dp, dp]
+    L.ref_receivers.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int]
 
     def d(a):
         return a.ctypes.data_as(dp)
@@ -112,6 +148,20 @@ def _main(fin, fout):
         s = np.empty(npt * 12)
         L.ref_get_sigma(d(s))
         out[f"sigma_{int(fi)}"] = s.view(np.complex128).reshape(npt, 6)
+    x = z["readout_x"]
+    if len(x):
+        xv = np.ascontiguousarray(x, np.complex128).view(np.float64)
+        eo = np.empty(2 * npt * 3 * 2)
+        ho = np.empty(2 * npt * 3 * 2)
+        L.ref_node_solution(d(xv), d(eo), d(ho))
+        # Fortran (2*npt, 3) column-major -> (2, npt, 3)
+        out["esol"] = eo.view(np.complex128).reshape(3, 2, npt).transpose(1, 2, 0)
+        out["hsol"] = ho.view(np.complex128).reshape(3, 2, npt).transpose(1, 2, 0)
+        rxy = z["rx_xy"]
+        if len(rxy):
+            xs = np.ascontiguousarray(rxy[:, 0])
+            ys = np.ascontiguousarray(rxy[:, 1])
+            L.ref_receivers(len(xs), d(xs), d(ys), int(z["rx_nzl"][0]), int(z["rx_nzl"][1]))
     np.savez(fout, **out)
 
 
