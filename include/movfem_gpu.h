@@ -176,6 +176,11 @@ typedef struct mvf_solve_info {
   int32_t static_pivots;   /* pivots replaced because the owned block was singular */
   int32_t precond_applies; /* triangular-solve sweeps performed by this call */
   int32_t pad2_;
+  /* batched complex GEMM launches of the last factorization (its dominant
+   * kernel): summed HIP-event time, algorithmic real flops, launch count */
+  double gemm_ms;
+  double gemm_flops;
+  int64_t gemm_launches;
 } mvf_solve_info;
 
 void mvf_default_solver_opts(mvf_solver_opts *o);

@@ -552,6 +552,11 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
     info->factorizations = bi.factorizations;
     info->static_pivots = std::max(info->static_pivots, bi.static_pivots);
     info->precond_applies += bi.precond_applies;
+    if (bi.gemm_launches) {
+      info->gemm_ms = bi.gemm_ms;
+      info->gemm_flops = bi.gemm_flops;
+      info->gemm_launches = bi.gemm_launches;
+    }
     info->iterations += bi.iterations;
     info->refinements += bi.refinements;
     info->fallbacks += bi.fallbacks;

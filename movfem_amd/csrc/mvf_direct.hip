@@ -83,6 +83,43 @@ struct DirectPlan {
   int npert = 0;
   double t_symbolic = 0.0, t_factor = 0.0;
   int64_t factorizations = 0;
+  // GEMM launches of the last factorization: HIP-event time and algorithmic flops
+  double gemm_ms = 0.0, gemm_flops = 0.0;
+  int64_t gemm_launches = 0;
+};
+
+// events around the factorization's GEMM launches (ring drained at level ends)
+struct GemmTimer {
+  static constexpr int kRing = 512;
+  hipEvent_t ev[kRing][2];
+  double fl[kRing];
+  int used = 0;
+  double ms = 0.0, flops = 0.0;
+  int64_t launches = 0;
+  void init() {
+    for (int i = 0; i < kRing; i++) {
+      (void)hipEventCreate(&ev[i][0]);
+      (void)hipEventCreate(&ev[i][1]);
+    }
+  }
+  void drain() {
+    for (int i = 0; i < used; i++) {
+      float t = 0.f;
+      if (hipEventSynchronize(ev[i][1]) == hipSuccess && hipEventElapsedTime(&t, ev[i][0], ev[i][1]) == hipSuccess) {
+        ms += t;
+        flops += fl[i];
+        launches++;
+      }
+    }
+    used = 0;
+  }
+  void destroy() {
+    drain();
+    for (int i = 0; i < kRing; i++) {
+      (void)hipEventDestroy(ev[i][0]);
+      (void)hipEventDestroy(ev[i][1]);
+    }
+  }
 };
 
 static double now_s() {
@@ -1076,6 +1113,17 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   const double big = P->maxK + w * P->maxM;
   const double tiny = 1e-14 * big;
   MVF_HIP(h, hipMemsetAsync(P->d_npert, 0, sizeof(int), st));
+  GemmTimer *gt_ = new GemmTimer();
+  gt_->init();
+  auto gemm_begin = [&](double flops) {
+    if (gt_->used == GemmTimer::kRing) gt_->drain();
+    gt_->fl[gt_->used] = flops;
+    (void)hipEventRecord(gt_->ev[gt_->used][0], st);
+  };
+  auto gemm_end = [&]() {
+    (void)hipEventRecord(gt_->ev[gt_->used][1], st);
+    gt_->used++;
+  };
   double2 *prev = nullptr;
   for (int d = P->maxdepth; d >= 0; d--) {
     const int nl = (int)P->levels[d].size();
@@ -1105,12 +1153,14 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                          P->d_nodes, lev, P->d_fac, pool);
       for (int64_t k = 0; k < mn1; k += kNB) {
         int64_t gt = 0, gc = 0;
+        double gfl = 0.0;
         for (int t : P->levels[d]) {
           const DNode &n = P->nodes[t];
           if (k >= n.n1) continue;
           const int64_t k1 = std::min(k + kNB, n.n1);
           gt = std::max(gt, ((n.n1 - k1 + kTile - 1) / kTile) * ((n.m - k1 + kTile - 1) / kTile));
           gc = std::max(gc, n.m);
+          gfl += 8.0 * (double)(n.n1 - k1) * (double)(n.m - k1) * (double)(k1 - k);
         }
         hipLaunchKernelGGL((k_fa_panel<kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_piv,
                            P->d_npert, big, tiny);
@@ -1118,30 +1168,45 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                            k, P->d_fac, P->d_piv);
         hipLaunchKernelGGL((k_fa_trsm_u<kNB>), dim3(cap_grid((gc - k + 127) / 128), nl), dim3(128), 0, st,
                            P->d_nodes, lev, k, P->d_fac);
-        if (gt > 0)
+        if (gt > 0) {
+          gemm_begin(gfl);
           hipLaunchKernelGGL((k_fa_gemm<0>), dim3(cap_grid(gt), nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac,
                              pool);
+          gemm_end();
+        }
       }
       if (mn2 > 0) {
         const int64_t nsteps = (mn1 + kNB - 1) / kNB;
         for (int64_t s = 0; s < nsteps; s++) {
           int64_t gt = 0;
+          double gfl = 0.0;
           for (int t : P->levels[d]) {
             const DNode &n = P->nodes[t];
             const int64_t r1 = n.n1 - s * kNB;
             if (r1 <= 0 || n.n2 == 0) continue;
             const int64_t r0 = std::max<int64_t>(0, r1 - kNB);
             gt = std::max(gt, ((r0 + kTile - 1) / kTile) * ((n.n2 + kTile - 1) / kTile));
+            gfl += 8.0 * (double)r0 * (double)n.n2 * (double)(r1 - r0);
           }
           hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + 127) / 128), nl), dim3(128), 0, st, P->d_nodes,
                              lev, s, P->d_fac);
-          if (gt > 0)
+          if (gt > 0) {
+            gemm_begin(gfl);
             hipLaunchKernelGGL((k_fa_gemm<1>), dim3(cap_grid(gt), nl), dim3(256), 0, st, P->d_nodes, lev, s,
                                P->d_fac, pool);
+            gemm_end();
+          }
         }
         const int64_t tq = (mn2 + kTile - 1) / kTile;
+        double sfl = 0.0;
+        for (int t : P->levels[d]) {
+          const DNode &n = P->nodes[t];
+          sfl += 8.0 * (double)n.n2 * (double)(n.n2 + 1) / 2.0 * (double)n.n1;
+        }
+        gemm_begin(sfl);
         hipLaunchKernelGGL((k_fa_gemm<2>), dim3(cap_grid(tq * tq), nl), dim3(256), 0, st, P->d_nodes, lev, 0,
                            P->d_fac, pool);
+        gemm_end();
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
     }
@@ -1162,6 +1227,11 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   }
   MVF_HIP(h, hipStreamSynchronize(st));
   dfree(prev);
+  gt_->destroy();
+  P->gemm_ms = gt_->ms;
+  P->gemm_flops = gt_->flops;
+  P->gemm_launches = gt_->launches;
+  delete gt_;
   int np = 0;
   MVF_HIP(h, hipMemcpy(&np, P->d_npert, sizeof(int), hipMemcpyDeviceToHost));
   P->npert = np;
@@ -1304,6 +1374,9 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   info->static_pivots = P->npert;
   info->factor_flops = P->flops;
   info->factor_bytes = (double)P->fac_entries * 16.0;
+  info->gemm_ms = P->gemm_ms;
+  info->gemm_flops = P->gemm_flops;
+  info->gemm_launches = P->gemm_launches;
   double2 *V, *z, *wv, *r;
   double *part, *dout, *coef, *dnorm;
   MVF_TRY(dalloc(h, &V, (size_t)(mres + 1) * nv));

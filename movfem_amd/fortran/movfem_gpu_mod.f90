@@ -49,6 +49,8 @@ module movfem_gpu
     integer(c_int32_t) :: spmv_ncol, pad_
     real(c_double) :: factor_seconds, factor_flops, factor_bytes
     integer(c_int32_t) :: factorizations, static_pivots, precond_applies, pad2_
+    real(c_double) :: gemm_ms, gemm_flops
+    integer(c_int64_t) :: gemm_launches
   end type mvf_solve_info
 
   interface
