@@ -126,7 +126,7 @@ def main():
     ap.add_argument("--workload", default="halfspace8")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--maxit", type=int, default=60000)
+    ap.add_argument("--maxit", type=int, default=3000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -146,38 +146,56 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from movfem_amd.vfem import GpuVfem
+    from movfem_amd import _lib as L
+    from movfem_amd.vfem import GpuVfem, MvfError
 
     m, desc = workload(args.workload)
     freqs = list(m.freqs)
-    g = GpuVfem(m)
+    # The reference's update_sigma refreshes only the first npt+30 linear
+    # entries of sigma (geometry.f90:146-152): from the second frequency on,
+    # air keeps the first frequency's i*omega*eps and the right-hand side gets a
+    # component the air null space cannot absorb, which caps the attainable
+    # relative residual above 1e-8.  The timed solves use the corrected
+    # update; every other reference semantics stays emulated.
+    g = GpuVfem(m, emulate=L.MVF_EMU_GPML_PREV_FLAGS)
     print(f"# setup nne={g.nne} nnz={g.nnze} times={g.setup_times()}", file=sys.stderr, flush=True)
 
     def step():
         ms = g.assemble("fp64")
-        info = g.solve_frequencies(freqs, precision="mixed", tol=1e-8, maxit=args.maxit, check_every=50)
+        info = g.solve_frequencies(freqs, method="gmres", precond="direct", tol=1e-8)
         return ms, info
 
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
+        t_w = time.perf_counter()
         step()
+        print(f"# warmup step {w} {time.perf_counter() - t_w:.2f}s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     infos, elem_ms = [], []
-    for _ in range(args.steps):
+    for k_ in range(args.steps):
         ms, info = step()
         elem_ms.append(ms)
         infos.append(info)
+        print(f"# step {k_} its={info['iterations']} factor={info['factor_seconds']:.2f}s "
+              f"resid={max(info['true_resid'][:2 * len(freqs)]):.2e}", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     value = len(freqs) * args.steps / dt
     last = infos[-1]
-    # dominant kernel: the SpMV variant with the most device time
+    # dominant kernel: the factorization's batched complex GEMM
+    gemm_tfs = last["gemm_flops"] / (last["gemm_ms"] * 1e-3) / 1e12 if last["gemm_ms"] > 0 else 0.0
+    roofline = {"bound": "mfma", "kernel": "k_fa_gemm (FP64 complex, VALU; MI355X FP64 vector peak = FP64 matrix "
+                "peak)", "achieved": gemm_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": gemm_tfs / FP64_PEAK_TFS, "traffic": TRAFFIC_MEASURED.get((args.workload, "k_fa_gemm")),
+                "avg_launch_ms": last["gemm_ms"] / max(1, last["gemm_launches"]),
+                "launches": last["gemm_launches"], "flops_per_factorization": last["factor_flops"],
+                "factor_seconds_per_frequency": last["factor_seconds"] / len(freqs),
+                "factor_tflops_all_kernels": last["factor_flops"] * len(freqs) / max(last["factor_seconds"], 1e-9)
+                / 1e12}
     k = int(np.argmax(last["spmv_ms"]))
     avg_ms = last["spmv_ms"][k] / max(1, last["spmv_launches"][k])
-    achieved = last["spmv_bytes"][k] / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_spmv<float>" if k == 1 else "k_spmv<double>",
-                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": TRAFFIC_MEASURED.get((args.workload, "k_spmv<float>" if k == 1 else "k_spmv<double>")),
-                "avg_launch_ms": avg_ms, "launches": last["spmv_launches"][k],
-                "bytes_per_launch": last["spmv_bytes"][k], "columns": last["spmv_ncol"]}
+    spmv = {"bound": "hbm", "kernel": "k_spmv<double>", "achieved": last["spmv_bytes"][k] / (avg_ms * 1e-3) / 1e9
+            if avg_ms > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s", "avg_launch_ms": avg_ms,
+            "launches": last["spmv_launches"][k], "bytes_per_launch": last["spmv_bytes"][k]}
+    spmv["frac"] = spmv["achieved"] / HBM_PEAK_GBS
     # element kernel (FP64 and FP32 integration)
     e64 = float(np.median(elem_ms))
     e32 = g.assemble("fp32")
@@ -190,23 +208,34 @@ def main():
                               "flops_per_elem": F}}
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "complex: fp32 inner Krylov + fp64 refinement/residual",
+           "vs_baseline": None, "dtype": "c128 (FP64 complex factorization and GMRES)",
            "data": "synthetic", "config": {"workload": args.workload, "description": desc, "elements": m.ne,
                                            "nne": g.nne, "nnz": g.nnze, "frequencies": len(freqs),
-                                           "path": "batched mixed-precision COCG (Jacobi)"},
-           "iterations": last["iterations"], "refinements": last["refinements"],
-           "true_resid_max": max(last["true_resid"][:2 * len(freqs)]), "roofline": roofline}
+                                           "path": "device multifrontal LU (FP64, nested dissection) preconditioning "
+                                                   "FP64 GMRES, one factorization per frequency"},
+           "iterations": last["iterations"], "precond_applies": last["precond_applies"],
+           "true_resid_max": max(last["true_resid"][:2 * len(freqs)]), "roofline": roofline, "spmv_roofline": spmv}
     out.update(elem)
     if not args.no_plain:
-        t1 = time.perf_counter()
-        its = 0
-        for f in freqs:
-            inf = g.solve_frequencies([f], precision="fp64", tol=1e-8, maxit=args.maxit, check_every=50)
-            its += inf["iterations"]
-        tp = time.perf_counter() - t1
-        out["plain_gpu_solves_per_s"] = len(freqs) / tp
-        out["plain_gpu_iterations"] = its
-        out["speedup_vs_plain"] = value / out["plain_gpu_solves_per_s"]
+        # plain GPU Krylov (Jacobi COCG, FP64) at the lowest and highest frequency
+        plain = []
+        for f in (freqs[0], freqs[-1]):
+            t1 = time.perf_counter()
+            try:
+                inf = g.solve_frequencies([f], precision="fp64", tol=1e-8, maxit=args.maxit, check_every=50)
+                ok, res, its = True, max(inf["true_resid"][:2]), inf["iterations"]
+            except MvfError as e:
+                ok, res, its = False, None, args.maxit
+                print(f"# plain COCG at {f} Hz: {str(e)[:160]}", file=sys.stderr, flush=True)
+            plain.append({"freq_hz": f, "converged": ok, "iterations": its, "true_resid": res,
+                          "seconds": time.perf_counter() - t1})
+        out["plain_gpu"] = plain
+        if all(p["converged"] for p in plain):
+            out["plain_gpu_solves_per_s"] = len(plain) / sum(p["seconds"] for p in plain)
+            out["speedup_vs_plain"] = value / out["plain_gpu_solves_per_s"]
+        else:
+            out["plain_gpu_solves_per_s"] = 0.0
+            out["speedup_vs_plain"] = None
     if not args.no_cpu:
         m.nne_hint = g.nne
         cb = cpu_baseline(m)

@@ -73,10 +73,13 @@ def test_direct_frequency_sweep(need_gpu):
     freqs = (1e-3, 0.3, 30.0)
     m = M.anisotropic_blocks(nx=6, ny=6, nz_earth=4, nz_air=2, nodes=20, freqs=freqs, block=2, seed=5, nextd=2)
     g = GpuVfem(m, emulate=0)
-    info = g.solve_frequencies(freqs, method="gmres", precond="direct", tol=1e-8)
+    # 1e-10: the PML modes with mass but no stiffness are weakly determined,
+    # so a 1e-8 residual leaves ~1e-5 differences there (observed at 30 Hz);
+    # FP64 GMRES bottoms out near 1e-11 on this system
+    info = g.solve_frequencies(freqs, method="gmres", precond="direct", tol=1e-10)
     print("sweep", info["iterations"], info["factorizations"], info["factor_seconds"], flush=True)
     assert info["converged"], info
-    assert max(info["true_resid"][:6]) <= 1e-8 * 1.0001
+    assert max(info["true_resid"][:6]) <= 1e-10 * 1.0001
     x = g.solution(6)
     bb = g.rhs(6)
     mask = determined_dofs(m, g.gne())
@@ -88,5 +91,7 @@ def test_direct_frequency_sweep(need_gpu):
         lu = spla.splu(A)
         for d in range(2):
             xr = lu.solve(b[d])
+            rr = np.linalg.norm(b[d] - A @ xr) / np.linalg.norm(b[d])
             e = np.linalg.norm(x[2 * i + d][mask] - xr[mask]) / np.linalg.norm(xr[mask])
-            assert e <= 1e-6, (i, d, e)
+            print(f"f={freqs[i]} pol {d}: SuperLU residual {rr:.1e}, determined-DOF difference {e:.1e}")
+            assert e <= 1e-6, (i, d, e, rr)
