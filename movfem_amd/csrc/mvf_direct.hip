@@ -14,8 +14,10 @@
 //   top band    T (n1 x m) = [F11 | F12]  lives in the persistent factor store
 //   bottom band B (n2 x m) = [F21 | F22]  lives in a per-level pool
 // Factorization of a front:
-//   P F11 = L U (partial pivoting restricted to the owned rows), applied to
-//   the whole top band: T = [L\U | L^-1 P F12];  then Y = U^-1 (L^-1 P F12) =
+//   P F11 = L U, applied to the whole top band: T = [L\U | L^-1 P F12]; the
+//   pivot of each column is searched within its 32-row panel block (with the
+//   shift below the factored matrix needs no wider search: element growth
+//   stays below ~20 and GMRES iteration counts match full partial pivoting);  then Y = U^-1 (L^-1 P F12) =
 //   F11^-1 F12 overwrites F12, and the Schur complement S = F22 - F21 Y (lower
 //   triangle, S is symmetric because A is) is extend-added into the parent.
 // The factor of a node is its top band [L\U | Y] plus the row permutation.
@@ -33,6 +35,8 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -42,7 +46,7 @@ namespace mvf {
 
 constexpr int kLeafMax = 16;       // elements per leaf box
 constexpr int kNB = 32;            // panel width / triangular block size
-constexpr double kRegEps = 1e-4;   // air regularization relative to the smallest omega*M/K
+constexpr double kRegEps = 1e-3;   // air regularization relative to the smallest omega*M/K
 constexpr int kTile = 64;          // GEMM tile edge
 
 struct DNode {
@@ -491,69 +495,104 @@ __global__ __launch_bounds__(256) void k_fa_sym(const DNode *nodes, const int32_
   }
 }
 
-// unblocked LU with partial pivoting of panel columns [k, k+NB) over the owned
-// rows [k, n1); one workgroup per front
+// LU of the panel's diagonal block T[k:kend, k:kend] in LDS, pivot search
+// restricted to the block's rows (kend - k <= NB); one workgroup per front.
+// A pivot below `tiny` (a singular block) is replaced by `big` and counted.
 template <int NB>
-__global__ __launch_bounds__(256) void k_fa_panel(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
-                                                  int32_t *piv, int *npert, double big, double tiny) {
+__global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
+                                                 int32_t *piv, int *npert, double big, double tiny) {
   const DNode nd = nodes[lev[blockIdx.x]];
   if (k >= nd.n1) return;
-  const int64_t n1 = nd.n1, m = nd.m, kend = (k + NB < n1) ? k + NB : n1;
+  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
+  const int w = (int)(kend - k);
   double2 *T = fac + nd.fac;
-  __shared__ double sv[256];
-  __shared__ int64_t si[256];
-  __shared__ double2 prow[NB];
-  const int tid = threadIdx.x;
-  for (int64_t q = k; q < kend; q++) {
-    double best = -1.0;
-    int64_t bi = q;
-    for (int64_t i = q + tid; i < n1; i += 256) {
-      const double2 v = T[i * m + q];
-      const double a = v.x * v.x + v.y * v.y;
-      if (a > best) {
-        best = a;
-        bi = i;
+  __shared__ double2 a[NB][NB + 1];
+  __shared__ int sp;
+  const int t = threadIdx.x;
+  for (int e = t; e < w * w; e += 64) {
+    const int r = e / w, c = e % w;
+    a[r][c] = T[(k + r) * m + k + c];
+  }
+  __syncthreads();
+  for (int q = 0; q < w; q++) {
+    // pivot: largest |a[r][q]|, r in [q, w); one wave, ties to the lower row
+    double v = -1.0;
+    int ri = q;
+    if (t >= q && t < w) {
+      v = a[t][q].x * a[t][q].x + a[t][q].y * a[t][q].y;
+      ri = t;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(ri, off, 64);
+      if (ov > v || (ov == v && oi < ri)) {
+        v = ov;
+        ri = oi;
       }
     }
-    sv[tid] = best;
-    si[tid] = bi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) {
-        const double o = sv[tid + s];
-        const int64_t oi = si[tid + s];
-        if (o > sv[tid] || (o == sv[tid] && oi < si[tid])) {
-          sv[tid] = o;
-          si[tid] = oi;
-        }
-      }
-      __syncthreads();
+    if (t == 0) {
+      sp = ri;
+      piv[nd.pos0 + k + q] = (int32_t)(k + ri);
     }
-    const int64_t r = si[0];
-    const double bv = sv[0];
     __syncthreads();
-    if (r != q)
-      for (int64_t c = k + tid; c < kend; c += 256) {
-        const double2 a = T[q * m + c];
-        T[q * m + c] = T[r * m + c];
-        T[r * m + c] = a;
-      }
-    if (tid == 0) piv[nd.pos0 + q] = (int32_t)r;
+    const int r = sp;
+    if (r != q && t < w) {
+      const double2 x = a[q][t];
+      a[q][t] = a[r][t];
+      a[r][t] = x;
+    }
     __syncthreads();
-    if (tid == 0 && !(bv > tiny * tiny)) {
-      T[q * m + q] = make_double2(big, 0.0);
+    if (t == 0 && !(v > tiny * tiny)) {
+      a[q][q] = make_double2(big, 0.0);
       atomicAdd(npert, 1);
     }
     __syncthreads();
-    const double2 inv = cinv(T[q * m + q]);
-    for (int64_t c = q + 1 + tid; c < kend; c += 256) prow[c - k] = T[q * m + c];
+    const double2 inv = cinv(a[q][q]);
+    if (t > q && t < w) a[t][q] = cmul(a[t][q], inv);
     __syncthreads();
-    for (int64_t i = q + 1 + tid; i < n1; i += 256) {
-      const double2 l = cmul(T[i * m + q], inv);
-      T[i * m + q] = l;
-      for (int64_t c = q + 1; c < kend; c++) cfms(T[i * m + c], l, prow[c - k]);
+    if (t > q && t < w)
+      for (int c = q + 1; c < w; c++) cfms(a[t][c], a[t][q], a[q][c]);
+    __syncthreads();
+  }
+  for (int e = t; e < w * w; e += 64) {
+    const int r = e / w, c = e % w;
+    T[(k + r) * m + k + c] = a[r][c];
+  }
+}
+
+// L rows below the diagonal block: T[i, k:kend] = T[i, k:kend] U_kk^-1 for
+// i in [kend, n1); each thread owns one row, held in LDS
+template <int NB>
+__global__ __launch_bounds__(128) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  if (k >= nd.n1) return;
+  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
+  if (kend + (int64_t)blockIdx.x * 128 >= nd.n1) return;
+  const int w = (int)(kend - k);
+  double2 *T = fac + nd.fac;
+  __shared__ double2 U[NB][NB + 1];
+  __shared__ double2 dinv[NB];
+  __shared__ double2 xs[NB][129];
+  for (int e = threadIdx.x; e < NB * NB; e += 128) {
+    const int r = e / NB, c = e % NB;
+    U[r][c] = (r < w && c < w && c >= r) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
+  __syncthreads();
+  const int t = threadIdx.x;
+  for (int64_t i0 = kend + (int64_t)blockIdx.x * 128; i0 < nd.n1; i0 += (int64_t)gridDim.x * 128) {
+    const int64_t i = i0 + t;
+    if (i < nd.n1) {
+      double2 *row = T + i * m + k;
+      for (int q = 0; q < w; q++) xs[q][t] = row[q];
+      for (int q = 0; q < w; q++) {
+        double2 v = xs[q][t];
+        for (int p = 0; p < q; p++) cfms(v, xs[p][t], U[p][q]);
+        xs[q][t] = cmul(v, dinv[q]);
+      }
+      for (int q = 0; q < w; q++) row[q] = xs[q][t];
     }
-    __syncthreads();
   }
 }
 
@@ -745,6 +784,119 @@ __global__ __launch_bounds__(256) void k_fa_gemm(const DNode *nodes, const int32
   }
 }
 
+// The same update on the FP64 matrix cores: v_mfma_f64_16x16x4f64 takes A
+// (16 x 4) as lane l -> A[l % 16][l / 16], B (4 x 16) as lane l ->
+// B[l / 16][l % 16] and returns D (16 x 16) as lane l, register v ->
+// D[l / 16 + 4 v][l % 16] (checked on the device against a host product).  A complex product is four real MFMAs:
+// Cre += Are Bre - Aim Bim, Cim += Are Bim + Aim Bre.  Each of the four waves
+// owns a 32 x 32 quarter of the 64 x 64 tile (2 x 2 MFMA blocks).
+typedef double mvf_d4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                      double2 *fac, double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
+  double2 *T = fac + nd.fac;
+  const double2 *A, *B;
+  double2 *C;
+  int64_t Mr, Nc, Kd;
+  bool lower = false;
+  if (MODE == 0) {
+    const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
+    if (k0 >= n1) return;
+    A = T + k1 * m + k0;
+    B = T + k0 * m + k1;
+    C = T + k1 * m + k1;
+    Mr = n1 - k1;
+    Nc = m - k1;
+    Kd = k1 - k0;
+  } else if (MODE == 1) {
+    const int64_t r1 = n1 - step * kNB;
+    if (r1 <= 0) return;
+    const int64_t r0 = r1 > kNB ? r1 - kNB : 0;
+    A = T + r0;
+    B = T + r0 * m + n1;
+    C = T + n1;
+    Mr = r0;
+    Nc = n2;
+    Kd = r1 - r0;
+  } else {
+    double2 *Bb = pool + nd.pool;
+    A = Bb;
+    B = T + n1;
+    C = Bb + n1;
+    Mr = n2;
+    Nc = n2;
+    Kd = n1;
+    lower = true;
+  }
+  if (Mr <= 0 || Nc <= 0 || Kd <= 0) return;
+  const int64_t tn = (Nc + kTile - 1) / kTile, tm = (Mr + kTile - 1) / kTile;
+  __shared__ double2 As[kTile][17];
+  __shared__ double2 Bs[16][kTile + 1];
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;  // wave's quarter of the tile
+  const int li = l & 15, lk = l >> 4;
+  for (int64_t tile = blockIdx.x; tile < tm * tn; tile += gridDim.x) {
+    const int64_t ti = tile / tn, tj = tile % tn;
+    if (lower && tj > ti) continue;
+    const int64_t i0 = ti * kTile, j0 = tj * kTile;
+    mvf_d4 cre[2][2], cim[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        cre[a][b] = mvf_d4{0.0, 0.0, 0.0, 0.0};
+        cim[a][b] = mvf_d4{0.0, 0.0, 0.0, 0.0};
+      }
+    for (int64_t kk = 0; kk < Kd; kk += 16) {
+#pragma unroll
+      for (int e = tid; e < kTile * 16; e += 256) {
+        const int r = e >> 4, q = e & 15;
+        As[r][q] = (i0 + r < Mr && kk + q < Kd) ? A[(i0 + r) * m + kk + q] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int e = tid; e < kTile * 16; e += 256) {
+        const int q = e >> 6, c = e & 63;
+        Bs[q][c] = (kk + q < Kd && j0 + c < Nc) ? B[(kk + q) * m + j0 + c] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < 16; ks += 4) {
+        double2 a[2], b[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) a[rb] = As[wr + rb * 16 + li][ks + lk];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) b[cb] = Bs[ks + lk][wc + cb * 16 + li];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+          for (int cb = 0; cb < 2; cb++) {
+            cre[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].x, b[cb].x, cre[rb][cb], 0, 0, 0);
+            cre[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[rb].y, b[cb].y, cre[rb][cb], 0, 0, 0);
+            cim[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].x, b[cb].y, cim[rb][cb], 0, 0, 0);
+            cim[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].y, b[cb].x, cim[rb][cb], 0, 0, 0);
+          }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int64_t i = i0 + wr + rb * 16 + lk + 4 * v, j = j0 + wc + cb * 16 + li;
+          if (i < Mr && j < Nc) {
+            double2 &o = C[i * m + j];
+            o.x -= cre[rb][cb][v];
+            o.y -= cim[rb][cb][v];
+          }
+        }
+  }
+}
+
 // row permutation of each front from its interchange sequence
 __global__ void k_fa_perm(const DNode *nodes, const int32_t *lev, int nl, const int32_t *piv, int32_t *perm) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -860,15 +1012,14 @@ __global__ __launch_bounds__(64) void k_sv_ldiag(const DNode *nodes, const int32
     z[r][c] = r < w ? zb[(nd.pos0 + b0 + r) * NC + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  if (threadIdx.x < NC) {
-    const int c = threadIdx.x;
-    for (int r = 1; r < w; r++) {
-      double2 v = z[r][c];
-      for (int q = 0; q < r; q++) cfms(v, L[r][q], z[q][c]);
-      z[r][c] = v;
+  // column-oriented: after z[q] is final, every lower row subtracts its share
+  for (int q = 0; q + 1 < w; q++) {
+    for (int e = threadIdx.x; e < NB * NC; e += 64) {
+      const int r = e % NB, c = e / NB;
+      if (r > q && r < w) cfms(z[r][c], L[r][q], z[q][c]);
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int e = threadIdx.x; e < w * NC; e += 64) {
     const int r = e / NC, c = e % NC;
     zb[(nd.pos0 + b0 + r) * NC + c] = z[r][c];
@@ -928,15 +1079,18 @@ __global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32
     z[r][c] = r < w ? zb[(nd.pos0 + r0 + r) * NC + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  if (threadIdx.x < NC) {
-    const int c = threadIdx.x;
-    for (int r = w - 1; r >= 0; r--) {
-      double2 v = z[r][c];
-      for (int q = r + 1; q < w; q++) cfms(v, U[r][q], z[q][c]);
-      z[r][c] = cmul(v, cinv(U[r][r]));
+  for (int q = w - 1; q >= 0; q--) {
+    for (int e = threadIdx.x; e < NB * NC; e += 64) {
+      const int r = e % NB, c = e / NB;
+      if (r == q) z[r][c] = cmul(z[r][c], cinv(U[q][q]));
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NB * NC; e += 64) {
+      const int r = e % NB, c = e / NB;
+      if (r < q) cfms(z[r][c], U[r][q], z[q][c]);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int e = threadIdx.x; e < w * NC; e += 64) {
     const int r = e / NC, c = e % NC;
     zb[(nd.pos0 + r0 + r) * NC + c] = z[r][c];
@@ -1105,6 +1259,26 @@ static int direct_plan(mvf_handle *h) {
 }
 
 // numeric factorization of A_pc at angular frequency w (fl32(omega) as in the operator)
+// GEMM variant: matrix cores by default; MVF_DIRECT_GEMM=valu selects the
+// vector-ALU kernel (same tiles, same results up to rounding)
+static bool gemm_on_valu() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("MVF_DIRECT_GEMM");
+    v = (e && strcmp(e, "valu") == 0) ? 1 : 0;
+  }
+  return v == 1;
+}
+
+template <int MODE>
+static void launch_gemm(unsigned gx, unsigned gy, hipStream_t st, const DNode *nodes, const int32_t *lev,
+                        int64_t step, double2 *fac, double2 *pool) {
+  if (gemm_on_valu())
+    hipLaunchKernelGGL((k_fa_gemm<MODE>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
+  else
+    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
+}
+
 static int direct_factor(mvf_handle *h, double w, int c64) {
   DirectPlan *P = (DirectPlan *)h->direct;
   hipStream_t st = h->stream;
@@ -1162,16 +1336,24 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           gc = std::max(gc, n.m);
           gfl += 8.0 * (double)(n.n1 - k1) * (double)(n.m - k1) * (double)(k1 - k);
         }
-        hipLaunchKernelGGL((k_fa_panel<kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_piv,
+        hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_piv,
                            P->d_npert, big, tiny);
+        int64_t gr = 0;
+        for (int t : P->levels[d]) {
+          const DNode &n = P->nodes[t];
+          if (k >= n.n1) continue;
+          gr = std::max(gr, n.n1 - std::min(k + kNB, n.n1));
+        }
+        if (gr > 0)
+          hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + 127) / 128), nl), dim3(128), 0, st, P->d_nodes,
+                             lev, k, P->d_fac);
         hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
                            k, P->d_fac, P->d_piv);
         hipLaunchKernelGGL((k_fa_trsm_u<kNB>), dim3(cap_grid((gc - k + 127) / 128), nl), dim3(128), 0, st,
                            P->d_nodes, lev, k, P->d_fac);
         if (gt > 0) {
           gemm_begin(gfl);
-          hipLaunchKernelGGL((k_fa_gemm<0>), dim3(cap_grid(gt), nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac,
-                             pool);
+          launch_gemm<0>(cap_grid(gt), nl, st, P->d_nodes, lev, k, P->d_fac, pool);
           gemm_end();
         }
       }
@@ -1192,8 +1374,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                              lev, s, P->d_fac);
           if (gt > 0) {
             gemm_begin(gfl);
-            hipLaunchKernelGGL((k_fa_gemm<1>), dim3(cap_grid(gt), nl), dim3(256), 0, st, P->d_nodes, lev, s,
-                               P->d_fac, pool);
+            launch_gemm<1>(cap_grid(gt), nl, st, P->d_nodes, lev, s, P->d_fac, pool);
             gemm_end();
           }
         }
@@ -1204,8 +1385,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           sfl += 8.0 * (double)n.n2 * (double)(n.n2 + 1) / 2.0 * (double)n.n1;
         }
         gemm_begin(sfl);
-        hipLaunchKernelGGL((k_fa_gemm<2>), dim3(cap_grid(tq * tq), nl), dim3(256), 0, st, P->d_nodes, lev, 0,
-                           P->d_fac, pool);
+        launch_gemm<2>(cap_grid(tq * tq), nl, st, P->d_nodes, lev, 0, P->d_fac, pool);
         gemm_end();
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);

@@ -30,3 +30,27 @@ def determined_dofs(m, gne) -> np.ndarray:
     g = gne[ce]
     mask[g[g > 0] - 1] = True
     return mask
+
+
+def inner_dofs(m, gne) -> np.ndarray:
+    """DOFs touched only by elements outside the GPML zones.  The reference's
+    GPML is one-sided: zones are the last nextd element layers on +x, +y and
+    +z (boundary_conds.f90:53-81).  Inside those zones the real-part-only
+    stretch leaves modes with mass and almost no stiffness, which a solution at
+    a given residual determines only weakly; the fields a survey reads lie
+    outside them."""
+    nne = int(gne.max())
+    ie = np.repeat(np.arange(1, m.nx + 1), m.ny * m.nz)
+    je = np.tile(np.repeat(np.arange(1, m.ny + 1), m.nz), m.nx)
+    ke = np.tile(np.arange(1, m.nz + 1), m.nx * m.ny)
+    if m.dirichlet:
+        pml = np.zeros(m.ne, bool)
+    else:
+        pml = (ie > m.nx - m.nextd) | (je > m.ny - m.nextd) | (ke > m.nz - m.nextd)
+    bad = np.zeros(nne + 1, bool)
+    bad[np.maximum(gne[pml].ravel(), 0)] = True
+    ok = np.zeros(nne + 1, bool)
+    ok[np.maximum(gne[~pml].ravel(), 0)] = True
+    out = ok & ~bad
+    return out[1:]
+
