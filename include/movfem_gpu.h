@@ -152,6 +152,9 @@ typedef struct mvf_solver_opts {
   double inner_tol;     /* MIXED: FP32 residual drop before an FP64 refinement step */
   int32_t use_graph;    /* capture the iteration block in a HIP graph */
   int32_t gmres_m;      /* GMRES restart length (MVF_PC_DIRECT), 0 = 30 */
+  int32_t freqs_per_factor; /* MVF_PC_DIRECT in mvf_solve_frequencies: consecutive frequencies sharing one
+                               factorization (at their geometric-mean frequency), 0 or 1 = one each */
+  int32_t pad_;
 } mvf_solver_opts;
 
 typedef struct mvf_solve_info {

@@ -528,8 +528,8 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
   while (f < nfreq && !rc) {
     int nb = nfreq - f;
     if (h->carry_nonzero && h->calls == 0) nb = 1;
-    // the direct preconditioner is factored for one frequency: one frequency per batch
-    if (o->precond == MVF_PC_DIRECT) nb = 1;
+    // the direct preconditioner serves freqs_per_factor consecutive frequencies
+    if (o->precond == MVF_PC_DIRECT) nb = std::min(nb, std::max(1, std::min(o->freqs_per_factor, 8)));
     MVF_TRY(ensure_call_state(h, h->calls));
     int pc = pad_cols(2 * nb);
     double *bb, *xb;

@@ -45,7 +45,9 @@
 namespace mvf {
 
 constexpr int kLeafMax = 16;       // elements per leaf box
-constexpr int kNB = 32;            // panel width / triangular block size
+constexpr int kNB = 64;            // panel width / triangular block size
+// threads of the kernels that keep one column (row) per thread in LDS
+constexpr int kColT = kNB <= 32 ? 128 : 64;
 constexpr double kRegEps = 1e-3;   // air regularization relative to the smallest omega*M/K
 constexpr int kTile = 64;          // GEMM tile edge
 
@@ -563,17 +565,17 @@ __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32
 // L rows below the diagonal block: T[i, k:kend] = T[i, k:kend] U_kk^-1 for
 // i in [kend, n1); each thread owns one row, held in LDS
 template <int NB>
-__global__ __launch_bounds__(128) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
+__global__ __launch_bounds__(kColT) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
   const DNode nd = nodes[lev[blockIdx.y]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
-  if (kend + (int64_t)blockIdx.x * 128 >= nd.n1) return;
+  if (kend + (int64_t)blockIdx.x * kColT >= nd.n1) return;
   const int w = (int)(kend - k);
   double2 *T = fac + nd.fac;
   __shared__ double2 U[NB][NB + 1];
   __shared__ double2 dinv[NB];
-  __shared__ double2 xs[NB][129];
-  for (int e = threadIdx.x; e < NB * NB; e += 128) {
+  __shared__ double2 xs[NB][kColT + 1];
+  for (int e = threadIdx.x; e < NB * NB; e += kColT) {
     const int r = e / NB, c = e % NB;
     U[r][c] = (r < w && c < w && c >= r) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
   }
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(128) void k_fa_prows(const DNode *nodes, const int3
   if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
   __syncthreads();
   const int t = threadIdx.x;
-  for (int64_t i0 = kend + (int64_t)blockIdx.x * 128; i0 < nd.n1; i0 += (int64_t)gridDim.x * 128) {
+  for (int64_t i0 = kend + (int64_t)blockIdx.x * kColT; i0 < nd.n1; i0 += (int64_t)gridDim.x * kColT) {
     const int64_t i = i0 + t;
     if (i < nd.n1) {
       double2 *row = T + i * m + k;
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(256) void k_fa_swap(const DNode *nodes, const int32
 // U row block: T[k:kend, kend:m] = L_kk^-1 T[k:kend, kend:m] (unit lower).
 // Each of the 128 threads owns one column, held in LDS.
 template <int NB>
-__global__ __launch_bounds__(128) void k_fa_trsm_u(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
+__global__ __launch_bounds__(kColT) void k_fa_trsm_u(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
   const DNode nd = nodes[lev[blockIdx.y]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
@@ -628,14 +630,14 @@ __global__ __launch_bounds__(128) void k_fa_trsm_u(const DNode *nodes, const int
   const int w = (int)(kend - k);
   double2 *T = fac + nd.fac;
   __shared__ double2 L[NB][NB + 1];
-  __shared__ double2 xs[NB][129];
-  for (int e = threadIdx.x; e < NB * NB; e += 128) {
+  __shared__ double2 xs[NB][kColT + 1];
+  for (int e = threadIdx.x; e < NB * NB; e += kColT) {
     const int r = e / NB, c = e % NB;
     L[r][c] = (r < w && c < w) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   const int t = threadIdx.x;
-  for (int64_t j0 = kend + (int64_t)blockIdx.x * 128; j0 < m; j0 += (int64_t)gridDim.x * 128) {
+  for (int64_t j0 = kend + (int64_t)blockIdx.x * kColT; j0 < m; j0 += (int64_t)gridDim.x * kColT) {
     const int64_t j = j0 + t;
     if (j < m) {
       for (int r = 0; r < w; r++) xs[r][t] = T[(k + r) * m + j];
@@ -651,7 +653,7 @@ __global__ __launch_bounds__(128) void k_fa_trsm_u(const DNode *nodes, const int
 
 // Y block rows from the bottom: T[r0:r1, n1:m] = U_bb^-1 T[r0:r1, n1:m]
 template <int NB>
-__global__ __launch_bounds__(128) void k_fa_trsm_y(const DNode *nodes, const int32_t *lev, int64_t step,
+__global__ __launch_bounds__(kColT) void k_fa_trsm_y(const DNode *nodes, const int32_t *lev, int64_t step,
                                                    double2 *fac) {
   const DNode nd = nodes[lev[blockIdx.y]];
   const int64_t r1 = nd.n1 - step * NB;
@@ -661,8 +663,8 @@ __global__ __launch_bounds__(128) void k_fa_trsm_y(const DNode *nodes, const int
   double2 *T = fac + nd.fac;
   __shared__ double2 U[NB][NB + 1];
   __shared__ double2 dinv[NB];
-  __shared__ double2 xs[NB][129];
-  for (int e = threadIdx.x; e < NB * NB; e += 128) {
+  __shared__ double2 xs[NB][kColT + 1];
+  for (int e = threadIdx.x; e < NB * NB; e += kColT) {
     const int r = e / NB, c = e % NB;
     U[r][c] = (r < w && c < w && c >= r) ? T[(r0 + r) * m + r0 + c] : make_double2(0.0, 0.0);
   }
@@ -670,7 +672,7 @@ __global__ __launch_bounds__(128) void k_fa_trsm_y(const DNode *nodes, const int
   if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
   __syncthreads();
   const int t = threadIdx.x;
-  for (int64_t j0 = nd.n1 + (int64_t)blockIdx.x * 128; j0 < m; j0 += (int64_t)gridDim.x * 128) {
+  for (int64_t j0 = nd.n1 + (int64_t)blockIdx.x * kColT; j0 < m; j0 += (int64_t)gridDim.x * kColT) {
     const int64_t j = j0 + t;
     if (j < m) {
       for (int r = 0; r < w; r++) xs[r][t] = T[(r0 + r) * m + j];
@@ -1345,11 +1347,11 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           gr = std::max(gr, n.n1 - std::min(k + kNB, n.n1));
         }
         if (gr > 0)
-          hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + 127) / 128), nl), dim3(128), 0, st, P->d_nodes,
-                             lev, k, P->d_fac);
+          hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
+                             P->d_nodes, lev, k, P->d_fac);
         hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
                            k, P->d_fac, P->d_piv);
-        hipLaunchKernelGGL((k_fa_trsm_u<kNB>), dim3(cap_grid((gc - k + 127) / 128), nl), dim3(128), 0, st,
+        hipLaunchKernelGGL((k_fa_trsm_u<kNB>), dim3(cap_grid((gc - k + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
                            P->d_nodes, lev, k, P->d_fac);
         if (gt > 0) {
           gemm_begin(gfl);
@@ -1370,8 +1372,8 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
             gt = std::max(gt, ((r0 + kTile - 1) / kTile) * ((n.n2 + kTile - 1) / kTile));
             gfl += 8.0 * (double)r0 * (double)n.n2 * (double)(r1 - r0);
           }
-          hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + 127) / 128), nl), dim3(128), 0, st, P->d_nodes,
-                             lev, s, P->d_fac);
+          hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
+                             P->d_nodes, lev, s, P->d_fac);
           if (gt > 0) {
             gemm_begin(gfl);
             launch_gemm<1>(cap_grid(gt), nl, st, P->d_nodes, lev, s, P->d_fac, pool);
@@ -1547,7 +1549,13 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   const int mres = o->gmres_m > 0 ? std::min(o->gmres_m, 64) : 30;
   MVF_TRY(direct_plan(h));
   DirectPlan *P = (DirectPlan *)h->direct;
-  const double w0 = S.w.w[0];
+  // factor at the geometric mean of the batch's frequencies
+  double wlo = S.w.w[0], whi = S.w.w[0];
+  for (int c = 1; c < S.ncol; c++) {
+    wlo = std::min(wlo, S.w.w[c]);
+    whi = std::max(whi, S.w.w[c]);
+  }
+  const double w0 = std::sqrt(wlo * whi);
   if (!P->factored || P->omega != w0 || P->c64 != (int)S.c64) MVF_TRY(direct_factor(h, w0, (int)S.c64));
   info->factor_seconds += P->t_factor;
   info->factorizations = (int32_t)P->factorizations;
