@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--maxit", type=int, default=3000)
+    ap.add_argument("--nfreq", type=int, default=0, help="solve only the first N frequencies (profiling runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -153,7 +154,7 @@ def main():
     from movfem_amd.vfem import GpuVfem, MvfError
 
     m, desc = workload(args.workload)
-    freqs = list(m.freqs)
+    freqs = list(m.freqs)[:args.nfreq] if args.nfreq > 0 else list(m.freqs)
     # The reference's update_sigma refreshes only the first npt+30 linear
     # entries of sigma (geometry.f90:146-152): from the second frequency on,
     # air keeps the first frequency's i*omega*eps and the right-hand side gets a

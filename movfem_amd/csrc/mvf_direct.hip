@@ -46,10 +46,10 @@ namespace mvf {
 
 constexpr int kLeafMax = 16;       // elements per leaf box
 constexpr int kNB = 64;            // panel width / triangular block size
-// threads of the kernels that keep one column (row) per thread in LDS
-constexpr int kColT = kNB <= 32 ? 128 : 64;
 constexpr double kRegEps = 1e-3;   // air regularization relative to the smallest omega*M/K
 constexpr int kTile = 64;          // GEMM tile edge
+// threads of the kernels that keep one column (row) per thread in LDS
+constexpr int kColT = kNB <= 32 ? 128 : 64;
 
 struct DNode {
   int64_t n1, n2, m;
@@ -59,6 +59,7 @@ struct DNode {
   int64_t rel;   // offset of the parent-relative map of the update DOFs (n2 entries)
   int64_t pos0;  // elimination position of the first owned DOF (also pivot offset)
   int64_t sv;    // offset (rows) of the node's solve work vector (m rows)
+  int64_t inv;   // offset of the inverses of the unit-lower diagonal blocks of L (w^2 per panel block)
   int32_t child[2];
   int32_t parent;
   int32_t depth;
@@ -70,7 +71,7 @@ struct DirectPlan {
   std::vector<DNode> nodes;
   std::vector<std::vector<int32_t>> levels;
   std::vector<int64_t> lev_off, lev_pool, lev_fac0, lev_fac1, lev_maxn1, lev_maxn2, lev_maxm;
-  int64_t fac_entries = 0, sv_rows = 0, nidx = 0, nrel = 0;
+  int64_t fac_entries = 0, sv_rows = 0, nidx = 0, nrel = 0, inv_entries = 0;
   double flops = 0.0;
   double minMK = 1.0, maxK = 0.0, maxM = 0.0;
   int64_t nne = 0;
@@ -78,7 +79,7 @@ struct DirectPlan {
   DNode *d_nodes = nullptr;
   int32_t *d_lev = nullptr, *d_idx = nullptr, *d_fpos = nullptr, *d_rel = nullptr, *d_dst = nullptr;
   int32_t *d_piv = nullptr, *d_perm = nullptr, *d_pos = nullptr;
-  double2 *d_fac = nullptr;
+  double2 *d_fac = nullptr, *d_inv = nullptr;
   int *d_npert = nullptr;
   double2 *d_sv = nullptr, *d_zb = nullptr;
   int sv_cols = 0;
@@ -273,7 +274,7 @@ static int build_plan_host(DirectPlan &P, int nx, int ny, int nz, int me, const 
   P.maxdepth = maxd;
   P.levels.assign(maxd + 1, {});
   for (int t = 0; t < T; t++) P.levels[tb.depth[t]].push_back(t);
-  int64_t idxoff = 0, reloff = 0, svoff = 0;
+  int64_t idxoff = 0, reloff = 0, svoff = 0, invoff = 0;
   for (int t = 0; t < T; t++) {
     DNode &n = P.nodes[t];
     n.n1 = optr[t + 1] - optr[t];
@@ -286,6 +287,12 @@ static int build_plan_host(DirectPlan &P, int nx, int ny, int nz, int me, const 
     n.pos0 = optr[t];
     n.sv = svoff;
     svoff += n.m;
+    {
+      // panel blocks of width kNB, the last one n1 - kNB*(nb-1) wide
+      const int64_t nbk = (n.n1 + kNB - 1) / kNB, wl = n.n1 - kNB * (nbk - 1);
+      n.inv = invoff;
+      invoff += nbk > 0 ? kNB * kNB * (nbk - 1) + wl * wl : 0;
+    }
     n.child[0] = tb.child[t][0];
     n.child[1] = tb.child[t][1];
     n.parent = tb.parent[t];
@@ -300,6 +307,7 @@ static int build_plan_host(DirectPlan &P, int nx, int ny, int nz, int me, const 
   }
   P.nidx = idxoff;
   P.nrel = reloff;
+  P.inv_entries = invoff;
   P.sv_rows = svoff;
   P.lev_off.assign(maxd + 2, 0);
   P.lev_pool.assign(maxd + 1, 0);
@@ -502,7 +510,7 @@ __global__ __launch_bounds__(256) void k_fa_sym(const DNode *nodes, const int32_
 // A pivot below `tiny` (a singular block) is replaced by `big` and counted.
 template <int NB>
 __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
-                                                 int32_t *piv, int *npert, double big, double tiny) {
+                                                 double2 *inv, int32_t *piv, int *npert, double big, double tiny) {
   const DNode nd = nodes[lev[blockIdx.x]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
@@ -560,6 +568,22 @@ __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32
     const int r = e / w, c = e % w;
     T[(k + r) * m + k + c] = a[r][c];
   }
+  // inverse of the unit lower L_kk (thread t owns column t).  L's multipliers
+  // are bounded by 1, so the explicit inverse is safe; U_kk can hold pivots
+  // ~1e-11 of the largest (the air shift) and is only ever used by
+  // substitution.
+  __shared__ double2 x[NB][NB + 1];
+  double2 *Li = inv + nd.inv + (int64_t)NB * NB * (k / NB);
+  if (t < w) {
+    for (int r = 0; r < w; r++) x[r][t] = make_double2(r == t ? 1.0 : 0.0, 0.0);
+    for (int r = t + 1; r < w; r++) {
+      double2 v = make_double2(0.0, 0.0);
+      for (int p = t; p < r; p++) cfms(v, a[r][p], x[p][t]);
+      x[r][t] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < w * w; e += 64) Li[e] = x[e / w][e % w];
 }
 
 // L rows below the diagonal block: T[i, k:kend] = T[i, k:kend] U_kk^-1 for
@@ -598,67 +622,15 @@ __global__ __launch_bounds__(kColT) void k_fa_prows(const DNode *nodes, const in
   }
 }
 
-// the panel's row interchanges applied to the top band outside the panel
-template <int NB>
-__global__ __launch_bounds__(256) void k_fa_swap(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
-                                                 const int32_t *piv) {
-  const DNode nd = nodes[lev[blockIdx.y]];
-  if (k >= nd.n1) return;
-  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
-  double2 *T = fac + nd.fac;
-  for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += (int64_t)gridDim.x * 256) {
-    if (j >= k && j < kend) continue;
-    for (int64_t q = k; q < kend; q++) {
-      const int64_t r = piv[nd.pos0 + q];
-      if (r != q) {
-        const double2 a = T[q * m + j];
-        T[q * m + j] = T[r * m + j];
-        T[r * m + j] = a;
-      }
-    }
-  }
-}
-
-// U row block: T[k:kend, kend:m] = L_kk^-1 T[k:kend, kend:m] (unit lower).
-// Each of the 128 threads owns one column, held in LDS.
-template <int NB>
-__global__ __launch_bounds__(kColT) void k_fa_trsm_u(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
-  const DNode nd = nodes[lev[blockIdx.y]];
-  if (k >= nd.n1) return;
-  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
-  if (kend >= m) return;
-  const int w = (int)(kend - k);
-  double2 *T = fac + nd.fac;
-  __shared__ double2 L[NB][NB + 1];
-  __shared__ double2 xs[NB][kColT + 1];
-  for (int e = threadIdx.x; e < NB * NB; e += kColT) {
-    const int r = e / NB, c = e % NB;
-    L[r][c] = (r < w && c < w) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  const int t = threadIdx.x;
-  for (int64_t j0 = kend + (int64_t)blockIdx.x * kColT; j0 < m; j0 += (int64_t)gridDim.x * kColT) {
-    const int64_t j = j0 + t;
-    if (j < m) {
-      for (int r = 0; r < w; r++) xs[r][t] = T[(k + r) * m + j];
-      for (int r = 1; r < w; r++) {
-        double2 v = xs[r][t];
-        for (int c = 0; c < r; c++) cfms(v, L[r][c], xs[c][t]);
-        xs[r][t] = v;
-      }
-      for (int r = 0; r < w; r++) T[(k + r) * m + j] = xs[r][t];
-    }
-  }
-}
-
-// Y block rows from the bottom: T[r0:r1, n1:m] = U_bb^-1 T[r0:r1, n1:m]
+// Y block rows by back substitution, panel blocks from the last one (b = nb-1-step):
+// T[r0:r0+w, n1:m] = U_bb^-1 T[r0:r0+w, n1:m]
 template <int NB>
 __global__ __launch_bounds__(kColT) void k_fa_trsm_y(const DNode *nodes, const int32_t *lev, int64_t step,
                                                    double2 *fac) {
   const DNode nd = nodes[lev[blockIdx.y]];
-  const int64_t r1 = nd.n1 - step * NB;
-  if (r1 <= 0 || nd.n2 == 0) return;
-  const int64_t r0 = r1 > NB ? r1 - NB : 0, m = nd.m;
+  const int64_t b = (nd.n1 + NB - 1) / NB - 1 - step;
+  if (b < 0 || nd.n2 == 0) return;
+  const int64_t r0 = b * NB, r1 = r0 + NB < nd.n1 ? r0 + NB : nd.n1, m = nd.m;
   const int w = (int)(r1 - r0);
   double2 *T = fac + nd.fac;
   __shared__ double2 U[NB][NB + 1];
@@ -686,107 +658,28 @@ __global__ __launch_bounds__(kColT) void k_fa_trsm_y(const DNode *nodes, const i
   }
 }
 
-// batched complex GEMM update C -= A B on 64 x 64 tiles, one front per blockIdx.y
-//   MODE 0: LU trailing update of the top band after panel [k0, k1)
-//   MODE 1: Y solve, rows above the solved block rows [k0, k1)
-//   MODE 2: Schur complement F22 -= F21 Y (lower tiles only)
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fa_gemm(const DNode *nodes, const int32_t *lev, int64_t step, double2 *fac,
-                                                 double2 *pool) {
+// the panel's row interchanges applied to the top band outside the panel
+template <int NB>
+__global__ __launch_bounds__(256) void k_fa_swap(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
+                                                 const int32_t *piv) {
   const DNode nd = nodes[lev[blockIdx.y]];
-  const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
+  if (k >= nd.n1) return;
+  const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
   double2 *T = fac + nd.fac;
-  const double2 *A, *B;
-  double2 *C;
-  int64_t Mr, Nc, Kd;
-  bool lower = false;
-  if (MODE == 0) {
-    const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
-    if (k0 >= n1) return;
-    A = T + k1 * m + k0;
-    B = T + k0 * m + k1;
-    C = T + k1 * m + k1;
-    Mr = n1 - k1;
-    Nc = m - k1;
-    Kd = k1 - k0;
-  } else if (MODE == 1) {
-    const int64_t r1 = n1 - step * kNB;
-    if (r1 <= 0) return;
-    const int64_t r0 = r1 > kNB ? r1 - kNB : 0;
-    A = T + r0;
-    B = T + r0 * m + n1;
-    C = T + n1;
-    Mr = r0;
-    Nc = n2;
-    Kd = r1 - r0;
-  } else {
-    double2 *Bb = pool + nd.pool;
-    A = Bb;
-    B = T + n1;
-    C = Bb + n1;
-    Mr = n2;
-    Nc = n2;
-    Kd = n1;
-    lower = true;
-  }
-  if (Mr <= 0 || Nc <= 0 || Kd <= 0) return;
-  const int64_t tn = (Nc + kTile - 1) / kTile, tm = (Mr + kTile - 1) / kTile;
-  __shared__ double2 As[kTile][17];
-  __shared__ double2 Bs[16][kTile + 1];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  for (int64_t tile = blockIdx.x; tile < tm * tn; tile += gridDim.x) {
-  const int64_t ti = tile / tn, tj = tile % tn;
-  if (lower && tj > ti) continue;
-  const int64_t i0 = ti * kTile, j0 = tj * kTile;
-  double2 acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = make_double2(0.0, 0.0);
-  for (int64_t kk = 0; kk < Kd; kk += 16) {
-#pragma unroll
-    for (int e = tid; e < kTile * 16; e += 256) {
-      const int r = e >> 4, q = e & 15;
-      As[r][q] = (i0 + r < Mr && kk + q < Kd) ? A[(i0 + r) * m + kk + q] : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-    for (int e = tid; e < kTile * 16; e += 256) {
-      const int q = e >> 6, c = e & 63;
-      Bs[q][c] = (kk + q < Kd && j0 + c < Nc) ? B[(kk + q) * m + j0 + c] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      double2 a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) a[r] = As[ty * 4 + r][q];
-#pragma unroll
-      for (int c = 0; c < 4; c++) b[c] = Bs[q][tx * 4 + c];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) cfma(acc[r][c], a[r], b[c]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int64_t i = i0 + ty * 4 + r;
-    if (i >= Mr) continue;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int64_t j = j0 + tx * 4 + c;
-      if (j < Nc) {
-        double2 &o = C[i * m + j];
-        o.x -= acc[r][c].x;
-        o.y -= acc[r][c].y;
+  for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += (int64_t)gridDim.x * 256) {
+    if (j >= k && j < kend) continue;
+    for (int64_t q = k; q < kend; q++) {
+      const int64_t r = piv[nd.pos0 + q];
+      if (r != q) {
+        const double2 a = T[q * m + j];
+        T[q * m + j] = T[r * m + j];
+        T[r * m + j] = a;
       }
     }
   }
-  }
 }
 
-// The same update on the FP64 matrix cores: v_mfma_f64_16x16x4f64 takes A
+// Batched complex GEMM updates on the FP64 matrix cores: v_mfma_f64_16x16x4f64 takes A
 // (16 x 4) as lane l -> A[l % 16][l / 16], B (4 x 16) as lane l ->
 // B[l / 16][l % 16] and returns D (16 x 16) as lane l, register v ->
 // D[l / 16 + 4 v][l % 16] (checked on the device against a host product).  A complex product is four real MFMAs:
@@ -794,46 +687,14 @@ __global__ __launch_bounds__(256) void k_fa_gemm(const DNode *nodes, const int32
 // owns a 32 x 32 quarter of the 64 x 64 tile (2 x 2 MFMA blocks).
 typedef double mvf_d4 __attribute__((ext_vector_type(4)));
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev, int64_t step,
-                                                      double2 *fac, double2 *pool) {
-  const DNode nd = nodes[lev[blockIdx.y]];
-  const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
-  double2 *T = fac + nd.fac;
-  const double2 *A, *B;
-  double2 *C;
-  int64_t Mr, Nc, Kd;
-  bool lower = false;
-  if (MODE == 0) {
-    const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
-    if (k0 >= n1) return;
-    A = T + k1 * m + k0;
-    B = T + k0 * m + k1;
-    C = T + k1 * m + k1;
-    Mr = n1 - k1;
-    Nc = m - k1;
-    Kd = k1 - k0;
-  } else if (MODE == 1) {
-    const int64_t r1 = n1 - step * kNB;
-    if (r1 <= 0) return;
-    const int64_t r0 = r1 > kNB ? r1 - kNB : 0;
-    A = T + r0;
-    B = T + r0 * m + n1;
-    C = T + n1;
-    Mr = r0;
-    Nc = n2;
-    Kd = r1 - r0;
-  } else {
-    double2 *Bb = pool + nd.pool;
-    A = Bb;
-    B = T + n1;
-    C = Bb + n1;
-    Mr = n2;
-    Nc = n2;
-    Kd = n1;
-    lower = true;
-  }
-  if (Mr <= 0 || Nc <= 0 || Kd <= 0) return;
+// C (Mr x Nc) -= A (Mr x Kd) B (Kd x Nc), or C = A B with OVERWRITE (in place
+// over B allowed: each workgroup reads its whole B column tile before it
+// writes); row-major operands with leading dimensions lda, ldb, ldc; 64 x 64
+// tiles, tile t handled by workgroup t mod gridDim.x; LOWER skips tiles above
+// the diagonal.
+template <bool OVERWRITE>
+__device__ inline void mfma_tiles(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C,
+                                  int64_t ldc, int64_t Mr, int64_t Nc, int64_t Kd, bool lower) {
   const int64_t tn = (Nc + kTile - 1) / kTile, tm = (Mr + kTile - 1) / kTile;
   __shared__ double2 As[kTile][17];
   __shared__ double2 Bs[16][kTile + 1];
@@ -856,12 +717,12 @@ __global__ __launch_bounds__(256) void k_fa_gemm_mfma(const DNode *nodes, const 
 #pragma unroll
       for (int e = tid; e < kTile * 16; e += 256) {
         const int r = e >> 4, q = e & 15;
-        As[r][q] = (i0 + r < Mr && kk + q < Kd) ? A[(i0 + r) * m + kk + q] : make_double2(0.0, 0.0);
+        As[r][q] = (i0 + r < Mr && kk + q < Kd) ? A[(i0 + r) * lda + kk + q] : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int e = tid; e < kTile * 16; e += 256) {
         const int q = e >> 6, c = e & 63;
-        Bs[q][c] = (kk + q < Kd && j0 + c < Nc) ? B[(kk + q) * m + j0 + c] : make_double2(0.0, 0.0);
+        Bs[q][c] = (kk + q < Kd && j0 + c < Nc) ? B[(kk + q) * ldb + j0 + c] : make_double2(0.0, 0.0);
       }
       __syncthreads();
 #pragma unroll
@@ -891,13 +752,57 @@ __global__ __launch_bounds__(256) void k_fa_gemm_mfma(const DNode *nodes, const 
         for (int v = 0; v < 4; v++) {
           const int64_t i = i0 + wr + rb * 16 + lk + 4 * v, j = j0 + wc + cb * 16 + li;
           if (i < Mr && j < Nc) {
-            double2 &o = C[i * m + j];
-            o.x -= cre[rb][cb][v];
-            o.y -= cim[rb][cb][v];
+            double2 &o = C[i * ldc + j];
+            if (OVERWRITE) {
+              o = make_double2(cre[rb][cb][v], cim[rb][cb][v]);
+            } else {
+              o.x -= cre[rb][cb][v];
+              o.y -= cim[rb][cb][v];
+            }
           }
         }
   }
 }
+
+//   MODE 0: LU trailing update of the top band after panel [k0, k1)
+//   MODE 1: Y solve, rows above panel block b = nb-1-step
+//   MODE 2: Schur complement F22 -= F21 Y (lower tiles only)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                      double2 *fac, double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
+  double2 *T = fac + nd.fac;
+  if (MODE == 0) {
+    const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
+    if (k0 >= n1) return;
+    mfma_tiles<false>(T + k1 * m + k0, m, T + k0 * m + k1, m, T + k1 * m + k1, m, n1 - k1, m - k1, k1 - k0, false);
+  } else if (MODE == 1) {
+    const int64_t nbk = (n1 + kNB - 1) / kNB, b = nbk - 1 - step;
+    if (b <= 0 || n2 == 0) return;  // block 0 has no rows above it
+    const int64_t r0 = b * kNB, r1 = (r0 + kNB < n1) ? r0 + kNB : n1;
+    mfma_tiles<false>(T + r0, m, T + r0 * m + n1, m, T + n1, m, r0, n2, r1 - r0, false);
+  } else {
+    double2 *Bb = pool + nd.pool;
+    if (n1 == 0 || n2 == 0) return;
+    mfma_tiles<false>(Bb, m, T + n1, m, Bb + n1, m, n2, n2, n1, true);
+  }
+}
+
+// U row block on the matrix cores, in place:
+//   T[k:k+w, k+w:m] = Linv_b T[k:k+w, k+w:m],  b = k / kNB
+__global__ __launch_bounds__(256) void k_fa_urow(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
+                                                 const double2 *inv) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t n1 = nd.n1, m = nd.m;
+  if (k >= n1) return;
+  const int64_t b = k / kNB, w = (k + kNB < n1) ? kNB : n1 - k;
+  if (k + w >= m) return;
+  double2 *T = fac + nd.fac;
+  mfma_tiles<true>(inv + nd.inv + kNB * kNB * b, w, T + k * m + k + w, m, T + k * m + k + w, m, w, m - k - w, w,
+                   false);
+}
+
 
 // row permutation of each front from its interchange sequence
 __global__ void k_fa_perm(const DNode *nodes, const int32_t *lev, int nl, const int32_t *piv, int32_t *perm) {
@@ -994,41 +899,28 @@ __global__ __launch_bounds__(256) void k_sv_perm(const DNode *nodes, const int32
   }
 }
 
-// forward substitution (unit lower L) inside block rows [b*NB, b*NB+NB)
+// z_b = Linv_b z_b for panel block b of each front: a (w x w) by (w x NC)
+// product, one workgroup per front
 template <int NC, int NB>
-__global__ __launch_bounds__(64) void k_sv_ldiag(const DNode *nodes, const int32_t *lev, int64_t b, const double2 *fac,
-                                                 double2 *zb) {
+__global__ __launch_bounds__(256) void k_sv_ldiag(const DNode *nodes, const int32_t *lev, int64_t b,
+                                                  const double2 *inv, double2 *zb) {
   const DNode nd = nodes[lev[blockIdx.x]];
-  const int64_t b0 = b * NB;
-  if (b0 >= nd.n1) return;
-  const int w = (int)((b0 + NB < nd.n1 ? b0 + NB : nd.n1) - b0);
-  const double2 *T = fac + nd.fac;
-  __shared__ double2 L[NB][NB + 1];
+  const int64_t r0 = b * NB;
+  if (r0 >= nd.n1) return;
+  const int w = (int)((r0 + NB < nd.n1) ? NB : nd.n1 - r0);
+  const double2 *Ai = inv + nd.inv + (int64_t)NB * NB * b;
   __shared__ double2 z[NB][NC];
-  for (int e = threadIdx.x; e < NB * NB; e += 64) {
-    const int r = e / NB, c = e % NB;
-    L[r][c] = (r < w && c < r) ? T[(b0 + r) * nd.m + b0 + c] : make_double2(0.0, 0.0);
-  }
-  for (int e = threadIdx.x; e < NB * NC; e += 64) {
-    const int r = e / NC, c = e % NC;
-    z[r][c] = r < w ? zb[(nd.pos0 + b0 + r) * NC + c] : make_double2(0.0, 0.0);
-  }
+  for (int e = threadIdx.x; e < w * NC; e += 256) z[e / NC][e % NC] = zb[(nd.pos0 + r0) * NC + e];
   __syncthreads();
-  // column-oriented: after z[q] is final, every lower row subtracts its share
-  for (int q = 0; q + 1 < w; q++) {
-    for (int e = threadIdx.x; e < NB * NC; e += 64) {
-      const int r = e % NB, c = e / NB;
-      if (r > q && r < w) cfms(z[r][c], L[r][q], z[q][c]);
-    }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < w * NC; e += 64) {
+  for (int e = threadIdx.x; e < w * NC; e += 256) {
     const int r = e / NC, c = e % NC;
-    zb[(nd.pos0 + b0 + r) * NC + c] = z[r][c];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int q = 0; q <= r; q++) cfma(acc, Ai[r * w + q], z[q][c]);
+    zb[(nd.pos0 + r0) * NC + e] = acc;
   }
 }
 
-// rows below the block: z[r] -= L[r][b0:b1] z[b0:b1]
+// rows below block b: z[r] -= L[r][b0:b1] z[b0:b1]
 template <int NC, int NB>
 __global__ __launch_bounds__(256) void k_sv_lupd(const DNode *nodes, const int32_t *lev, int64_t b, const double2 *fac,
                                                  double2 *zb) {
@@ -1060,14 +952,14 @@ __global__ __launch_bounds__(256) void k_sv_lupd(const DNode *nodes, const int32
   }
 }
 
-// back substitution (upper U) inside block rows [r0, r1), r1 = n1 - step*NB
+// back substitution (upper U) inside panel block b = nb-1-step
 template <int NC, int NB>
 __global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32_t *lev, int64_t step,
                                                  const double2 *fac, double2 *zb) {
   const DNode nd = nodes[lev[blockIdx.x]];
-  const int64_t r1 = nd.n1 - step * NB;
-  if (r1 <= 0) return;
-  const int64_t r0 = r1 > NB ? r1 - NB : 0;
+  const int64_t b = (nd.n1 + NB - 1) / NB - 1 - step;
+  if (b < 0) return;
+  const int64_t r0 = b * NB, r1 = r0 + NB < nd.n1 ? r0 + NB : nd.n1;
   const int w = (int)(r1 - r0);
   const double2 *T = fac + nd.fac;
   __shared__ double2 U[NB][NB + 1];
@@ -1099,14 +991,14 @@ __global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32
   }
 }
 
-// rows above the block: z[r] -= U[r][r0:r1] z[r0:r1]
+// rows above block b = nb-1-step: z[r] -= U[r][r0:r1] z[r0:r1]
 template <int NC, int NB>
 __global__ __launch_bounds__(256) void k_sv_uupd(const DNode *nodes, const int32_t *lev, int64_t step,
                                                  const double2 *fac, double2 *zb) {
   const DNode nd = nodes[lev[blockIdx.y]];
-  const int64_t r1 = nd.n1 - step * NB;
-  if (r1 <= 0) return;
-  const int64_t r0 = r1 > NB ? r1 - NB : 0;
+  const int64_t b = (nd.n1 + NB - 1) / NB - 1 - step;
+  if (b <= 0) return;
+  const int64_t r0 = b * NB, r1 = r0 + NB < nd.n1 ? r0 + NB : nd.n1;
   const int w = (int)(r1 - r0);
   if ((int64_t)blockIdx.x * 256 >= r0) return;
   const double2 *T = fac + nd.fac;
@@ -1180,7 +1072,8 @@ void direct_free(mvf_handle *h) {
   DirectPlan *P = (DirectPlan *)h->direct;
   if (!P) return;
   dfree(P->d_nodes); dfree(P->d_lev); dfree(P->d_idx); dfree(P->d_fpos); dfree(P->d_rel); dfree(P->d_dst);
-  dfree(P->d_piv); dfree(P->d_perm); dfree(P->d_pos); dfree(P->d_fac); dfree(P->d_npert); dfree(P->d_sv);
+  dfree(P->d_piv); dfree(P->d_perm); dfree(P->d_pos); dfree(P->d_fac); dfree(P->d_inv); dfree(P->d_npert);
+  dfree(P->d_sv);
   dfree(P->d_zb);
   delete P;
   h->direct = nullptr;
@@ -1249,6 +1142,7 @@ static int direct_plan(mvf_handle *h) {
   MVF_TRY(dalloc(h, &P->d_perm, nne));
   MVF_TRY(dalloc(h, &P->d_npert, 1));
   MVF_TRY(dalloc(h, &P->d_fac, P->fac_entries));
+  MVF_TRY(dalloc(h, &P->d_inv, P->inv_entries));
   MVF_HIP(h, hipMemcpy(P->d_nodes, P->nodes.data(), sizeof(DNode) * P->T, hipMemcpyHostToDevice));
   MVF_HIP(h, hipMemcpy(P->d_lev, lev.data(), lev.size() * 4, hipMemcpyHostToDevice));
   MVF_HIP(h, hipMemcpy(P->d_idx, A.idx.data(), A.idx.size() * 4, hipMemcpyHostToDevice));
@@ -1261,24 +1155,10 @@ static int direct_plan(mvf_handle *h) {
 }
 
 // numeric factorization of A_pc at angular frequency w (fl32(omega) as in the operator)
-// GEMM variant: matrix cores by default; MVF_DIRECT_GEMM=valu selects the
-// vector-ALU kernel (same tiles, same results up to rounding)
-static bool gemm_on_valu() {
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("MVF_DIRECT_GEMM");
-    v = (e && strcmp(e, "valu") == 0) ? 1 : 0;
-  }
-  return v == 1;
-}
-
 template <int MODE>
 static void launch_gemm(unsigned gx, unsigned gy, hipStream_t st, const DNode *nodes, const int32_t *lev,
                         int64_t step, double2 *fac, double2 *pool) {
-  if (gemm_on_valu())
-    hipLaunchKernelGGL((k_fa_gemm<MODE>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
-  else
-    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
+  hipLaunchKernelGGL((k_fa_gemm_mfma<MODE>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
 }
 
 static int direct_factor(mvf_handle *h, double w, int c64) {
@@ -1338,8 +1218,8 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           gc = std::max(gc, n.m);
           gfl += 8.0 * (double)(n.n1 - k1) * (double)(n.m - k1) * (double)(k1 - k);
         }
-        hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_piv,
-                           P->d_npert, big, tiny);
+        hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
+                           P->d_piv, P->d_npert, big, tiny);
         int64_t gr = 0;
         for (int t : P->levels[d]) {
           const DNode &n = P->nodes[t];
@@ -1351,8 +1231,8 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                              P->d_nodes, lev, k, P->d_fac);
         hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
                            k, P->d_fac, P->d_piv);
-        hipLaunchKernelGGL((k_fa_trsm_u<kNB>), dim3(cap_grid((gc - k + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
-                           P->d_nodes, lev, k, P->d_fac);
+        hipLaunchKernelGGL(k_fa_urow, dim3(cap_grid((gc - k + kTile - 1) / kTile), nl), dim3(256), 0, st, P->d_nodes,
+                           lev, k, P->d_fac, P->d_inv);
         if (gt > 0) {
           gemm_begin(gfl);
           launch_gemm<0>(cap_grid(gt), nl, st, P->d_nodes, lev, k, P->d_fac, pool);
@@ -1366,9 +1246,9 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           double gfl = 0.0;
           for (int t : P->levels[d]) {
             const DNode &n = P->nodes[t];
-            const int64_t r1 = n.n1 - s * kNB;
-            if (r1 <= 0 || n.n2 == 0) continue;
-            const int64_t r0 = std::max<int64_t>(0, r1 - kNB);
+            const int64_t b = (n.n1 + kNB - 1) / kNB - 1 - s;
+            if (b < 0 || n.n2 == 0) continue;
+            const int64_t r0 = b * kNB, r1 = std::min(r0 + kNB, n.n1);
             gt = std::max(gt, ((r0 + kTile - 1) / kTile) * ((n.n2 + kTile - 1) / kTile));
             gfl += 8.0 * (double)r0 * (double)n.n2 * (double)(r1 - r0);
           }
@@ -1459,16 +1339,16 @@ static int direct_apply(mvf_handle *h, const double2 *in, double2 *out) {
                        P->d_perm, W, zb);
     const int64_t nb = (mn1 + kNB - 1) / kNB;
     for (int64_t b = 0; b < nb; b++) {
-      hipLaunchKernelGGL((k_sv_ldiag<NC, kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, b, P->d_fac, zb);
+      hipLaunchKernelGGL((k_sv_ldiag<NC, kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, b, P->d_inv, zb);
       if ((b + 1) * kNB < mn1)
         hipLaunchKernelGGL((k_sv_lupd<NC, kNB>), dim3(cap_grid((mn1 - (b + 1) * kNB + 255) / 256), nl), dim3(256), 0,
                            st, P->d_nodes, lev, b, P->d_fac, zb);
     }
     for (int64_t s = 0; s < nb; s++) {
       hipLaunchKernelGGL((k_sv_udiag<NC, kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, s, P->d_fac, zb);
-      if ((s + 1) * kNB < mn1)
-        hipLaunchKernelGGL((k_sv_uupd<NC, kNB>), dim3(cap_grid((mn1 - (s + 1) * kNB + 255) / 256), nl), dim3(256), 0,
-                           st, P->d_nodes, lev, s, P->d_fac, zb);
+      if (s + 1 < nb)
+        hipLaunchKernelGGL((k_sv_uupd<NC, kNB>), dim3(cap_grid((mn1 + 255) / 256), nl), dim3(256), 0, st, P->d_nodes,
+                           lev, s, P->d_fac, zb);
     }
   }
   for (int d = 0; d <= P->maxdepth; d++) {
