@@ -54,6 +54,11 @@ class MTModel:
     nextd: int = 2
     nzl_top: int = 2
     ztop: float = 0.0
+    # boundary 1-D model for Dirichlet inimod 2 (half-space h_sigma) and
+    # inimod 3 (layers l_sigma with thicknesses l_dz in km, top layer first)
+    h_sigma: float = 0.01
+    l_sigma: tuple = (0.01,)
+    l_dz: tuple = ()
     name: str = "model"
 
     @property

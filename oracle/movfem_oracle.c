@@ -93,6 +93,11 @@ typedef struct orc_state {
   int dirichlet, inimod, gpml_sch;
   double a0, b0, nn;
   int nextd, nzl_top;
+  /* boundary 1-D model of bd_setmodel (boundary_conds.f90:396-430) */
+  double ztop, h_sigma;
+  int nl;
+  double *lsig, *ldz, *zl; /* (nl), (nl-1), (nl) */
+  zc *cz, *ez;             /* Wait recursion state (nl) */
   double omega;
   int el_xa[2], el_xb[2], el_ya[2], el_yb[2], el_za[2], el_zb[2];
   double xa[2], xb[2], ya[2], yb[2], za[2], zb[2];
@@ -298,6 +303,8 @@ static inline zc sig_z(const orc_state *s, int64_t node, int c) {
   return CMPLX(s->sig[2 * (node * 6 + c)], s->sig[2 * (node * 6 + c) + 1]);
 }
 
+int orc_set_bdmodel(orc_state *s, double ztop, double h_sigma, int nl, const double *l_sigma, const double *l_dz);
+
 orc_state *orc_create(int nx, int ny, int nz, int mn, const double *xp, const double *yp, const double *zp,
                       const double *sigma, const double *mu, int dirichlet, int inimod, int gpml_sch,
                       double a0, double b0, double nn, int nextd, int nzl_top) {
@@ -356,6 +363,8 @@ orc_state *orc_create(int nx, int ny, int nz, int mn, const double *xp, const do
   s->yb[0] = s->yp[0]; s->yb[1] = s->yp[s->nny - 1];
   s->zb[0] = s->zp[0]; s->zb[1] = s->zp[s->npt - 1];
   s->in_pml[0] = s->in_pml[1] = s->in_pml[2] = 0;
+  const double one_layer = 0.01;
+  orc_set_bdmodel(s, 0.0, 0.01, 1, &one_layer, NULL);
   return s;
 }
 
@@ -363,11 +372,34 @@ void orc_free(orc_state *s) {
   if (!s) return;
   free(s->xp); free(s->yp); free(s->zp); free(s->sig); free(s->mu);
   free(s->gne); free(s->nzindx);
+  free(s->lsig); free(s->ldz); free(s->zl); free(s->cz); free(s->ez);
   free(s);
 }
 
 /* update_omega (geometry.f90:137-140) */
 void orc_set_omega(orc_state *s, double omega) { s->omega = omega; }
+
+/* bd_setmodel (boundary_conds.f90:396-430) with g_ztop (geometry.f90:389-394):
+ * half-space conductivity h_sigma (inimod 2) or nl layers l_sigma with nl-1
+ * thicknesses l_dz in km, interfaces at zl(1) = 0, zl(i) = zl(i-1) - dl(i-1)
+ * (inimod 3).  Returns 0, or -1 for nl < 1. */
+int orc_set_bdmodel(orc_state *s, double ztop, double h_sigma, int nl, const double *l_sigma, const double *l_dz) {
+  if (nl < 1) return -1;
+  free(s->lsig); free(s->ldz); free(s->zl); free(s->cz); free(s->ez);
+  s->ztop = ztop;
+  s->h_sigma = h_sigma;
+  s->nl = nl;
+  s->lsig = (double *)malloc(sizeof(double) * nl);
+  s->ldz = (double *)malloc(sizeof(double) * nl);
+  s->zl = (double *)malloc(sizeof(double) * nl);
+  s->cz = (zc *)malloc(sizeof(zc) * nl);
+  s->ez = (zc *)malloc(sizeof(zc) * nl);
+  memcpy(s->lsig, l_sigma, sizeof(double) * nl);
+  if (nl > 1) memcpy(s->ldz, l_dz, sizeof(double) * (nl - 1));
+  s->zl[0] = 0.0;
+  for (int i = 1; i < nl; i++) s->zl[i] = s->zl[i - 1] - s->ldz[i - 1];
+  return 0;
+}
 
 /* update_sigma (geometry.f90:144-153): the loop runs i<=npt, j<=6 over a
  * (6,npt) array, so it touches linear entries 0..npt+29 only.  Each touched
@@ -859,6 +891,177 @@ void orc_set_in_pml(orc_state *s, int a, int b, int c) {
   s->in_pml[0] = a; s->in_pml[1] = b; s->in_pml[2] = c;
 }
 
+/* ----------------------- Dirichlet boundary values ----------------------- */
+/* Complex arithmetic in the form the reference's Fortran evaluates it: a real
+ * operand is promoted to (r, 0) and multiplied in full; the quotient is the
+ * scaled formula of C99 Annex G.5.1 (divisor scaled by 2^-logb(max(|c|,|d|)),
+ * an exact rescaling), which is what the reference build's compiler emits. */
+static inline zc zr(double r) { return CMPLX(r, 0.0); }
+static inline zc zmul(zc a, zc b) {
+  return CMPLX(creal(a) * creal(b) - cimag(a) * cimag(b), creal(a) * cimag(b) + cimag(a) * creal(b));
+}
+static inline zc zadd(zc a, zc b) { return CMPLX(creal(a) + creal(b), cimag(a) + cimag(b)); }
+static inline zc zsub(zc a, zc b) { return CMPLX(creal(a) - creal(b), cimag(a) - cimag(b)); }
+static inline zc zneg(zc a) { return CMPLX(-creal(a), -cimag(a)); }
+static zc zdiv(zc x, zc y) {
+  double a = creal(x), b = cimag(x), c = creal(y), d = cimag(y);
+  int k = 0;
+  double lg = logb(fmax(fabs(c), fabs(d)));
+  if (isfinite(lg)) {
+    k = (int)lg;
+    c = scalbn(c, -k);
+    d = scalbn(d, -k);
+  }
+  double den = c * c + d * d;
+  return CMPLX(scalbn((a * c + b * d) / den, -k), scalbn((b * c - a * d) / den, -k));
+}
+
+/* gamma = sqrt(i alpha), alpha = omega mu0 psigma with psigma = float32(sigma)
+ * + i float32(eps omega): bd_setmodel's default-kind CMPLX keeps the real part,
+ * bd_updatemodel replaces the imaginary one (boundary_conds.f90:47-49,411-428,
+ * 452-454, 532-533) */
+static zc bd_gamma(const orc_state *s, double sigma) {
+  zc psig = CMPLX((double)(float)sigma, (double)(float)(ORC_EPS * s->omega));
+  zc alpha = zmul(zr(s->omega * (4 * ORC_PI * 1.e-7)), psig);
+  return csqrt(zmul(CMPLX(0.0, 1.0), alpha));
+}
+
+/* wait_recursion (boundary_conds.f90:521-553): surface transfer functions cz
+ * from the bottom layer up, then the interface amplitudes ez downwards */
+static void orc_wait(orc_state *s) {
+  int nl = s->nl;
+  zc g = bd_gamma(s, s->lsig[nl - 1]);
+  s->cz[nl - 1] = zdiv(zr(1.0), g);
+  for (int l = nl - 2; l >= 0; l--) {
+    g = bd_gamma(s, s->lsig[l]);
+    zc t = zmul(g, s->cz[l + 1]);
+    zc r = zdiv(zsub(zr(1.0), t), zadd(zr(1.0), t));
+    zc e = cexp(zneg(zmul(zmul(zr(2.0), g), zr(s->ldz[l]))));
+    zc re = zmul(r, e);
+    s->cz[l] = zdiv(zsub(zr(1.0), re), zmul(g, zadd(zr(1.0), re)));
+  }
+  s->ez[0] = zr(1.0);
+  for (int l = 0; l < nl - 1; l++) {
+    g = bd_gamma(s, s->lsig[l]);
+    zc e = cexp(zneg(zmul(g, zr(s->ldz[l]))));
+    zc p = zmul(zmul(s->ez[l], s->cz[l + 1]), zadd(zr(1.0), zmul(s->cz[l], g)));
+    zc q = zmul(s->cz[l], zadd(zr(1.0), zmul(s->cz[l + 1], g)));
+    s->ez[l + 1] = zdiv(zmul(e, p), q);
+  }
+}
+
+/* ezl(0, z) (boundary_conds.f90:558-578): the layer test compares the depth
+ * z >= 0 with the interface heights zl <= 0, so below the surface only z = 0
+ * (layer 1, or the half-space when nl = 1) yields a non-zero value */
+static zc orc_ezl0(const orc_state *s, double z) {
+  int nl = s->nl;
+  zc v = 0.0;
+  for (int l = 0; l < nl - 1; l++) {
+    if (z <= s->zl[l] && z > s->zl[l + 1]) {
+      zc g = bd_gamma(s, s->lsig[l]);
+      zc t = zmul(g, s->cz[l + 1]);
+      zc r = zdiv(zsub(zr(1.0), t), zadd(zr(1.0), t));
+      zc a1 = zadd(zr(1.0), zdiv(zr(1.0), zmul(s->cz[l], g)));
+      zc a2 = zsub(zr(1.0), zmul(r, cexp(zneg(zmul(zmul(zr(2.0), g), zr(z - s->zl[l + 1]))))));
+      zc a3 = cexp(zneg(zmul(g, zr(s->zl[l] - z))));
+      v = zmul(zmul(zmul(zdiv(s->ez[l], zr(2.0)), a1), a2), a3);
+    }
+  }
+  if (z <= s->zl[nl - 1]) {
+    zc g = bd_gamma(s, s->lsig[nl - 1]);
+    v = zmul(s->ez[nl - 1], cexp(zneg(zmul(g, zr(s->zl[nl - 1] - z)))));
+  }
+  return v;
+}
+
+/* p_pfields of the boundary module (boundary_conds.f90:436-516), E part for
+ * pe_sch = 1: ep[0] = Ex of polarization 1, ep[1] = Ey of polarization 2 at a
+ * node of height zn (every other component is zero).  Requires orc_wait for
+ * inimod 3. */
+static void orc_bd_pfields(const orc_state *s, double zn, zc ep[2]) {
+  double z = (s->ztop - zn) / 1000.0;
+  int air = zn > s->ztop;
+  double cm = (double)(float)(-2.0 * s->omega * ORC_B0), cp = (double)(float)(2.0 * s->omega * ORC_B0);
+  ep[0] = ep[1] = 0.0;
+  if (s->inimod == 2) {
+    zc sq = bd_gamma(s, air ? 0.0 : s->h_sigma);
+    zc q = zdiv(zr(1.0), sq);
+    zc fp = air ? zmul(q, zsub(zr(1.0), zmul(zr(z), sq))) : zmul(q, cexp(zneg(zmul(zr(z), sq))));
+    ep[0] = zmul(CMPLX(0.0, cm), fp);
+    ep[1] = zmul(CMPLX(0.0, cp), fp);
+  } else if (s->inimod == 3) {
+    if (air) {
+      zc t = zsub(s->cz[0], zr(z));
+      ep[0] = zmul(CMPLX(0.0, cp), t);
+      ep[1] = zmul(CMPLX(0.0, cm), t);
+    } else {
+      zc e = orc_ezl0(s, z);
+      ep[0] = zmul(zmul(CMPLX(0.0, cp), s->cz[0]), e);
+      ep[1] = zmul(zmul(CMPLX(0.0, cm), s->cz[0]), e);
+    }
+  }
+}
+
+/* f_boundary (boundary_conds.f90:188-240) for every Dirichlet edge of the
+ * element whose nodes w holds: faces 1,2,4,5 (x and y) take the tangential
+ * component E_p . dr/dxi_d at the edge's basis node, faces 3,6 and inimod 1
+ * are zero.  fb: [me][2]; entries of non-Dirichlet edges are left untouched. */
+static void orc_f_boundary(const orc_state *s, const orc_ws *w, const int32_t *g, zc (*fb)[2]) {
+  for (int jm = 0; jm < s->me; jm++) {
+    if (g[jm] >= 0) continue;
+    int code = -g[jm];
+    fb[jm][0] = fb[jm][1] = 0.0;
+    if (!s->dirichlet || (s->inimod != 2 && s->inimod != 3)) continue;
+    if (code == 3 || code == 6) continue;
+    int i = s->ed[jm][0], d = s->ed[jm][1] - 1;
+    double x[3] = {NR27[i - 1][0], NR27[i - 1][1], NR27[i - 1][2]};
+    double J[3][3], Ji[3][3], det;
+    orc_jac(s, w->re, x, J, Ji, &det); /* only J is used (nf_jacobian, nf_dr_dxi) */
+    zc ep[2];
+    orc_bd_pfields(s, w->re[i - 1][2], ep);
+    zc e1[3] = {ep[0], 0.0, 0.0}, e2[3] = {0.0, ep[1], 0.0};
+    zc f1 = 0.0, f2 = 0.0;
+    for (int c = 0; c < 3; c++) {
+      f1 = c ? zadd(f1, zmul(e1[c], zr(J[d][c]))) : zmul(e1[c], zr(J[d][c]));
+      f2 = c ? zadd(f2, zmul(e2[c], zr(J[d][c]))) : zmul(e2[c], zr(J[d][c]));
+    }
+    fb[jm][0] = f1;
+    fb[jm][1] = f2;
+  }
+}
+
+/* Boundary primary field at a node of height zn for the current omega:
+ * out complex (2) = {Ex of polarization 1, Ey of polarization 2}. */
+void orc_bd_field(orc_state *s, double zn, double *out) {
+  if (s->inimod == 3) orc_wait(s);
+  zc ep[2];
+  orc_bd_pfields(s, zn, ep);
+  for (int d = 0; d < 2; d++) {
+    out[2 * d] = creal(ep[d]);
+    out[2 * d + 1] = cimag(ep[d]);
+  }
+}
+
+/* Dirichlet values of every boundary edge of element (ie,je,ke), 1-based,
+ * after set_frequency: out complex (me, 2), zero for interior edges. */
+int orc_dirichlet_values(orc_state *s, int ie, int je, int ke, double *out) {
+  if (!s->gne) orc_build_gne(s);
+  if (!g_ws) g_ws = (orc_ws *)malloc(sizeof(orc_ws));
+  int rc = orc_elem_params(s, g_ws, ie, je, ke);
+  if (rc) return rc;
+  if (s->dirichlet && s->inimod == 3) orc_wait(s);
+  int64_t e = ((int64_t)(ie - 1) * s->ny + (je - 1)) * s->nz + (ke - 1);
+  zc fb[54][2];
+  for (int jm = 0; jm < s->me; jm++) fb[jm][0] = fb[jm][1] = 0.0;
+  orc_f_boundary(s, g_ws, s->gne + e * s->me, fb);
+  for (int jm = 0; jm < s->me; jm++)
+    for (int d = 0; d < 2; d++) {
+      out[2 * (jm * 2 + d)] = creal(fb[jm][d]);
+      out[2 * (jm * 2 + d) + 1] = cimag(fb[jm][d]);
+    }
+  return 0;
+}
+
 /* ------------------------------ assembly --------------------------------- */
 /* stable argsort of key[perm[*]] (merge_sort/ga_merge, global_assembly.f90:1908-1962) */
 static void stable_argsort(const int32_t *key, int64_t *perm, int64_t n) {
@@ -882,7 +1085,8 @@ static void stable_argsort(const int32_t *key, int64_t *perm, int64_t n) {
 int orc_global_vfem(orc_state *s, int32_t *ia, int32_t *ja, double *a, double *b) {
   if (!s->gne) orc_build_gne(s);
   if (!s->nzindx) orc_build_nzindx(s);
-  if (s->dirichlet && s->inimod != 1) return -4; /* f_boundary for inimod 2/3 not restated */
+  if (s->dirichlet && (s->inimod < 1 || s->inimod > 3)) return -4;
+  if (s->dirichlet && s->inimod == 3) orc_wait(s);
   int me = s->me;
   int64_t nnze = s->nnze;
   int32_t nne = s->nne;
@@ -932,10 +1136,18 @@ int orc_global_vfem(orc_state *s, int32_t *ia, int32_t *ja, double *a, double *b
             A[p - 1] += CMPLX(aloc[2 * (im * me + jm)], aloc[2 * (im * me + jm) + 1]);
           }
         }
+        /* Dirichlet lift: b(im) = blocal(im) - sum_jm f_boundary(jm) alocal(im,jm)
+         * over the element's Dirichlet edges jm (MoVFEM_3DMT.f90:251-261) */
+        zc fb[54][2];
+        orc_f_boundary(s, g_ws, g, fb);
         for (int im = 0; im < me; im++) {
           if (g[im] < 0) continue;
-          /* Dirichlet lift: f_boundary is zero for inimod 1 and for GPML */
           zc bda[2] = {0.0, 0.0};
+          for (int jm = 0; jm < me; jm++) {
+            if (g[jm] >= 0) continue;
+            zc al = CMPLX(aloc[2 * (im * me + jm)], aloc[2 * (im * me + jm) + 1]);
+            for (int d = 0; d < 2; d++) bda[d] = zadd(bda[d], zmul(fb[jm][d], al));
+          }
           for (int d = 0; d < 2; d++) {
             zc bi = CMPLX(bloc[2 * (im * 2 + d)], bloc[2 * (im * 2 + d) + 1]) - bda[d];
             B[(int64_t)d * nne + g[im] - 1] += bi;
@@ -988,7 +1200,8 @@ int orc_global_vfem(orc_state *s, int32_t *ia, int32_t *ja, double *a, double *b
  * polarization-major.  esol/hsol: complex (2*npt, 3), polarization-major. */
 int orc_node_solution(orc_state *s, const double *x, double *esol, double *hsol) {
   if (!s->gne) orc_build_gne(s);
-  if (s->dirichlet && s->inimod != 1) return -4;
+  if (s->dirichlet && (s->inimod < 1 || s->inimod > 3)) return -4;
+  if (s->dirichlet && s->inimod == 3) orc_wait(s);
   int o = s->nord - 1, mn = s->mn, me = s->me, nord = s->nord;
   int64_t npt = s->npt;
   char *ve = (char *)calloc((size_t)2 * npt, 1), *vh = (char *)calloc((size_t)2 * npt, 1);
@@ -1010,6 +1223,8 @@ int orc_node_solution(orc_state *s, const double *x, double *esol, double *hsol)
           return rc;
         }
         const int32_t *g = s->gne + ide * me;
+        zc fb[54][2];
+        orc_f_boundary(s, w, g, fb);
         for (int d = 0; d < 2; d++) {
           for (int i1 = 0; i1 < nord; i1++)
             for (int j1 = 0; j1 < nord; j1++)
@@ -1022,7 +1237,7 @@ int orc_node_solution(orc_state *s, const double *x, double *esol, double *hsol)
                 orc_jac(s, w->re, r, J, Ji, &det);
                 zc ef[3] = {0, 0, 0}, ep[3] = {0, 0, 0};
                 for (int im = 0; im < me; im++) {
-                  zc f = 0.0; /* f_boundary is zero for inimod 1 and GPML */
+                  zc f = fb[im][d]; /* Dirichlet edge: f_boundary (solution.f90:327-329) */
                   if (g[im] >= 0) f = CMPLX(x[2 * ((int64_t)g[im] - 1 + d * (int64_t)s->nne)],
                                             x[2 * ((int64_t)g[im] - 1 + d * (int64_t)s->nne) + 1]);
                   int node = s->ed[im][0], dir = s->ed[im][1] - 1;

@@ -13,7 +13,11 @@ Pinning status (what each restated function is checked against):
                                             primary field on a box
   global_vfem (assembly + sort, c64)     -- reference build, golden fixtures
   update_sigma linear-index semantics    -- reference build
-  Dirichlet lift for inimod 2/3           -- not restated (the oracle refuses inimod 2/3)
+  Dirichlet values (f_boundary, p_pfields
+    of the boundary module, Wait recursion,
+    ezl) and their lift, inimod 2/3         -- reference build (per-element values, global_vfem RHS,
+                                            node_solution), closed forms of the half-space and
+                                            one-layer cases, tangential-trace identity on a box
 """
 from __future__ import annotations
 
@@ -59,6 +63,12 @@ def lib():
         L.orc_node_solution.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_receivers.restype = C.c_int
         L.orc_receivers.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_set_bdmodel.restype = C.c_int
+        L.orc_set_bdmodel.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, dp]
+        L.orc_dirichlet_values.restype = C.c_int
+        L.orc_dirichlet_values.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]
+        L.orc_bd_field.restype = None
+        L.orc_bd_field.argtypes = [C.c_void_p, C.c_double, dp]
         L.orc_impedance.restype = None
         L.orc_impedance.argtypes = [dp, dp, C.c_double, dp, dp, dp]
         _LIB = L
@@ -90,6 +100,10 @@ class Oracle:
                               int(m.dirichlet), m.inimod, m.gpml_sch, m.a0, m.b0, m.nn, m.nextd, m.nzl_top)
         if not self.h:
             raise ValueError("oracle rejected the model")
+        ls = np.ascontiguousarray(m.l_sigma, np.float64)
+        ld = np.ascontiguousarray(np.append(np.asarray(m.l_dz, np.float64), 0.0))
+        if L.orc_set_bdmodel(self.h, float(m.ztop), float(m.h_sigma), len(ls), _dp(ls), _dp(ld)) != 0:
+            raise ValueError("oracle rejected the boundary model")
         L.orc_set_omega(self.h, m.omega(0))
         self.me = m.edges_per_elem
         self._nne = None
@@ -106,6 +120,22 @@ class Oracle:
         L = lib()
         L.orc_set_omega(self.h, self.m.omega(i))
         L.orc_update_sigma(self.h)
+
+    def dirichlet_values(self, ie, je, ke) -> np.ndarray:
+        """f_boundary of every edge of element (ie,je,ke) (1-based), complex (me, 2)
+        by polarization; zero for interior edges."""
+        out = np.zeros(self.me * 4)
+        rc = lib().orc_dirichlet_values(self.h, ie, je, ke, _dp(out))
+        if rc:
+            raise RuntimeError(f"orc_dirichlet_values failed ({rc})")
+        return out.view(np.complex128).reshape(self.me, 2)
+
+    def bd_field(self, zn: float) -> np.ndarray:
+        """Boundary primary field at node height zn: complex (2,) = (Ex of
+        polarization 1, Ey of polarization 2) at the current frequency."""
+        out = np.zeros(4)
+        lib().orc_bd_field(self.h, float(zn), _dp(out))
+        return out.view(np.complex128)
 
     def sigma(self) -> np.ndarray:
         out = np.empty(self.m.npt * 12)

@@ -29,14 +29,16 @@ module mvf_ref_harness
   integer, save :: h_me = 0
 contains
 
+  ! h_sigma, nl, l_sigma(nl), l_dz(nl-1): the boundary 1-D model handed to
+  ! bd_setmodel (MoVFEM_3DMT.f90:388)
   subroutine ref_setup(nx, ny, nz, mn, xp, yp, zp, sig, mu, dirich, inimod, sch, a0_, b0_, nn_, &
-       nextd_, nzl_top, ztop, nf, freqs) bind(C, name='ref_setup')
-    integer(c_int), value :: nx, ny, nz, mn, dirich, inimod, sch, nextd_, nzl_top, nf
-    real(c_double), value :: a0_, b0_, nn_, ztop
-    real(c_double), intent(in) :: xp(*), yp(*), zp(*), mu(*), freqs(*)
+       nextd_, nzl_top, ztop, nf, freqs, h_sigma, nl, l_sigma, l_dz) bind(C, name='ref_setup')
+    integer(c_int), value :: nx, ny, nz, mn, dirich, inimod, sch, nextd_, nzl_top, nf, nl
+    real(c_double), value :: a0_, b0_, nn_, ztop, h_sigma
+    real(c_double), intent(in) :: xp(*), yp(*), zp(*), mu(*), freqs(*), l_sigma(*), l_dz(*)
     complex(c_double_complex), intent(in) :: sig(*)
     integer :: o, i, k
-    real(kind=double) :: lsig(1), ldz(1)
+    real(kind=double), allocatable :: lsig(:), ldz(:)
 
     g_nx = nx + 1; g_ny = ny + 1; g_nz = nz + 1
     if (mn == 8) then
@@ -80,8 +82,10 @@ contains
     call init_problem(1, 2, 1)
     call init_integration()
     call ga_init(.true., dirich /= 0, inimod)
-    lsig = 0.01d0; ldz = 0.d0
-    call bd_setmodel(0.01d0, 1, lsig, ldz)
+    allocate(lsig(nl), ldz(max(nl - 1, 1)))
+    lsig = l_sigma(1:nl); ldz = 0.d0
+    if (nl > 1) ldz(1:nl - 1) = l_dz(1:nl - 1)
+    call bd_setmodel(h_sigma, nl, lsig, ldz)
   end subroutine ref_setup
 
   ! frequency-loop state update (MoVFEM_3DMT.f90:66-69); write_problem_ugrid is

@@ -28,7 +28,8 @@ def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=No
     np.savez(path, nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp,
              sigma=m.sigma, mu=m.mu, freqs=m.freqs, dirichlet=int(m.dirichlet), inimod=m.inimod,
              gpml_sch=m.gpml_sch, a0=m.a0, b0=m.b0, nn=m.nn, nextd=m.nextd, nzl_top=m.nzl_top,
-             ztop=m.ztop, elements=np.asarray(elements, np.int32).reshape(-1, 3),
+             ztop=m.ztop, h_sigma=m.h_sigma, l_sigma=np.asarray(m.l_sigma, np.float64),
+             l_dz=np.asarray(m.l_dz, np.float64), elements=np.asarray(elements, np.int32).reshape(-1, 3),
              vfem_freqs=np.asarray(vfem_freqs, np.int32), sigma_after=np.asarray(sigma_after, np.int32),
              readout_x=np.zeros(0, complex) if readout_x is None else np.asarray(readout_x, np.complex128),
              rx_xy=rx, rx_nzl=rz)
@@ -83,7 +84,7 @@ def _main(fin, fout):
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)
     L.ref_setup.argtypes = ([C.c_int] * 4 + [dp] * 5 + [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] * 2
-                            + [C.c_double, C.c_int, dp])
+                            + [C.c_double, C.c_int, dp, C.c_double, C.c_int, dp, dp])
     L.ref_set_frequency.argtypes = [C.c_int]
     L.ref_sizes.argtypes = [ip, C.POINTER(C.c_int64)]
     L.ref_gne.argtypes = [ip]
@@ -101,9 +102,12 @@ def _main(fin, fout):
     sg = np.ascontiguousarray(z["sigma"].astype(np.complex128)).view(np.float64)
     mu = np.ascontiguousarray(z["mu"], np.float64)
     fr = np.ascontiguousarray(z["freqs"], np.float64)
+    lsg = np.ascontiguousarray(z["l_sigma"], np.float64)
+    ldz = np.ascontiguousarray(np.append(z["l_dz"], 0.0), np.float64)
     L.ref_setup(int(z["nx"]), int(z["ny"]), int(z["nz"]), int(z["mn"]), d(xp), d(yp), d(zp), d(sg), d(mu),
                 int(z["dirichlet"]), int(z["inimod"]), int(z["gpml_sch"]), float(z["a0"]), float(z["b0"]),
-                float(z["nn"]), int(z["nextd"]), int(z["nzl_top"]), float(z["ztop"]), len(fr), d(fr))
+                float(z["nn"]), int(z["nextd"]), int(z["nzl_top"]), float(z["ztop"]), len(fr), d(fr),
+                float(z["h_sigma"]), len(lsg), d(lsg), d(ldz))
     nne = C.c_int32()
     nnze = C.c_int64()
     L.ref_sizes(C.byref(nne), C.byref(nnze))

@@ -6,7 +6,11 @@ Run in the container that holds the reference sources:
 Each fixture stores the model inputs (lattice, sigma, mu, boundary settings)
 and the reference's outputs: gne, nzindx, alocal/blocal of every element and
 global_vfem (ia, ja, a, b) for the first two frequencies, plus g_sigma after
-update_sigma for frequency 2.
+update_sigma for frequency 2.  The Dirichlet inimod 2/3 fixtures add the
+boundary 1-D model (h_sigma, l_sigma, l_dz) and node_solution's E and H for a
+seeded solution vector (readout_x, esol, hsol).
+
+    python -m tests.golden.make_golden [name ...]   # default: every fixture
 """
 from __future__ import annotations
 
@@ -28,13 +32,35 @@ def golden_cases():
         "g20_gpml_tensor": M.random_small(20, 2, 2, 2, seed=33, dirichlet=False, nextd=1, full_tensor=True,
                                           topo=120.0, freqs=(1.0, 10.0)),
         "g27_dirichlet_aniso": M.random_small(27, 2, 2, 2, seed=34, anisotropic=True, nz_air=1, freqs=(3.0, 30.0)),
+        "g8_dirichlet_halfspace_bd": _bd(M.random_small(8, 3, 2, 3, seed=35, topo=180.0, nz_air=1,
+                                                        freqs=(0.2, 25.0)), 2, h_sigma=0.004),
+        "g20_dirichlet_layered_bd": _bd(M.random_small(20, 2, 2, 3, seed=36, nz_air=1, freqs=(1.0, 0.01)), 3,
+                                        l_sigma=(0.02, 0.0015, 0.4), l_dz=(0.35, 0.9)),
     }
 
 
-def main():
+def _bd(m, inimod, **kw):
+    m.inimod = inimod
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+def readout_vector(nne, seed=77):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.standard_normal(2 * nne) + 1j * rng.standard_normal(2 * nne)
+
+
+def main(names=()):
     for name, m in golden_cases().items():
+        if names and name not in names:
+            continue
         els = [(i, j, k) for i in range(1, m.nx + 1) for j in range(1, m.ny + 1) for k in range(1, m.nz + 1)]
-        r = run_reference(m, elements=els, vfem_freqs=[1, 2], sigma_after=[])
+        bd = m.dirichlet and m.inimod in (2, 3)
+        x = None
+        if bd:
+            x = readout_vector(int(run_reference(m)["nne"]))
+        r = run_reference(m, elements=els, vfem_freqs=[1, 2], sigma_after=[], readout_x=x)
         # sigma after update_sigma at frequency 2 (separate process: state order)
         r2 = run_reference(m, sigma_after=[2])
         out = dict(nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp, sigma=m.sigma, mu=m.mu,
@@ -42,6 +68,9 @@ def main():
                    b0=m.b0, nn=m.nn, nextd=m.nextd, nzl_top=m.nzl_top, ztop=m.ztop,
                    gne=r["gne"], nzindx=r["nzindx"], nne=r["nne"], nnze=r["nnze"], aloc=r["aloc"], bloc=r["bloc"],
                    sigma_2=r2["sigma_2"])
+        if bd:
+            out.update(h_sigma=m.h_sigma, l_sigma=np.asarray(m.l_sigma, np.float64),
+                       l_dz=np.asarray(m.l_dz, np.float64), readout_x=x, esol=r["esol"], hsol=r["hsol"])
         for fi in (1, 2):
             for k in ("ia", "ja", "a", "b"):
                 out[f"{k}_{fi}"] = r[f"{k}_{fi}"]
@@ -54,8 +83,14 @@ def load_model(z) -> M.MTModel:
                   z["mu"], z["freqs"], dirichlet=bool(z["dirichlet"]), inimod=int(z["inimod"]),
                   gpml_sch=int(z["gpml_sch"]), a0=float(z["a0"]), b0=float(z["b0"]), nn=float(z["nn"]),
                   nextd=int(z["nextd"]), nzl_top=int(z["nzl_top"]), ztop=float(z["ztop"]))
+    if "h_sigma" in z.files:
+        m.h_sigma = float(z["h_sigma"])
+        m.l_sigma = tuple(float(v) for v in z["l_sigma"])
+        m.l_dz = tuple(float(v) for v in z["l_dz"])
     return m
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+
+    main(tuple(sys.argv[1:]))

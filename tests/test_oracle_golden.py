@@ -54,3 +54,20 @@ def test_golden_update_sigma(path):
     o.set_frequency(0)
     o.set_frequency(1)
     assert np.array_equal(o.sigma(), z["sigma_2"])
+
+
+BD = [p for p in GOLD if "h_sigma" in np.load(p).files]
+
+
+@pytest.mark.parametrize("path", BD, ids=[p.stem for p in BD])
+def test_golden_dirichlet_readout(path):
+    """node_solution with the inimod 2/3 Dirichlet values on the boundary edges
+    (solution.f90:327-329) at frequency 2, where the reference's run left it."""
+    z = np.load(path)
+    m = load_model(z)
+    o = Oracle(m)
+    o.set_frequency(0)
+    o.set_frequency(1)
+    e, h = o.node_solution(z["readout_x"])
+    assert _rel(e, z["esol"]) < 1e-13
+    assert _rel(h, z["hsol"]) < 1e-13
