@@ -59,15 +59,22 @@ typedef struct mvf_model {
 } mvf_model;
 
 /* Boundary conditions (ga_init / init_gpml / bd_setmodel state,
- * global_assembly.f90:26-41, boundary_conds.f90:51-70). */
+ * global_assembly.f90:26-41, boundary_conds.f90:51-70,396-430). */
 typedef struct mvf_bc {
   int32_t dirichlet; /* 1: Dirichlet edges removed; 0: GPML                     */
-  int32_t inimod;    /* Dirichlet model: 1 zero (2/3 -> MVF_E_UNSUPPORTED)      */
+  int32_t inimod;    /* Dirichlet values: 1 zero, 2 half-space, 3 layered earth */
   int32_t gpml_sch;  /* 0 Fang, 1 Zhou (Re h = 1 under the reference semantics) */
   int32_t nextd;     /* extension cells; GPML zone = last nextd elements        */
   int32_t nzl_top;   /* g_nzl(g_nsf): top-zone layers, locates za(2)            */
   int32_t pad_;
   double a0, b0, nn; /* GPML constants                                          */
+  /* boundary 1-D model of bd_setmodel, read for dirichlet with inimod 2/3 */
+  double ztop;             /* g_ztop: surface height (geometry.f90:389-394)     */
+  double h_sigma;          /* inimod 2: half-space conductivity                 */
+  int32_t nl;              /* inimod 3: number of layers (>= 1)                 */
+  int32_t pad2_;
+  const double *l_sigma;   /* inimod 3: layer conductivities, top first (nl)    */
+  const double *l_dz;      /* inimod 3: layer thicknesses in km (nl-1)          */
 } mvf_bc;
 
 /* Emulation switches (default: all on = the reference's effective semantics). */

@@ -43,7 +43,8 @@ class Model(C.Structure):
 class BC(C.Structure):
     _fields_ = [("dirichlet", C.c_int32), ("inimod", C.c_int32), ("gpml_sch", C.c_int32), ("nextd", C.c_int32),
                 ("nzl_top", C.c_int32), ("pad_", C.c_int32), ("a0", C.c_double), ("b0", C.c_double),
-                ("nn", C.c_double)]
+                ("nn", C.c_double), ("ztop", C.c_double), ("h_sigma", C.c_double), ("nl", C.c_int32),
+                ("pad2_", C.c_int32), ("l_sigma", C.POINTER(C.c_double)), ("l_dz", C.POINTER(C.c_double))]
 
 
 class Options(C.Structure):

@@ -110,7 +110,7 @@ void mvf_free(mvf_handle *h) {
   void *ptrs[] = {h->d_xp,  h->d_yp,  h->d_zp,   h->d_sig,   h->d_inmu,     h->d_dmu,      h->d_tab,
                   h->d_gne, h->d_rowptr, h->d_col, h->d_diagpos, h->d_seg, h->d_contrib, h->d_rseg,
                   h->d_rcontrib, h->d_K, h->d_M, h->d_K32, h->d_M32, h->d_Ke, h->d_Me, h->d_dK0, h->d_dM0,
-                  h->d_x,  h->d_b, h->d_ntab};
+                  h->d_x,  h->d_b, h->d_ntab, h->d_bel, h->d_bidx, h->d_bK, h->d_bM, h->d_dirf, h->d_btab};
   for (void *p : ptrs) dfree(p);
   direct_free(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -131,8 +131,21 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
   h->emulate = opts ? opts->emulate : MVF_EMU_ALL;
   h->verbose = opts ? opts->verbose : 0;
   h->bc = *bc;
-  if (bc->dirichlet && bc->inimod != 1)
-    return set_err(h, MVF_E_UNSUPPORTED, "Dirichlet boundary model inimod=%d is not implemented (only 1)", bc->inimod);
+  h->bc.l_sigma = h->bc.l_dz = nullptr;  // the caller's arrays are copied below
+  if (bc->dirichlet) {
+    if (bc->inimod < 1 || bc->inimod > 3)
+      return set_err(h, MVF_E_ARG, "Dirichlet boundary model inimod=%d (must be 1, 2 or 3)", bc->inimod);
+    if (bc->inimod == 3) {
+      if (bc->nl < 1 || !bc->l_sigma || (bc->nl > 1 && !bc->l_dz))
+        return set_err(h, MVF_E_ARG, "inimod 3 needs nl >= 1 layer conductivities and nl-1 thicknesses");
+      h->bd_lsig.assign(bc->l_sigma, bc->l_sigma + bc->nl);
+      h->bd_ldz.assign(bc->nl - 1, 0.0);
+      for (int l = 0; l + 1 < bc->nl; l++) h->bd_ldz[l] = bc->l_dz[l];
+    }
+    h->bd_on = bc->inimod != 1;
+    h->bd_ztop = bc->ztop;
+    h->bd_hsigma = bc->h_sigma;
+  }
   MVF_HIP(h, hipSetDevice(h->device));
   // a blocking stream: synchronous runtime calls on the null stream stay ordered after its work
   MVF_HIP(h, hipStreamCreate(&h->stream));
@@ -242,6 +255,7 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   double t4 = now();
   MVF_TRY(assemble_operators(h));
+  if (h->bd_on) MVF_TRY(bd_setup(h, h->d_Ke, h->d_Me));
   // element 0 under the carried flags (see MVF_EMU_GPML_PREV_FLAGS)
   if (gp.on && (h->emulate & MVF_EMU_GPML_PREV_FLAGS) && h->flags_late != h->flags_first) {
     MVF_TRY(dalloc(h, &h->d_dK0, np));
@@ -390,6 +404,7 @@ static int build_rhs(mvf_handle *h, double omega, int64_t call, double *d_b, int
   double *be;
   MVF_TRY(dalloc(h, &be, (size_t)h->geo.ne * h->et.ne * 4));
   int rc = element_rhs_range(h, omega, first_flags_for(h, call), 0, h->geo.ne, be);
+  if (!rc) rc = bd_lift(h, omega, be);
   if (!rc) rc = assemble_rhs(h, be, d_b, ncol, col0);
   dfree(be);
   return rc;

@@ -57,6 +57,15 @@ struct Geom {
   int64_t nyz, npt, ne;
 };
 
+// per-frequency scalars of the Dirichlet boundary 1-D model (inimod 2/3)
+struct BdParams {
+  int inimod;
+  double ztop;
+  double cm, cp;           // float32(-2 omega b0), float32(2 omega b0)
+  double2 sa, qa, se, qe;  // inimod 2: gamma and 1/gamma of the air and the half-space
+  double2 cz1, e0;         // inimod 3: surface transfer function, ezl(0, 0)
+};
+
 }  // namespace mvf
 
 struct mvf_handle {
@@ -112,6 +121,18 @@ struct mvf_handle {
   double t_setup[4] = {0, 0, 0, 0};
   // sparse direct factorization state (mvf_direct.hip)
   void *direct = nullptr;
+  // Dirichlet inimod 2/3 values (mvf_boundary.hip): elements on the x and y
+  // faces, their K_e/M_e, f_boundary per (element, edge, polarization)
+  int bd_on = 0;
+  double bd_ztop = 0.0, bd_hsigma = 0.0;
+  std::vector<double> bd_lsig, bd_ldz;
+  int64_t nbel = 0;
+  int64_t *d_bel = nullptr;
+  int32_t *d_bidx = nullptr;  // element -> index in d_bel, or -1
+  double *d_bK = nullptr, *d_bM = nullptr;
+  double2 *d_dirf = nullptr;
+  double bd_omega = -1.0;     // frequency d_dirf holds
+  mvf::BasisTab *d_btab = nullptr;  // basis at the element's basis nodes
   // readout: basis at the element's lattice nodes; frequencies of the last batch
   mvf::BasisTab *d_ntab = nullptr;
   std::vector<double> last_freqs;
@@ -181,5 +202,8 @@ int receivers(mvf_handle *h, const double *es, const double *hs, int ke_top, int
               const int32_t *rje, const double *rx, const double *ry, double omega, double *oe, double *oh, double *oz,
               double *rho, double *phi);
 void direct_invalidate(mvf_handle *h);
+int bd_setup(mvf_handle *h, const double *dKe, const double *dMe);
+int bd_values(mvf_handle *h, double omega);
+int bd_lift(mvf_handle *h, double omega, double *d_be);
 
 }  // namespace mvf

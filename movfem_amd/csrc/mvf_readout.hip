@@ -48,7 +48,8 @@ __device__ inline int owner_setup(const BasisTab *tab, const Geom &g, const doub
 template <int NN, int NE>
 __global__ __launch_bounds__(256) void k_node_e(const BasisTab *ntab, Geom g, const double *xp, const double *yp,
                                                 const double *zp, const int32_t *gne, const double2 *x, int ncol,
-                                                int col0, double omega, double2 *esol) {
+                                                int col0, double omega, const int32_t *bidx, const double2 *dirf,
+                                                double2 *esol) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= 2 * g.npt) return;
   const int d = (int)(t / g.npt);
@@ -61,10 +62,13 @@ __global__ __launch_bounds__(256) void k_node_e(const BasisTab *ntab, Geom g, co
   jacobian<NN>(ntab, p, xs, Ji);
   const int64_t e = ((int64_t)(ie - 1) * g.ny + (je - 1)) * g.nz + (ke - 1);
   double er[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
+  // Dirichlet edges carry f_boundary (solution.f90:327-329): zero for
+  // inimod 1, the boundary values of the element's row of dirf otherwise
+  const int32_t be = bidx ? bidx[e] : -1;
   for (int im = 0; im < NE; im++) {
     const int32_t gi = gne[e * NE + im];
-    if (gi <= 0) continue;  // Dirichlet edge: f_boundary = 0 (inimod 1)
-    const double2 f = x[(int64_t)(gi - 1) * ncol + col0 + d];
+    if (gi == 0 || (gi < 0 && be < 0)) continue;
+    const double2 f = gi > 0 ? x[(int64_t)(gi - 1) * ncol + col0 + d] : dirf[((int64_t)be * NE + im) * 2 + d];
     const double ph = ntab->phi[p][im];
     const int dir = ntab->dir[im];
     for (int c = 0; c < 3; c++) {
@@ -190,7 +194,8 @@ static void launch_node_fields(mvf_handle *h, const double2 *x, int ncol, int co
                                double2 *hs) {
   const int64_t n = 2 * h->geo.npt;
   hipLaunchKernelGGL((k_node_e<NN, NE>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, h->d_ntab, h->geo, h->d_xp,
-                     h->d_yp, h->d_zp, h->d_gne, x, ncol, col0, omega, es);
+                     h->d_yp, h->d_zp, h->d_gne, x, ncol, col0, omega, h->bd_on ? h->d_bidx : nullptr,
+                     h->d_dirf, es);
   hipLaunchKernelGGL((k_node_h<NN>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, h->d_ntab, h->geo, h->d_xp,
                      h->d_yp, h->d_zp, h->d_inmu, omega, es, hs);
 }
@@ -221,6 +226,7 @@ int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega
     delete t;
     if (rc) return rc;
   }
+  MVF_TRY(bd_values(h, omega));
   const double2 *xv = (const double2 *)x;
   double2 *e2 = (double2 *)es, *h2 = (double2 *)hs;
   switch (h->nodes) {

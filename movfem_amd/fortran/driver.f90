@@ -75,7 +75,7 @@ program movfem_driver
   end do
   grid = mvf_grid(nx, ny, nz, 8, c_loc(xp), c_loc(yp), c_loc(zp))
   model = mvf_model(c_loc(sigma), c_loc(mu))
-  bc = mvf_bc(1, 1, 0, 2, 2, 0, 1.d0, 1.d0, 2.d0)
+  bc = mvf_bc(1, 1, 0, 2, 2, 0, 1.d0, 1.d0, 2.d0, ztop, 0.01d0, 1, 0, c_null_ptr, c_null_ptr)
   opts = mvf_options(MVF_EMU_ALL, 0, 0, 0)
   ierr = mvf_init(grid, model, bc, opts, hnd)
   if (ierr /= 0) then

@@ -26,6 +26,9 @@ module movfem_gpu
   type, bind(C) :: mvf_bc
     integer(c_int32_t) :: dirichlet, inimod, gpml_sch, nextd, nzl_top, pad_
     real(c_double) :: a0, b0, nn
+    real(c_double) :: ztop, h_sigma
+    integer(c_int32_t) :: nl, pad2_
+    type(c_ptr) :: l_sigma, l_dz
   end type mvf_bc
 
   type, bind(C) :: mvf_options

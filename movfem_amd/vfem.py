@@ -78,7 +78,10 @@ class GpuVfem:
         xp, yp, zp, sg, mu = self._keep
         self._grid = L.Grid(m.nx, m.ny, m.nz, m.nodes_per_elem, _dp(xp), _dp(yp), _dp(zp))
         self._model = L.Model(_dp(sg), _dp(mu))
-        self._bc = L.BC(int(m.dirichlet), m.inimod, m.gpml_sch, m.nextd, m.nzl_top, 0, m.a0, m.b0, m.nn)
+        self._lsig = np.ascontiguousarray(m.l_sigma, np.float64)
+        self._ldz = np.ascontiguousarray(np.append(np.asarray(m.l_dz, np.float64), 0.0))
+        self._bc = L.BC(int(m.dirichlet), m.inimod, m.gpml_sch, m.nextd, m.nzl_top, 0, m.a0, m.b0, m.nn,
+                        float(m.ztop), float(m.h_sigma), len(self._lsig), 0, _dp(self._lsig), _dp(self._ldz))
         self._opts = L.Options(emulate, device, verbose, 0)
         self.h = C.c_void_p()
         rc = self._lib.mvf_init(C.byref(self._grid), C.byref(self._model), C.byref(self._bc), C.byref(self._opts),

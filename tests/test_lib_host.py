@@ -42,10 +42,16 @@ def test_bad_arguments_return_codes():
     bc = L.BC(1, 1, 0, 2, 2, 0, 1.0, 1.0, 2.0)
     assert lib.mvf_init(C.byref(grid), C.byref(model), C.byref(bc), None, C.byref(h)) == -1
     grid.nodes_per_elem = 8
-    bc.inimod = 2  # Dirichlet half-space boundary fields are not implemented
+    bc.inimod = 4  # Dirichlet boundary models are 1, 2 and 3 (boundary_conds.f90:199-236)
     rc = lib.mvf_init(C.byref(grid), C.byref(model), C.byref(bc), None, C.byref(h))
-    assert rc == -5
+    assert rc == -1
     assert b"inimod" in lib.mvf_last_error(h)
+    lib.mvf_free(h)
+    bc.inimod = 3  # layered earth without layers
+    bc.nl = 0
+    rc = lib.mvf_init(C.byref(grid), C.byref(model), C.byref(bc), None, C.byref(h))
+    assert rc == -1
+    assert b"inimod 3" in lib.mvf_last_error(h)
     lib.mvf_free(h)
 
 
