@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--maxit", type=int, default=3000)
     ap.add_argument("--nfreq", type=int, default=0, help="solve only the first N frequencies (profiling runs)")
+    ap.add_argument("--direct-tile", type=int, default=0, help="factorization GEMM tile: 0 by front size, 64 or 128")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -166,7 +167,7 @@ def main():
 
     def step():
         ms = g.assemble("fp64")
-        info = g.solve_frequencies(freqs, method="gmres", precond="direct", tol=1e-8)
+        info = g.solve_frequencies(freqs, method="gmres", precond="direct", tol=1e-8, direct_tile=args.direct_tile)
         return ms, info
 
     for w in range(args.warmup):

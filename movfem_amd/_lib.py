@@ -55,7 +55,7 @@ class SolverOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("matrix", C.c_int32), ("precond", C.c_int32),
                 ("tol", C.c_double), ("maxit", C.c_int32), ("check_every", C.c_int32), ("inner_tol", C.c_double),
                 ("use_graph", C.c_int32), ("gmres_m", C.c_int32), ("freqs_per_factor", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("direct_tile", C.c_int32)]
 
 
 class SolveInfo(C.Structure):

@@ -74,12 +74,18 @@ struct DirectPlan {
   int64_t fac_entries = 0, sv_rows = 0, nidx = 0, nrel = 0, inv_entries = 0;
   double flops = 0.0;
   double minMK = 1.0, maxK = 0.0, maxM = 0.0;
+  int tile = 0;  // GEMM tile of the factorization: 0 by front size, 64 or 128 forced
   int64_t nne = 0;
   // device
   DNode *d_nodes = nullptr;
   int32_t *d_lev = nullptr, *d_idx = nullptr, *d_fpos = nullptr, *d_rel = nullptr, *d_dst = nullptr;
   int32_t *d_piv = nullptr, *d_perm = nullptr, *d_pos = nullptr;
   double2 *d_fac = nullptr, *d_inv = nullptr;
+  // level pools (the bottom bands of a level's fronts) live in one arena:
+  // even levels at its start, odd levels at its end, so a level and its
+  // children's level never overlap
+  double2 *d_arena = nullptr;
+  int64_t arena = 0;
   int *d_npert = nullptr;
   double2 *d_sv = nullptr, *d_zb = nullptr;
   int sv_cols = 0;
@@ -109,8 +115,11 @@ struct GemmTimer {
       (void)hipEventCreate(&ev[i][1]);
     }
   }
-  void drain() {
-    for (int i = 0; i < used; i++) {
+  // reads the oldest `count` timings (all by default), keeping the newer
+  // ones queued so that the device never runs dry while the host waits
+  void drain(int count = -1) {
+    if (count < 0 || count > used) count = used;
+    for (int i = 0; i < count; i++) {
       float t = 0.f;
       if (hipEventSynchronize(ev[i][1]) == hipSuccess && hipEventElapsedTime(&t, ev[i][0], ev[i][1]) == hipSuccess) {
         ms += t;
@@ -118,7 +127,12 @@ struct GemmTimer {
         launches++;
       }
     }
-    used = 0;
+    for (int i = count; i < used; i++) {
+      std::swap(ev[i - count][0], ev[i][0]);
+      std::swap(ev[i - count][1], ev[i][1]);
+      fl[i - count] = fl[i];
+    }
+    used -= count;
   }
   void destroy() {
     drain();
@@ -693,7 +707,7 @@ typedef double mvf_d4 __attribute__((ext_vector_type(4)));
 // tiles, tile t handled by workgroup t mod gridDim.x; LOWER skips tiles above
 // the diagonal.
 template <bool OVERWRITE>
-__device__ inline void mfma_tiles(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C,
+__device__ __forceinline__ void mfma_tiles(const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C,
                                   int64_t ldc, int64_t Mr, int64_t Nc, int64_t Kd, bool lower) {
   const int64_t tn = (Nc + kTile - 1) / kTile, tm = (Mr + kTile - 1) / kTile;
   __shared__ double2 As[kTile][17];
@@ -764,28 +778,134 @@ __device__ inline void mfma_tiles(const double2 *A, int64_t lda, const double2 *
   }
 }
 
+// C -= A B over 128x128 complex tiles (large fronts): eight waves of 64x32,
+// each 4x2 blocks of 16x16 accumulated in the matrix cores (128 accumulator
+// registers per lane, two waves per SIMD), 16-deep K chunks staged through
+// two LDS buffers with the next chunk's global loads in flight while the
+// current one is consumed.  A is staged transposed (k-major) so that both
+// operands are read as 256-byte rows per 16-lane group.  Blocks are
+// renumbered so that each XCD (blocks are dealt round-robin to the 8 XCDs)
+// walks a contiguous range of tiles and shares the A row panels in its L2.
+constexpr int kBig = 128;
+constexpr int kBigThreads = 512;
+__device__ __forceinline__ void mfma_tiles128(const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                                              double2 *C, int64_t ldc, int64_t Mr, int64_t Nc, int64_t Kd,
+                                              bool lower) {
+  const int64_t tn = (Nc + kBig - 1) / kBig, tm = (Mr + kBig - 1) / kBig;
+  __shared__ double2 As[2][16][kBig + 1];
+  __shared__ double2 Bs[2][16][kBig];
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+  const int wr = (wv >> 2) * 64, wc = (wv & 3) * 32;
+  const int li = l & 15, lk = l >> 4;
+  const unsigned G = gridDim.x;
+  const int64_t b0 = (G % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  // global -> register staging: A rows ar + 32 j at column aq; B rows bq + 4 j at column bc
+  const int aq = tid & 15, ar = tid >> 4, bc = tid & 127, bq = tid >> 7;
+  for (int64_t tile = b0; tile < tm * tn; tile += G) {
+    const int64_t ti = tile / tn, tj = tile % tn;
+    if (lower && tj > ti) continue;
+    const int64_t i0 = ti * kBig, j0 = tj * kBig;
+    mvf_d4 cre[4][2], cim[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        cre[r][c] = mvf_d4{0.0, 0.0, 0.0, 0.0};
+        cim[r][c] = mvf_d4{0.0, 0.0, 0.0, 0.0};
+      }
+    double2 ra[4], rb[4];
+    auto fetch = [&](int64_t kk) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int64_t r = i0 + ar + 32 * j, q = kk + aq;
+        ra[j] = (r < Mr && q < Kd) ? A[r * lda + q] : make_double2(0.0, 0.0);
+        const int64_t qb = kk + bq + 4 * j, c = j0 + bc;
+        rb[j] = (qb < Kd && c < Nc) ? B[qb * ldb + c] : make_double2(0.0, 0.0);
+      }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        As[buf][aq][ar + 32 * j] = ra[j];
+        Bs[buf][bq + 4 * j][bc] = rb[j];
+      }
+    };
+    const int64_t nch = (Kd + 15) / 16;
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int64_t ch = 0; ch < nch; ch++) {
+      const int buf = (int)(ch & 1);
+      if (ch + 1 < nch) fetch((ch + 1) * 16);
+#pragma unroll
+      for (int ks = 0; ks < 16; ks += 4) {
+        double2 a[4], b[2];
+#pragma unroll
+        for (int r = 0; r < 4; r++) a[r] = As[buf][ks + lk][wr + r * 16 + li];
+#pragma unroll
+        for (int c = 0; c < 2; c++) b[c] = Bs[buf][ks + lk][wc + c * 16 + li];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            cre[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, cre[r][c], 0, 0, 0);
+            cim[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].y, cim[r][c], 0, 0, 0);
+          }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            cre[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[r].y, b[c].y, cre[r][c], 0, 0, 0);
+            cim[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].x, cim[r][c], 0, 0, 0);
+          }
+      }
+      if (ch + 1 < nch) stash(buf ^ 1);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int64_t i = i0 + wr + r * 16 + lk + 4 * v, j = j0 + wc + c * 16 + li;
+          if (i < Mr && j < Nc) {
+            double2 &o = C[i * ldc + j];
+            o.x -= cre[r][c][v];
+            o.y -= cim[r][c][v];
+          }
+        }
+  }
+}
+
 //   MODE 0: LU trailing update of the top band after panel [k0, k1)
 //   MODE 1: Y solve, rows above panel block b = nb-1-step
 //   MODE 2: Schur complement F22 -= F21 Y (lower tiles only)
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev, int64_t step,
-                                                      double2 *fac, double2 *pool) {
+// BIG selects the 128x128 tiles (levels whose fronts are large)
+template <int MODE, bool BIG>
+__global__ __launch_bounds__(BIG ? kBigThreads : 256, 2) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                         double2 *fac, double2 *pool) {
   const DNode nd = nodes[lev[blockIdx.y]];
   const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
   double2 *T = fac + nd.fac;
+  auto run = [&](const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc, int64_t Mr,
+                 int64_t Nc, int64_t Kd, bool lower) {
+    if constexpr (BIG) mfma_tiles128(A, lda, B, ldb, C, ldc, Mr, Nc, Kd, lower);
+    else mfma_tiles<false>(A, lda, B, ldb, C, ldc, Mr, Nc, Kd, lower);
+  };
   if (MODE == 0) {
     const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
     if (k0 >= n1) return;
-    mfma_tiles<false>(T + k1 * m + k0, m, T + k0 * m + k1, m, T + k1 * m + k1, m, n1 - k1, m - k1, k1 - k0, false);
+    run(T + k1 * m + k0, m, T + k0 * m + k1, m, T + k1 * m + k1, m, n1 - k1, m - k1, k1 - k0, false);
   } else if (MODE == 1) {
     const int64_t nbk = (n1 + kNB - 1) / kNB, b = nbk - 1 - step;
     if (b <= 0 || n2 == 0) return;  // block 0 has no rows above it
     const int64_t r0 = b * kNB, r1 = (r0 + kNB < n1) ? r0 + kNB : n1;
-    mfma_tiles<false>(T + r0, m, T + r0 * m + n1, m, T + n1, m, r0, n2, r1 - r0, false);
+    run(T + r0, m, T + r0 * m + n1, m, T + n1, m, r0, n2, r1 - r0, false);
   } else {
     double2 *Bb = pool + nd.pool;
     if (n1 == 0 || n2 == 0) return;
-    mfma_tiles<false>(Bb, m, T + n1, m, Bb + n1, m, n2, n2, n1, true);
+    run(Bb, m, T + n1, m, Bb + n1, m, n2, n2, n1, true);
   }
 }
 
@@ -1075,6 +1195,7 @@ void direct_free(mvf_handle *h) {
   dfree(P->d_piv); dfree(P->d_perm); dfree(P->d_pos); dfree(P->d_fac); dfree(P->d_inv); dfree(P->d_npert);
   dfree(P->d_sv);
   dfree(P->d_zb);
+  dfree(P->d_arena);
   delete P;
   h->direct = nullptr;
 }
@@ -1142,6 +1263,10 @@ static int direct_plan(mvf_handle *h) {
   MVF_TRY(dalloc(h, &P->d_perm, nne));
   MVF_TRY(dalloc(h, &P->d_npert, 1));
   MVF_TRY(dalloc(h, &P->d_fac, P->fac_entries));
+  P->arena = 0;
+  for (int d = 0; d <= P->maxdepth; d++)
+    P->arena = std::max(P->arena, P->lev_pool[d] + (d < P->maxdepth ? P->lev_pool[d + 1] : 0));
+  MVF_TRY(dalloc(h, &P->d_arena, P->arena));
   MVF_TRY(dalloc(h, &P->d_inv, P->inv_entries));
   MVF_HIP(h, hipMemcpy(P->d_nodes, P->nodes.data(), sizeof(DNode) * P->T, hipMemcpyHostToDevice));
   MVF_HIP(h, hipMemcpy(P->d_lev, lev.data(), lev.size() * 4, hipMemcpyHostToDevice));
@@ -1155,10 +1280,17 @@ static int direct_plan(mvf_handle *h) {
 }
 
 // numeric factorization of A_pc at angular frequency w (fl32(omega) as in the operator)
+// levels whose largest front reaches this order use the 128x128 tiles
+constexpr int64_t kBigFront = 512;
+
 template <int MODE>
-static void launch_gemm(unsigned gx, unsigned gy, hipStream_t st, const DNode *nodes, const int32_t *lev,
+static void launch_gemm(bool big, unsigned gx, unsigned gy, hipStream_t st, const DNode *nodes, const int32_t *lev,
                         int64_t step, double2 *fac, double2 *pool) {
-  hipLaunchKernelGGL((k_fa_gemm_mfma<MODE>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
+  if (big)
+    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE, true>), dim3(gx, gy), dim3(kBigThreads), 0, st, nodes, lev, step, fac,
+                       pool);
+  else
+    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE, false>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
 }
 
 static int direct_factor(mvf_handle *h, double w, int c64) {
@@ -1172,7 +1304,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   GemmTimer *gt_ = new GemmTimer();
   gt_->init();
   auto gemm_begin = [&](double flops) {
-    if (gt_->used == GemmTimer::kRing) gt_->drain();
+    if (gt_->used == GemmTimer::kRing) gt_->drain(GemmTimer::kRing / 2);
     gt_->fl[gt_->used] = flops;
     (void)hipEventRecord(gt_->ev[gt_->used][0], st);
   };
@@ -1184,8 +1316,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   for (int d = P->maxdepth; d >= 0; d--) {
     const int nl = (int)P->levels[d].size();
     const int32_t *lev = P->d_lev + P->lev_off[d];
-    double2 *pool = nullptr;
-    MVF_TRY(dalloc(h, &pool, P->lev_pool[d]));
+    double2 *pool = (d % 2 == 0) ? P->d_arena : P->d_arena + (P->arena - P->lev_pool[d]);
     const int64_t nfac = P->lev_fac1[d] - P->lev_fac0[d];
     hipLaunchKernelGGL(k_fill2, dim3(cap_grid((nfac + 255) / 256)), dim3(256), 0, st, P->d_fac + P->lev_fac0[d], nfac);
     hipLaunchKernelGGL(k_fill2, dim3(cap_grid((P->lev_pool[d] + 255) / 256)), dim3(256), 0, st, pool, P->lev_pool[d]);
@@ -1199,11 +1330,8 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                            P->d_rel, s, P->d_fac, pool, prev);
     }
     MVF_HIP(h, hipGetLastError());
-    if (prev) {
-      MVF_HIP(h, hipStreamSynchronize(st));
-      dfree(prev);
-      prev = nullptr;
-    }
+    const bool big = P->tile ? P->tile == kBig : mm >= kBigFront;
+    const int64_t tile = big ? kBig : kTile;
     if (mn1 > 0) {
       hipLaunchKernelGGL(k_fa_sym, dim3(cap_grid(((mn1 + 31) / 32) * ((mm + 31) / 32)), nl), dim3(256), 0, st,
                          P->d_nodes, lev, P->d_fac, pool);
@@ -1214,7 +1342,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           const DNode &n = P->nodes[t];
           if (k >= n.n1) continue;
           const int64_t k1 = std::min(k + kNB, n.n1);
-          gt = std::max(gt, ((n.n1 - k1 + kTile - 1) / kTile) * ((n.m - k1 + kTile - 1) / kTile));
+          gt = std::max(gt, ((n.n1 - k1 + tile - 1) / tile) * ((n.m - k1 + tile - 1) / tile));
           gc = std::max(gc, n.m);
           gfl += 8.0 * (double)(n.n1 - k1) * (double)(n.m - k1) * (double)(k1 - k);
         }
@@ -1235,7 +1363,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                            lev, k, P->d_fac, P->d_inv);
         if (gt > 0) {
           gemm_begin(gfl);
-          launch_gemm<0>(cap_grid(gt), nl, st, P->d_nodes, lev, k, P->d_fac, pool);
+          launch_gemm<0>(big, cap_grid(gt), nl, st, P->d_nodes, lev, k, P->d_fac, pool);
           gemm_end();
         }
       }
@@ -1249,25 +1377,25 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
             const int64_t b = (n.n1 + kNB - 1) / kNB - 1 - s;
             if (b < 0 || n.n2 == 0) continue;
             const int64_t r0 = b * kNB, r1 = std::min(r0 + kNB, n.n1);
-            gt = std::max(gt, ((r0 + kTile - 1) / kTile) * ((n.n2 + kTile - 1) / kTile));
+            gt = std::max(gt, ((r0 + tile - 1) / tile) * ((n.n2 + tile - 1) / tile));
             gfl += 8.0 * (double)r0 * (double)n.n2 * (double)(r1 - r0);
           }
           hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
                              P->d_nodes, lev, s, P->d_fac);
           if (gt > 0) {
             gemm_begin(gfl);
-            launch_gemm<1>(cap_grid(gt), nl, st, P->d_nodes, lev, s, P->d_fac, pool);
+            launch_gemm<1>(big, cap_grid(gt), nl, st, P->d_nodes, lev, s, P->d_fac, pool);
             gemm_end();
           }
         }
-        const int64_t tq = (mn2 + kTile - 1) / kTile;
+        const int64_t tq = (mn2 + tile - 1) / tile;
         double sfl = 0.0;
         for (int t : P->levels[d]) {
           const DNode &n = P->nodes[t];
           sfl += 8.0 * (double)n.n2 * (double)(n.n2 + 1) / 2.0 * (double)n.n1;
         }
         gemm_begin(sfl);
-        launch_gemm<2>(cap_grid(tq * tq), nl, st, P->d_nodes, lev, 0, P->d_fac, pool);
+        launch_gemm<2>(big, cap_grid(tq * tq), nl, st, P->d_nodes, lev, 0, P->d_fac, pool);
         gemm_end();
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
@@ -1288,7 +1416,6 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
     }
   }
   MVF_HIP(h, hipStreamSynchronize(st));
-  dfree(prev);
   gt_->destroy();
   P->gemm_ms = gt_->ms;
   P->gemm_flops = gt_->flops;
@@ -1436,7 +1563,12 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
     whi = std::max(whi, S.w.w[c]);
   }
   const double w0 = std::sqrt(wlo * whi);
-  if (!P->factored || P->omega != w0 || P->c64 != (int)S.c64) MVF_TRY(direct_factor(h, w0, (int)S.c64));
+  if (o->direct_tile != 0 && o->direct_tile != kTile && o->direct_tile != kBig)
+    return set_err(h, MVF_E_ARG, "direct_tile must be 0, %d or %d", kTile, kBig);
+  if (!P->factored || P->omega != w0 || P->c64 != (int)S.c64 || P->tile != o->direct_tile) {
+    P->tile = o->direct_tile;
+    MVF_TRY(direct_factor(h, w0, (int)S.c64));
+  }
   info->factor_seconds += P->t_factor;
   info->factorizations = (int32_t)P->factorizations;
   info->static_pivots = P->npert;

@@ -40,7 +40,7 @@ module movfem_gpu
     real(c_double) :: tol
     integer(c_int32_t) :: maxit, check_every
     real(c_double) :: inner_tol
-    integer(c_int32_t) :: use_graph, gmres_m, freqs_per_factor, pad_
+    integer(c_int32_t) :: use_graph, gmres_m, freqs_per_factor, direct_tile
   end type mvf_solver_opts
 
   type, bind(C) :: mvf_solve_info

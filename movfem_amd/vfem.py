@@ -37,7 +37,7 @@ def _ip(a):
 
 
 def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacobi", tol=1e-8, maxit=100000,
-                check_every=25, inner_tol=1e-2, gmres_m=0, freqs_per_factor=1) -> L.SolverOpts:
+                check_every=25, inner_tol=1e-2, gmres_m=0, freqs_per_factor=1, direct_tile=0) -> L.SolverOpts:
     o = L.SolverOpts()
     L.lib().mvf_default_solver_opts(C.byref(o))
     o.method = {"cocg": L.MVF_KRYLOV_COCG, "bicgstab": L.MVF_KRYLOV_BICGSTAB, "gmres": L.MVF_KRYLOV_GMRES}[method]
@@ -46,6 +46,7 @@ def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacob
     o.precond = {"jacobi": L.MVF_PC_JACOBI, "none": L.MVF_PC_NONE, "direct": L.MVF_PC_DIRECT}[precond]
     o.gmres_m = gmres_m
     o.freqs_per_factor = freqs_per_factor
+    o.direct_tile = direct_tile
     o.tol = tol
     o.maxit = maxit
     o.check_every = check_every

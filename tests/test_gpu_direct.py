@@ -41,17 +41,20 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("tile", [0, 128])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_direct_matches_superlu(name, need_gpu):
+def test_direct_matches_superlu(name, tile, need_gpu):
+    """tile 0 picks the GEMM tile by front size (64 on these small fronts);
+    128 forces the large-front kernel onto every level and edge shape."""
     m = CASES[name]()
-    print(f"direct {name}: building", flush=True)
+    print(f"direct {name} tile {tile}: building", flush=True)
     g = GpuVfem(m)
     g.set_frequency(0)
     A, b = reference_system(g)
     lu = spla.splu(A)
     xref = np.stack([lu.solve(b[d]) for d in range(2)])
     print(f"direct {name}: nne={g.nne} solving", flush=True)
-    x, info = g.solve(b.ravel(), method="gmres", precond="direct", tol=1e-8)
+    x, info = g.solve(b.ravel(), method="gmres", precond="direct", tol=1e-8, direct_tile=tile)
     x = x.reshape(2, -1)
     print(f"direct {name}: its={info['iterations']} applies={info['precond_applies']} "
           f"pert={info['static_pivots']} factor {info['factor_seconds']:.3f}s", flush=True)
