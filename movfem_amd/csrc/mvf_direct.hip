@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "mvf_internal.h"
@@ -878,51 +879,123 @@ __device__ __forceinline__ void mfma_tiles128(const double2 *A, int64_t lda, con
   }
 }
 
-//   MODE 0: LU trailing update of the top band after panel [k0, k1)
-//   MODE 1: Y solve, rows above panel block b = nb-1-step
+// Two-level blocking of the top band's LU and of the Y solve: 64-wide panels
+// (kNB) inside outer blocks of kOuter columns (rows for Y).  Updates that
+// reach past the outer block are deferred and applied once per outer block
+// with K = kOuter, so the trailing part of a front is read and written
+// kOuter/kNB times less often than by rank-64 updates.
+constexpr int64_t kOuter = 4 * kNB;
+
+// operands of the batched GEMM launches, as offsets into the front's top
+// band T (or its pool band for the Schur update), all with leading dimension m
+//   MODE 0: LU, inner: T[ke:n1, ke:K1] -= T[ke:n1, k:ke] T[k:ke, ke:K1]
+//   MODE 3: LU, rows of the outer block past it: T[ke:K1, K1:m] -= T[ke:K1, k:ke] T[k:ke, K1:m]
+//   MODE 4: LU, outer: T[K1:n1, K1:m] -= T[K1:n1, K0:K1] T[K0:K1, K1:m]
+//   MODE 1: Y, inner: rows [R0, r0) of the outer group of block b = nb-1-step
+//   MODE 5: Y, outer: rows [0, R0) once the group's first block b = R0/kNB is solved
 //   MODE 2: Schur complement F22 -= F21 Y (lower tiles only)
-// BIG selects the 128x128 tiles (levels whose fronts are large)
-template <int MODE, bool BIG>
-__global__ __launch_bounds__(BIG ? kBigThreads : 256, 2) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev, int64_t step,
-                                                         double2 *fac, double2 *pool) {
-  const DNode nd = nodes[lev[blockIdx.y]];
+// with ke = min(k + kNB, n1) and K1 = min(K0 + kOuter, n1).
+struct GemmArgs {
+  int64_t a, b, c, Mr, Nc, Kd;
+  bool ok;
+};
+__host__ __device__ inline GemmArgs gemm_args(int mode, const DNode &nd, int64_t step, int64_t K0) {
   const int64_t n1 = nd.n1, n2 = nd.n2, m = nd.m;
-  double2 *T = fac + nd.fac;
-  auto run = [&](const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc, int64_t Mr,
-                 int64_t Nc, int64_t Kd, bool lower) {
-    if constexpr (BIG) mfma_tiles128(A, lda, B, ldb, C, ldc, Mr, Nc, Kd, lower);
-    else mfma_tiles<false>(A, lda, B, ldb, C, ldc, Mr, Nc, Kd, lower);
-  };
-  if (MODE == 0) {
-    const int64_t k0 = step, k1 = (step + kNB < n1) ? step + kNB : n1;
-    if (k0 >= n1) return;
-    run(T + k1 * m + k0, m, T + k0 * m + k1, m, T + k1 * m + k1, m, n1 - k1, m - k1, k1 - k0, false);
-  } else if (MODE == 1) {
-    const int64_t nbk = (n1 + kNB - 1) / kNB, b = nbk - 1 - step;
-    if (b <= 0 || n2 == 0) return;  // block 0 has no rows above it
-    const int64_t r0 = b * kNB, r1 = (r0 + kNB < n1) ? r0 + kNB : n1;
-    run(T + r0, m, T + r0 * m + n1, m, T + n1, m, r0, n2, r1 - r0, false);
+  GemmArgs g{0, 0, 0, 0, 0, 0, false};
+  if (mode == 0 || mode == 3) {
+    const int64_t k = step;
+    if (k >= n1) return g;
+    const int64_t ke = (k + kNB < n1) ? k + kNB : n1, K1 = (K0 + kOuter < n1) ? K0 + kOuter : n1;
+    g.a = ke * m + k;
+    g.Kd = ke - k;
+    if (mode == 0) {
+      g.b = k * m + ke;
+      g.c = ke * m + ke;
+      g.Mr = n1 - ke;
+      g.Nc = K1 - ke;
+    } else {
+      g.b = k * m + K1;
+      g.c = ke * m + K1;
+      g.Mr = K1 - ke;
+      g.Nc = m - K1;
+    }
+  } else if (mode == 4) {
+    if (K0 >= n1) return g;
+    const int64_t K1 = (K0 + kOuter < n1) ? K0 + kOuter : n1;
+    g.a = K1 * m + K0;
+    g.b = K0 * m + K1;
+    g.c = K1 * m + K1;
+    g.Mr = n1 - K1;
+    g.Nc = m - K1;
+    g.Kd = K1 - K0;
+  } else if (mode == 1 || mode == 5) {
+    const int64_t nbk = (n1 + kNB - 1) / kNB, bb = nbk - 1 - step;
+    if (bb < 0 || n2 == 0) return g;
+    const int64_t r0 = bb * kNB, r1 = (r0 + kNB < n1) ? r0 + kNB : n1;
+    g.Nc = n2;
+    if (mode == 1) {
+      const int64_t R0 = (bb / 4) * 4 * kNB;
+      g.a = R0 * m + r0;
+      g.b = r0 * m + n1;
+      g.c = R0 * m + n1;
+      g.Mr = r0 - R0;
+      g.Kd = r1 - r0;
+    } else {
+      if (bb % 4 != 0) return g;
+      const int64_t R1 = (r0 + kOuter < n1) ? r0 + kOuter : n1;
+      g.a = r0;
+      g.b = r0 * m + n1;
+      g.c = n1;
+      g.Mr = r0;
+      g.Kd = R1 - r0;
+    }
   } else {
-    double2 *Bb = pool + nd.pool;
-    if (n1 == 0 || n2 == 0) return;
-    run(Bb, m, T + n1, m, Bb + n1, m, n2, n2, n1, true);
+    g.a = 0;
+    g.b = n1;
+    g.c = n1;
+    g.Mr = n2;
+    g.Nc = n2;
+    g.Kd = n1;
   }
+  g.ok = g.Mr > 0 && g.Nc > 0 && g.Kd > 0;
+  return g;
 }
 
-// U row block on the matrix cores, in place:
-//   T[k:k+w, k+w:m] = Linv_b T[k:k+w, k+w:m],  b = k / kNB
-__global__ __launch_bounds__(256) void k_fa_urow(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
-                                                 const double2 *inv) {
+// BIG selects the 128x128 tiles (levels whose fronts are large)
+template <int MODE, bool BIG>
+__global__ __launch_bounds__(BIG ? kBigThreads : 256, 2) void k_fa_gemm_mfma(const DNode *nodes, const int32_t *lev,
+                                                                            int64_t step, int64_t K0, double2 *fac,
+                                                                            double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const GemmArgs g = gemm_args(MODE, nd, step, K0);
+  if (!g.ok) return;
+  const int64_t m = nd.m;
+  double2 *T = fac + nd.fac;
+  double2 *AC = (MODE == 2) ? pool + nd.pool : T;  // the Schur update reads F21 and writes F22 in the pool band
+  if constexpr (BIG) mfma_tiles128(AC + g.a, m, T + g.b, m, AC + g.c, m, g.Mr, g.Nc, g.Kd, MODE == 2);
+  else mfma_tiles<false>(AC + g.a, m, T + g.b, m, AC + g.c, m, g.Mr, g.Nc, g.Kd, MODE == 2);
+}
+
+// U row block on the matrix cores, in place, for columns [c0, c1):
+//   T[k:k+w, c0:c1] = Linv_b T[k:k+w, c0:c1],  b = k / kNB
+// part 0: c0 = k + w, c1 = K1 (inside the outer block); part 1: c0 = K1, c1 = m
+__host__ __device__ inline void urow_cols(const DNode &nd, int64_t k, int64_t K0, int part, int64_t &c0, int64_t &c1) {
+  const int64_t n1 = nd.n1, w = (k + kNB < n1) ? kNB : n1 - k, K1 = (K0 + kOuter < n1) ? K0 + kOuter : n1;
+  c0 = part ? K1 : k + w;
+  c1 = part ? nd.m : K1;
+}
+__global__ __launch_bounds__(256) void k_fa_urow(const DNode *nodes, const int32_t *lev, int64_t k, int64_t K0,
+                                                 int part, double2 *fac, const double2 *inv) {
   const DNode nd = nodes[lev[blockIdx.y]];
   const int64_t n1 = nd.n1, m = nd.m;
   if (k >= n1) return;
   const int64_t b = k / kNB, w = (k + kNB < n1) ? kNB : n1 - k;
-  if (k + w >= m) return;
+  int64_t c0, c1;
+  urow_cols(nd, k, K0, part, c0, c1);
+  if (c0 >= c1) return;
   double2 *T = fac + nd.fac;
-  mfma_tiles<true>(inv + nd.inv + kNB * kNB * b, w, T + k * m + k + w, m, T + k * m + k + w, m, w, m - k - w, w,
-                   false);
+  mfma_tiles<true>(inv + nd.inv + kNB * kNB * b, w, T + k * m + c0, m, T + k * m + c0, m, w, c1 - c0, w, false);
 }
-
 
 // row permutation of each front from its interchange sequence
 __global__ void k_fa_perm(const DNode *nodes, const int32_t *lev, int nl, const int32_t *piv, int32_t *perm) {
@@ -1285,12 +1358,13 @@ constexpr int64_t kBigFront = 512;
 
 template <int MODE>
 static void launch_gemm(bool big, unsigned gx, unsigned gy, hipStream_t st, const DNode *nodes, const int32_t *lev,
-                        int64_t step, double2 *fac, double2 *pool) {
+                        int64_t step, int64_t K0, double2 *fac, double2 *pool) {
   if (big)
-    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE, true>), dim3(gx, gy), dim3(kBigThreads), 0, st, nodes, lev, step, fac,
-                       pool);
+    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE, true>), dim3(gx, gy), dim3(kBigThreads), 0, st, nodes, lev, step, K0,
+                       fac, pool);
   else
-    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE, false>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, fac, pool);
+    hipLaunchKernelGGL((k_fa_gemm_mfma<MODE, false>), dim3(gx, gy), dim3(256), 0, st, nodes, lev, step, K0, fac,
+                       pool);
 }
 
 static int direct_factor(mvf_handle *h, double w, int c64) {
@@ -1335,68 +1409,80 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
     if (mn1 > 0) {
       hipLaunchKernelGGL(k_fa_sym, dim3(cap_grid(((mn1 + 31) / 32) * ((mm + 31) / 32)), nl), dim3(256), 0, st,
                          P->d_nodes, lev, P->d_fac, pool);
-      for (int64_t k = 0; k < mn1; k += kNB) {
-        int64_t gt = 0, gc = 0;
+      // one GEMM launch over the level's fronts: grid from the largest operand, algorithmic flops summed
+      auto gemm = [&](auto mode_tag, int64_t step, int64_t K0) {
+        constexpr int MODE = decltype(mode_tag)::value;
+        int64_t gt = 0;
         double gfl = 0.0;
         for (int t : P->levels[d]) {
-          const DNode &n = P->nodes[t];
-          if (k >= n.n1) continue;
-          const int64_t k1 = std::min(k + kNB, n.n1);
-          gt = std::max(gt, ((n.n1 - k1 + tile - 1) / tile) * ((n.m - k1 + tile - 1) / tile));
-          gc = std::max(gc, n.m);
-          gfl += 8.0 * (double)(n.n1 - k1) * (double)(n.m - k1) * (double)(k1 - k);
+          const GemmArgs g = gemm_args(MODE, P->nodes[t], step, K0);
+          if (!g.ok) continue;
+          gt = std::max(gt, ((g.Mr + tile - 1) / tile) * ((g.Nc + tile - 1) / tile));
+          gfl += MODE == 2 ? 8.0 * (double)g.Mr * (double)(g.Mr + 1) / 2.0 * (double)g.Kd
+                           : 8.0 * (double)g.Mr * (double)g.Nc * (double)g.Kd;
         }
-        hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
-                           P->d_piv, P->d_npert, big, tiny);
-        int64_t gr = 0;
+        if (gt == 0) return;
+        gemm_begin(gfl);
+        launch_gemm<MODE>(big, cap_grid(gt), nl, st, P->d_nodes, lev, step, K0, P->d_fac, pool);
+        gemm_end();
+      };
+      auto urow = [&](int64_t k, int64_t K0, int part) {
+        int64_t gu = 0;
         for (int t : P->levels[d]) {
           const DNode &n = P->nodes[t];
           if (k >= n.n1) continue;
-          gr = std::max(gr, n.n1 - std::min(k + kNB, n.n1));
+          int64_t c0, c1;
+          urow_cols(n, k, K0, part, c0, c1);
+          if (c1 > c0) gu = std::max(gu, (c1 - c0 + kTile - 1) / kTile);
         }
-        if (gr > 0)
-          hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
-                             P->d_nodes, lev, k, P->d_fac);
-        hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes, lev,
-                           k, P->d_fac, P->d_piv);
-        hipLaunchKernelGGL(k_fa_urow, dim3(cap_grid((gc - k + kTile - 1) / kTile), nl), dim3(256), 0, st, P->d_nodes,
-                           lev, k, P->d_fac, P->d_inv);
-        if (gt > 0) {
-          gemm_begin(gfl);
-          launch_gemm<0>(big, cap_grid(gt), nl, st, P->d_nodes, lev, k, P->d_fac, pool);
-          gemm_end();
-        }
-      }
-      if (mn2 > 0) {
-        const int64_t nsteps = (mn1 + kNB - 1) / kNB;
-        for (int64_t s = 0; s < nsteps; s++) {
-          int64_t gt = 0;
-          double gfl = 0.0;
+        if (gu > 0)
+          hipLaunchKernelGGL(k_fa_urow, dim3(cap_grid(gu), nl), dim3(256), 0, st, P->d_nodes, lev, k, K0, part,
+                             P->d_fac, P->d_inv);
+      };
+      using M0 = std::integral_constant<int, 0>;
+      using M1 = std::integral_constant<int, 1>;
+      using M2 = std::integral_constant<int, 2>;
+      using M3 = std::integral_constant<int, 3>;
+      using M4 = std::integral_constant<int, 4>;
+      using M5 = std::integral_constant<int, 5>;
+      // LU of the top bands, two-level: panels of kNB inside outer blocks of kOuter
+      for (int64_t K0 = 0; K0 < mn1; K0 += kOuter) {
+        const int64_t Kend = std::min(K0 + kOuter, mn1);
+        for (int64_t k = K0; k < Kend; k += kNB) {
+          int64_t gc = 0, gr = 0;
           for (int t : P->levels[d]) {
             const DNode &n = P->nodes[t];
-            const int64_t b = (n.n1 + kNB - 1) / kNB - 1 - s;
-            if (b < 0 || n.n2 == 0) continue;
-            const int64_t r0 = b * kNB, r1 = std::min(r0 + kNB, n.n1);
-            gt = std::max(gt, ((r0 + tile - 1) / tile) * ((n.n2 + tile - 1) / tile));
-            gfl += 8.0 * (double)r0 * (double)n.n2 * (double)(r1 - r0);
+            if (k >= n.n1) continue;
+            gc = std::max(gc, n.m);
+            gr = std::max(gr, n.n1 - std::min(k + kNB, n.n1));
           }
+          hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
+                             P->d_piv, P->d_npert, big, tiny);
+          if (gr > 0)
+            hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
+                               P->d_nodes, lev, k, P->d_fac);
+          hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes,
+                             lev, k, P->d_fac, P->d_piv);
+          urow(k, K0, 0);
+          gemm(M0{}, k, K0);
+        }
+        // deferred part of the outer block's rows past it, then the rest of the front with K = kOuter
+        for (int64_t k = K0; k < Kend; k += kNB) {
+          urow(k, K0, 1);
+          gemm(M3{}, k, K0);
+        }
+        gemm(M4{}, 0, K0);
+      }
+      if (mn2 > 0) {
+        // Y = U^-1 (L^-1 P F12), blocks from the bottom, outer groups of four blocks
+        const int64_t nsteps = (mn1 + kNB - 1) / kNB;
+        for (int64_t s = 0; s < nsteps; s++) {
           hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
                              P->d_nodes, lev, s, P->d_fac);
-          if (gt > 0) {
-            gemm_begin(gfl);
-            launch_gemm<1>(big, cap_grid(gt), nl, st, P->d_nodes, lev, s, P->d_fac, pool);
-            gemm_end();
-          }
+          gemm(M1{}, s, 0);
+          gemm(M5{}, s, 0);
         }
-        const int64_t tq = (mn2 + tile - 1) / tile;
-        double sfl = 0.0;
-        for (int t : P->levels[d]) {
-          const DNode &n = P->nodes[t];
-          sfl += 8.0 * (double)n.n2 * (double)(n.n2 + 1) / 2.0 * (double)n.n1;
-        }
-        gemm_begin(sfl);
-        launch_gemm<2>(big, cap_grid(tq * tq), nl, st, P->d_nodes, lev, 0, P->d_fac, pool);
-        gemm_end();
+        gemm(M2{}, 0, 0);
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
     }
