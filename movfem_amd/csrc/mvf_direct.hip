@@ -601,74 +601,124 @@ __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32
   for (int e = t; e < w * w; e += 64) Li[e] = x[e / w][e % w];
 }
 
+constexpr int kTrsmThreads = 256;
+constexpr int kTrsmChunk = 16;
+
 // L rows below the diagonal block: T[i, k:kend] = T[i, k:kend] U_kk^-1 for
-// i in [kend, n1); each thread owns one row, held in LDS
+// i in [kend, n1).  One thread per row walks the block's columns in 16-wide
+// chunks: a chunk is loaded into registers, reduced by the chunks solved
+// before it (read back from the row, where the thread itself stored them),
+// solved by forward substitution and stored; U_kk and the reciprocal pivots
+// are read from LDS as broadcasts.
 template <int NB>
-__global__ __launch_bounds__(kColT) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
+__global__ __launch_bounds__(kTrsmThreads, 2) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k,
+                                                            double2 *fac) {
   const DNode nd = nodes[lev[blockIdx.y]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
-  if (kend + (int64_t)blockIdx.x * kColT >= nd.n1) return;
+  if (kend + (int64_t)blockIdx.x * kTrsmThreads >= nd.n1) return;
   const int w = (int)(kend - k);
   double2 *T = fac + nd.fac;
-  __shared__ double2 U[NB][NB + 1];
+  __shared__ double2 U[NB][NB];
   __shared__ double2 dinv[NB];
-  __shared__ double2 xs[NB][kColT + 1];
-  for (int e = threadIdx.x; e < NB * NB; e += kColT) {
+  for (int e = threadIdx.x; e < NB * NB; e += kTrsmThreads) {
     const int r = e / NB, c = e % NB;
     U[r][c] = (r < w && c < w && c >= r) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
   __syncthreads();
-  const int t = threadIdx.x;
-  for (int64_t i0 = kend + (int64_t)blockIdx.x * kColT; i0 < nd.n1; i0 += (int64_t)gridDim.x * kColT) {
-    const int64_t i = i0 + t;
-    if (i < nd.n1) {
-      double2 *row = T + i * m + k;
-      for (int q = 0; q < w; q++) xs[q][t] = row[q];
-      for (int q = 0; q < w; q++) {
-        double2 v = xs[q][t];
-        for (int p = 0; p < q; p++) cfms(v, xs[p][t], U[p][q]);
-        xs[q][t] = cmul(v, dinv[q]);
+  constexpr int C = kTrsmChunk / 2, NQ = NB / C;
+  for (int64_t i0 = kend + (int64_t)blockIdx.x * kTrsmThreads; i0 < nd.n1; i0 += (int64_t)gridDim.x * kTrsmThreads) {
+    const int64_t i = i0 + threadIdx.x;
+    if (i >= nd.n1) continue;
+    double2 *row = T + i * m + k;
+    #pragma unroll 1
+    for (int q = 0; q < NQ && q * C < w; q++) {
+      double2 x[C];
+#pragma unroll
+      for (int c = 0; c < C; c++) x[c] = (q * C + c < w) ? row[q * C + c] : make_double2(0.0, 0.0);
+      #pragma unroll 1
+      for (int q2 = 0; q2 < q; q2++) {
+        double2 y[C];
+#pragma unroll
+        for (int p = 0; p < C; p++) y[p] = row[q2 * C + p];
+#pragma unroll
+        for (int p = 0; p < C; p++)
+#pragma unroll
+          for (int c = 0; c < C; c++) cfms(x[c], y[p], U[q2 * C + p][q * C + c]);
       }
-      for (int q = 0; q < w; q++) row[q] = xs[q][t];
+      // columns past w hold zeros and zero U entries, so they stay zero
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        double2 v = x[c];
+#pragma unroll
+        for (int p = 0; p < c; p++) cfms(v, x[p], U[q * C + p][q * C + c]);
+        x[c] = cmul(v, dinv[q * C + c]);
+      }
+#pragma unroll
+      for (int c = 0; c < C; c++)
+        if (q * C + c < w) row[q * C + c] = x[c];
     }
   }
 }
 
 // Y block rows by back substitution, panel blocks from the last one (b = nb-1-step):
-// T[r0:r0+w, n1:m] = U_bb^-1 T[r0:r0+w, n1:m]
+// T[r0:r0+w, n1:m] = U_bb^-1 T[r0:r0+w, n1:m].  One thread per column walks
+// the block in 16-row chunks from the bottom: a chunk is loaded into
+// registers, reduced by the chunks solved before it (read back from the
+// band, where the thread itself stored them), solved by substitution and
+// stored.  U_bb and the reciprocal pivots are read from LDS as broadcasts.
 template <int NB>
-__global__ __launch_bounds__(kColT) void k_fa_trsm_y(const DNode *nodes, const int32_t *lev, int64_t step,
-                                                   double2 *fac) {
+__global__ __launch_bounds__(kTrsmThreads, 2) void k_fa_trsm_y(const DNode *nodes, const int32_t *lev, int64_t step,
+                                                             double2 *fac) {
   const DNode nd = nodes[lev[blockIdx.y]];
   const int64_t b = (nd.n1 + NB - 1) / NB - 1 - step;
   if (b < 0 || nd.n2 == 0) return;
   const int64_t r0 = b * NB, r1 = r0 + NB < nd.n1 ? r0 + NB : nd.n1, m = nd.m;
   const int w = (int)(r1 - r0);
   double2 *T = fac + nd.fac;
-  __shared__ double2 U[NB][NB + 1];
+  __shared__ double2 U[NB][NB];
   __shared__ double2 dinv[NB];
-  __shared__ double2 xs[NB][kColT + 1];
-  for (int e = threadIdx.x; e < NB * NB; e += kColT) {
+  for (int e = threadIdx.x; e < NB * NB; e += kTrsmThreads) {
     const int r = e / NB, c = e % NB;
     U[r][c] = (r < w && c < w && c >= r) ? T[(r0 + r) * m + r0 + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
   __syncthreads();
-  const int t = threadIdx.x;
-  for (int64_t j0 = nd.n1 + (int64_t)blockIdx.x * kColT; j0 < m; j0 += (int64_t)gridDim.x * kColT) {
-    const int64_t j = j0 + t;
-    if (j < m) {
-      for (int r = 0; r < w; r++) xs[r][t] = T[(r0 + r) * m + j];
-      for (int r = w - 1; r >= 0; r--) {
-        double2 v = xs[r][t];
-        for (int c = r + 1; c < w; c++) cfms(v, U[r][c], xs[c][t]);
-        xs[r][t] = cmul(v, dinv[r]);
+  constexpr int C = kTrsmChunk, NQ = NB / kTrsmChunk;
+  for (int64_t j0 = nd.n1 + (int64_t)blockIdx.x * kTrsmThreads; j0 < m; j0 += (int64_t)gridDim.x * kTrsmThreads) {
+    const int64_t j = j0 + threadIdx.x;
+    if (j >= m) continue;
+    double2 *col = T + r0 * m + j;
+    #pragma unroll 1
+    for (int q = NQ - 1; q >= 0; q--) {
+      if (q * C >= w) continue;
+      double2 x[C];
+#pragma unroll
+      for (int r = 0; r < C; r++) x[r] = (q * C + r < w) ? col[(int64_t)(q * C + r) * m] : make_double2(0.0, 0.0);
+      #pragma unroll 1
+      for (int q2 = q + 1; q2 < NQ && q2 * C < w; q2++) {
+        double2 y[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) y[c] = (q2 * C + c < w) ? col[(int64_t)(q2 * C + c) * m] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < C; r++)
+#pragma unroll
+          for (int c = 0; c < C; c++) cfms(x[r], U[q * C + r][q2 * C + c], y[c]);
       }
-      for (int r = 0; r < w; r++) T[(r0 + r) * m + j] = xs[r][t];
+      // rows past w hold zeros and zero U entries, so they stay zero
+#pragma unroll
+      for (int r = C - 1; r >= 0; r--) {
+        double2 v = x[r];
+#pragma unroll
+        for (int c = r + 1; c < C; c++) cfms(v, U[q * C + r][q * C + c], x[c]);
+        x[r] = cmul(v, dinv[q * C + r]);
+      }
+#pragma unroll
+      for (int r = 0; r < C; r++)
+        if (q * C + r < w) col[(int64_t)(q * C + r) * m] = x[r];
     }
   }
 }
@@ -1459,7 +1509,8 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
                              P->d_piv, P->d_npert, big, tiny);
           if (gr > 0)
-            hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
+            hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kTrsmThreads - 1) / kTrsmThreads), nl),
+                               dim3(kTrsmThreads), 0, st,
                                P->d_nodes, lev, k, P->d_fac);
           hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes,
                              lev, k, P->d_fac, P->d_piv);
@@ -1477,7 +1528,8 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
         // Y = U^-1 (L^-1 P F12), blocks from the bottom, outer groups of four blocks
         const int64_t nsteps = (mn1 + kNB - 1) / kNB;
         for (int64_t s = 0; s < nsteps; s++) {
-          hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kColT - 1) / kColT), nl), dim3(kColT), 0, st,
+          hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kTrsmThreads - 1) / kTrsmThreads), nl),
+                             dim3(kTrsmThreads), 0, st,
                              P->d_nodes, lev, s, P->d_fac);
           gemm(M1{}, s, 0);
           gemm(M5{}, s, 0);
