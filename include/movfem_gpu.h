@@ -234,10 +234,52 @@ int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms);
  * element layer ke_top (1-based: the reference's top earth layer) inside the
  * region ie, je in [nextd+1, n-nextd] (get_rx_element); rx_e/rx_h (nrx, 2, 3)
  * complex, rx_z (nrx, 4) complex [Zxx Zxy Zyx Zyy], rho/phi (nrx, 4), degrees.
- * A receiver outside that region is MVF_E_ARG.  Outputs may be NULL. */
+ * A receiver outside that region is MVF_E_ARG.  rx_elev (nrx): the receiver's
+ * height on that face (interpolate_rx overwrites g_zsr with it,
+ * receiver_data.f90:222-225).  node_z (npt, 4) complex, node_rho/node_phi
+ * (npt, 4): Z, rho_a and phase at every lattice node (z_rho_phi,
+ * solution.f90:51-69).  Outputs may be NULL. */
 int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32_t nrx, const double *rx,
                 const double *ry, double *esol, double *hsol, double *rx_e, double *rx_h, double *rx_z, double *rho,
-                double *phi);
+                double *phi, double *rx_elev, double *node_z, double *node_rho, double *node_phi);
+
+/* ---- text writers (host only; no device use) ------------------------------
+ * The reference's output files, one frequency block per call: nf is the
+ * 1-based frequency index printed in the block headers; append = 0 creates
+ * the file and writes the file header the reference writes for nf = 1,
+ * append = 1 adds a block.  Column formats are the reference's edit
+ * descriptors (es20.5, f20.5, f25.6, A15, i5); its list-directed lines
+ * (write(u,*)) follow the reference build's list-directed form: a leading
+ * blank, one blank before every item, records wrapped before 80 columns,
+ * integers in minimal width and reals as the shortest decimal that reads
+ * back exactly.  Field arrays use mvf_readout's layouts.  Returns 0 or
+ * MVF_E_ARG (bad arguments or an I/O error). */
+
+/* SOLUTION.OUT (write_solution, solution.f90:73-205): nodal E, H (24es20.5
+ * per node), Z (8es20.5), rho_a/phase (8f20.5; 8f25.6 when a node selection
+ * is given, the reference's inner-region form).  sel: nsel 1-based node ids
+ * (the reference's g_nb), or NULL for every node in id order. */
+int mvf_write_solution(const char *path, int32_t nf, int32_t append, int64_t npt, const double *esol,
+                       const double *hsol, const double *node_z, const double *node_rho, const double *node_phi,
+                       int64_t nsel, const int32_t *sel);
+
+/* RECEIVER.OUT (write_receiver_data, receiver_data.f90:25-67): on create the
+ * receiver count, polarizations (2), frequency count nfreq and the receiver
+ * coordinates (x, y, elev); per block the fields (26es20.5), Z (10es20.5) and
+ * rho_a/phase (10f20.5) with the receiver coordinates in front. */
+int mvf_write_receivers(const char *path, int32_t nf, int32_t append, int32_t nfreq, int32_t nrx, const double *rx,
+                        const double *ry, const double *elev, const double *rx_e, const double *rx_h,
+                        const double *rx_z, const double *rho, const double *phi);
+
+/* SURFACE.OUT (write_surface_data, receiver_data.f90:69-109): nsp = nxb*nyb
+ * surface nodes ids (0-based lattice ids, x fastest within y as sur_data
+ * walks them) with coordinates xs (nxb), ys (nyb), zs (nsp); on create the
+ * counts and coordinates, per block their E, H (24es20.5), Z (8es20.5) and
+ * rho_a/phase (8f20.5) taken from the nodal arrays (npt nodes). */
+int mvf_write_surface(const char *path, int32_t nf, int32_t append, int32_t nfreq, int32_t nxb, int32_t nyb,
+                      const double *xs, const double *ys, const double *zs, const int64_t *ids, int64_t npt,
+                      const double *esol, const double *hsol, const double *node_z, const double *node_rho,
+                      const double *node_phi);
 
 /* Host-only symbolic analysis of the sparse direct factorization for a
  * numbering (gne, ne x me, 1-based, <= 0 removed) and the full CSR pattern

@@ -28,7 +28,7 @@ ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR"
 EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne", "mvf_set_frequency",
            "mvf_integrate", "mvf_global_vfem", "mvf_remove_zeros", "mvf_default_solver_opts", "mvf_solve",
            "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
-           "mvf_direct_plan_stats", "mvf_readout")
+           "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface")
 
 
 class Grid(C.Structure):
@@ -103,7 +103,11 @@ def lib():
     L.mvf_setup_times.argtypes = [vp, dp]
     L.mvf_assemble.argtypes = [vp, C.c_int32, dp]
     L.mvf_direct_plan_stats.argtypes = [C.c_int32] * 4 + [ip, C.c_int64, i64p, ip, dp]
-    L.mvf_readout.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_int32, dp, dp] + [dp] * 7
+    L.mvf_readout.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_int32, dp, dp] + [dp] * 11
+    cp = C.c_char_p
+    L.mvf_write_solution.argtypes = [cp, C.c_int32, C.c_int32, C.c_int64] + [dp] * 5 + [C.c_int64, ip]
+    L.mvf_write_receivers.argtypes = [cp] + [C.c_int32] * 4 + [dp] * 8
+    L.mvf_write_surface.argtypes = [cp] + [C.c_int32] * 5 + [dp] * 3 + [i64p, C.c_int64] + [dp] * 5
     for name in EXPORTS:
         if name not in ("mvf_free", "mvf_last_error", "mvf_default_solver_opts"):
             getattr(L, name).restype = C.c_int

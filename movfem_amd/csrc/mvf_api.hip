@@ -620,7 +620,7 @@ int mvf_get_rhs(mvf_handle *h, double *b_out, int32_t ncols) { return fetch_cols
 
 int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32_t nrx, const double *rx,
                 const double *ry, double *esol, double *hsol, double *rx_e, double *rx_h, double *rx_z, double *rho,
-                double *phi) {
+                double *phi, double *rx_elev, double *node_z, double *node_rho, double *node_phi) {
   if (!h) return MVF_E_ARG;
   MVF_HIP(h, hipSetDevice(h->device));
   const Geom &g = h->geo;
@@ -678,7 +678,7 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
   if (!rc) rc = node_fields(h, x ? dx : h->d_x, ncol, col0, omega, es, hs);
   if (!rc && nrx > 0) {
     double *drx = nullptr, *dry = nullptr, *doe = nullptr, *doh = nullptr, *doz = nullptr, *drho = nullptr,
-           *dphi = nullptr;
+           *dphi = nullptr, *delev = nullptr;
     int32_t *die = nullptr, *dje = nullptr;
     rc = dalloc(h, &drx, nrx);
     if (!rc) rc = dalloc(h, &dry, nrx);
@@ -689,6 +689,7 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
     if (!rc) rc = dalloc(h, &doz, (size_t)nrx * 8);
     if (!rc) rc = dalloc(h, &drho, (size_t)nrx * 4);
     if (!rc) rc = dalloc(h, &dphi, (size_t)nrx * 4);
+    if (!rc) rc = dalloc(h, &delev, (size_t)nrx);
     if (!rc) {
       hipError_t e = hipMemcpy(drx, rx, 8 * nrx, hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMemcpy(dry, ry, 8 * nrx, hipMemcpyHostToDevice);
@@ -696,7 +697,7 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
       if (e == hipSuccess) e = hipMemcpy(dje, rje.data(), 4 * nrx, hipMemcpyHostToDevice);
       if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
     }
-    if (!rc) rc = receivers(h, es, hs, ke_top, nrx, die, dje, drx, dry, omega, doe, doh, doz, drho, dphi);
+    if (!rc) rc = receivers(h, es, hs, ke_top, nrx, die, dje, drx, dry, omega, doe, doh, doz, drho, dphi, delev);
     if (!rc) {
       hipError_t e = hipStreamSynchronize(h->stream);
       if (e == hipSuccess && rx_e) e = hipMemcpy(rx_e, doe, 8 * (size_t)nrx * 12, hipMemcpyDeviceToHost);
@@ -704,9 +705,26 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
       if (e == hipSuccess && rx_z) e = hipMemcpy(rx_z, doz, 8 * (size_t)nrx * 8, hipMemcpyDeviceToHost);
       if (e == hipSuccess && rho) e = hipMemcpy(rho, drho, 8 * (size_t)nrx * 4, hipMemcpyDeviceToHost);
       if (e == hipSuccess && phi) e = hipMemcpy(phi, dphi, 8 * (size_t)nrx * 4, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && rx_elev) e = hipMemcpy(rx_elev, delev, 8 * (size_t)nrx, hipMemcpyDeviceToHost);
       if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
     }
     dfree(drx); dfree(dry); dfree(die); dfree(dje); dfree(doe); dfree(doh); dfree(doz); dfree(drho); dfree(dphi);
+    dfree(delev);
+  }
+  if (!rc && (node_z || node_rho || node_phi)) {
+    double *dz = nullptr, *drh = nullptr, *dph = nullptr;
+    rc = dalloc(h, &dz, (size_t)g.npt * 8);
+    if (!rc) rc = dalloc(h, &drh, (size_t)g.npt * 4);
+    if (!rc) rc = dalloc(h, &dph, (size_t)g.npt * 4);
+    if (!rc) rc = node_impedance(h, es, hs, omega, dz, drh, dph);
+    if (!rc) {
+      hipError_t e = hipStreamSynchronize(h->stream);
+      if (e == hipSuccess && node_z) e = hipMemcpy(node_z, dz, 8 * (size_t)g.npt * 8, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && node_rho) e = hipMemcpy(node_rho, drh, 8 * (size_t)g.npt * 4, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && node_phi) e = hipMemcpy(node_phi, dph, 8 * (size_t)g.npt * 4, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
+    }
+    dfree(dz); dfree(drh); dfree(dph);
   }
   if (!rc) {
     hipError_t e = hipStreamSynchronize(h->stream);

@@ -200,7 +200,9 @@ void direct_free(mvf_handle *h);
 int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega, double *es, double *hs);
 int receivers(mvf_handle *h, const double *es, const double *hs, int ke_top, int nrx, const int32_t *rie,
               const int32_t *rje, const double *rx, const double *ry, double omega, double *oe, double *oh, double *oz,
-              double *rho, double *phi);
+              double *rho, double *phi, double *elev);
+int node_impedance(mvf_handle *h, const double *es, const double *hs, double omega, double *oz, double *rho,
+                   double *phi);
 void direct_invalidate(mvf_handle *h);
 int bd_setup(mvf_handle *h, const double *dKe, const double *dMe);
 int bd_values(mvf_handle *h, double omega);

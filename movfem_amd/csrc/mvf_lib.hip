@@ -6,3 +6,4 @@
 #include "mvf_direct.hip"
 #include "mvf_readout.hip"
 #include "mvf_api.hip"
+#include "mvf_output.hip"

@@ -121,7 +121,9 @@ __global__ __launch_bounds__(256) void k_node_h(const BasisTab *ntab, Geom g, co
   double tr[3], ti[3];
   sym6_mul(m1, cr, tr);
   sym6_mul(m1, ci, ti);
-  for (int c = 0; c < 3; c++) hsol[(d * g.npt + id) * 3 + c] = make_double2(-mi * ti[c], mi * tr[c]);
+  // the full complex product (0, mi)(tr, ti), as the reference forms it (signed zeros included)
+  for (int c = 0; c < 3; c++)
+    hsol[(d * g.npt + id) * 3 + c] = make_double2(0.0 * tr[c] - mi * ti[c], 0.0 * ti[c] + mi * tr[c]);
 }
 
 __device__ inline double2 zmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
@@ -130,14 +132,52 @@ __device__ inline double2 zdiv(double2 a, double2 b) {
   return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 
+// Z = E H^-1 over the two polarizations (get_impedance, inv_matrix,
+// solution.f90:456-505) from e, h = (x pol1, x pol2, y pol1, y pol2), and
+// rho_a = |Z|^2 / (omega mu0), phase atan(Im/Re) in degrees, both 0 when
+// rho_a < 1e-2 (get_res_phase)
+__device__ inline void mt_impedance(const double2 *e, const double2 *hh, double omega, double2 *z, double *rho,
+                                    double *phi) {
+  const double2 m01 = zmul(hh[0], hh[3]), m12 = zmul(hh[1], hh[2]);
+  const double2 det = make_double2(m01.x - m12.x, m01.y - m12.y);
+  const double2 i0 = zdiv(hh[3], det), i1 = zdiv(make_double2(-hh[1].x, -hh[1].y), det);
+  const double2 i2 = zdiv(make_double2(-hh[2].x, -hh[2].y), det), i3 = zdiv(hh[0], det);
+  const double2 a0 = zmul(i0, e[0]), a1 = zmul(i2, e[1]), b0 = zmul(i1, e[0]), b1 = zmul(i3, e[1]);
+  const double2 c0 = zmul(i0, e[2]), c1 = zmul(i2, e[3]), q0 = zmul(i1, e[2]), q1 = zmul(i3, e[3]);
+  z[0] = make_double2(a0.x + a1.x, a0.y + a1.y);
+  z[1] = make_double2(b0.x + b1.x, b0.y + b1.y);
+  z[2] = make_double2(c0.x + c1.x, c0.y + c1.y);
+  z[3] = make_double2(q0.x + q1.x, q0.y + q1.y);
+  const double mu0 = 4 * MVF_PI * 1.e-7;
+  for (int i = 0; i < 4; i++) {
+    double r = (1.0 / (omega * mu0)) * (z[i].x * z[i].x + z[i].y * z[i].y);
+    double ph = 0.0;
+    if (r < 1e-2) r = 0.0;
+    else ph = atan(z[i].y / z[i].x) * (180.0 / MVF_PI);
+    rho[i] = r;
+    phi[i] = ph;
+  }
+}
+
+// Z, rho_a, phase at every lattice node from its nodal fields (z_rho_phi,
+// hor_fields, solution.f90:51-69,440-454)
+__global__ void k_node_z(int64_t npt, const double2 *esol, const double2 *hsol, double omega, double2 *oz,
+                         double *rho, double *phi) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= npt) return;
+  const double2 e[4] = {esol[id * 3 + 0], esol[(npt + id) * 3 + 0], esol[id * 3 + 1], esol[(npt + id) * 3 + 1]};
+  const double2 hh[4] = {hsol[id * 3 + 0], hsol[(npt + id) * 3 + 0], hsol[id * 3 + 1], hsol[(npt + id) * 3 + 1]};
+  mt_impedance(e, hh, omega, oz + id * 4, rho + id * 4, phi + id * 4);
+}
+
 // receivers: bilinear interpolation of the nodal fields on the top face of
 // element (ie, je, ke_top) (interpolate_rx), Z = E H^-1 over the two
 // polarizations (get_impedance), rho_a = |Z|^2 / (omega mu0) and phase
 // atan(Im/Re) in degrees, both 0 when rho_a < 1e-2 (get_res_phase)
 __global__ void k_receivers(Geom g, int nrx, const int32_t *rie, const int32_t *rje, int ke, const double *rx,
-                            const double *ry, const double *xp, const double *yp, const double2 *esol,
-                            const double2 *hsol, double omega, double2 *oe, double2 *oh, double2 *oz, double *rho,
-                            double *phi) {
+                            const double *ry, const double *xp, const double *yp, const double *zp,
+                            const double2 *esol, const double2 *hsol, double omega, double2 *oe, double2 *oh,
+                            double2 *oz, double *rho, double *phi, double *elev) {
   const int ir = blockIdx.x * blockDim.x + threadIdx.x;
   if (ir >= nrx) return;
   const int o = g.nord - 1, ie = rie[ir], je = rje[ir];
@@ -149,6 +189,9 @@ __global__ void k_receivers(Geom g, int nrx, const int32_t *rie, const int32_t *
   const double x = rx[ir], y = ry[ir];
   const double ax = (x2 - x) / (x2 - x1), bx = (x - x1) / (x2 - x1);
   const double ay = (y2 - y) / (y2 - y1), by = (y - y1) / (y2 - y1);
+  // the receiver's height on the element's top face (interpolate_rx, receiver_data.f90:222-225)
+  const double zx1 = ax * zp[id11] + bx * zp[id21], zx2 = ax * zp[id12] + bx * zp[id22];
+  elev[ir] = ay * zx1 + by * zx2;
   for (int t = 0; t < 2; t++) {
     const double2 *F = t == 0 ? esol : hsol;
     double2 *O = t == 0 ? oe : oh;
@@ -162,31 +205,11 @@ __global__ void k_receivers(Geom g, int nrx, const int32_t *rie, const int32_t *
       }
   }
   // assign_rxmatrix order: (x pol1, x pol2, y pol1, y pol2)
-  const double2 e0 = oe[(ir * 2 + 0) * 3 + 0], e1 = oe[(ir * 2 + 1) * 3 + 0];
-  const double2 e2 = oe[(ir * 2 + 0) * 3 + 1], e3 = oe[(ir * 2 + 1) * 3 + 1];
-  const double2 h0 = oh[(ir * 2 + 0) * 3 + 0], h1 = oh[(ir * 2 + 1) * 3 + 0];
-  const double2 h2 = oh[(ir * 2 + 0) * 3 + 1], h3 = oh[(ir * 2 + 1) * 3 + 1];
-  const double2 m01 = zmul(h0, h3), m12 = zmul(h1, h2);
-  const double2 det = make_double2(m01.x - m12.x, m01.y - m12.y);
-  const double2 i0 = zdiv(h3, det), i1 = zdiv(make_double2(-h1.x, -h1.y), det);
-  const double2 i2 = zdiv(make_double2(-h2.x, -h2.y), det), i3 = zdiv(h0, det);
-  double2 z[4];
-  const double2 a0 = zmul(i0, e0), a1 = zmul(i2, e1), b0 = zmul(i1, e0), b1 = zmul(i3, e1);
-  const double2 c0 = zmul(i0, e2), c1 = zmul(i2, e3), q0 = zmul(i1, e2), q1 = zmul(i3, e3);
-  z[0] = make_double2(a0.x + a1.x, a0.y + a1.y);
-  z[1] = make_double2(b0.x + b1.x, b0.y + b1.y);
-  z[2] = make_double2(c0.x + c1.x, c0.y + c1.y);
-  z[3] = make_double2(q0.x + q1.x, q0.y + q1.y);
-  const double mu0 = 4 * MVF_PI * 1.e-7;
-  for (int i = 0; i < 4; i++) {
-    oz[ir * 4 + i] = z[i];
-    double r = (1.0 / (omega * mu0)) * (z[i].x * z[i].x + z[i].y * z[i].y);
-    double ph = 0.0;
-    if (r < 1e-2) r = 0.0;
-    else ph = atan(z[i].y / z[i].x) * (180.0 / MVF_PI);
-    rho[ir * 4 + i] = r;
-    phi[ir * 4 + i] = ph;
-  }
+  const double2 e[4] = {oe[(ir * 2 + 0) * 3 + 0], oe[(ir * 2 + 1) * 3 + 0], oe[(ir * 2 + 0) * 3 + 1],
+                        oe[(ir * 2 + 1) * 3 + 1]};
+  const double2 hh[4] = {oh[(ir * 2 + 0) * 3 + 0], oh[(ir * 2 + 1) * 3 + 0], oh[(ir * 2 + 0) * 3 + 1],
+                         oh[(ir * 2 + 1) * 3 + 1]};
+  mt_impedance(e, hh, omega, oz + ir * 4, rho + ir * 4, phi + ir * 4);
 }
 
 template <int NN, int NE>
@@ -240,10 +263,20 @@ int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega
 
 int receivers(mvf_handle *h, const double *es, const double *hs, int ke_top, int nrx, const int32_t *rie,
               const int32_t *rje, const double *rx, const double *ry, double omega, double *oe, double *oh, double *oz,
-              double *rho, double *phi) {
+              double *rho, double *phi, double *elev) {
   hipLaunchKernelGGL(k_receivers, dim3(grid1d(nrx, 64)), dim3(64), 0, h->stream, h->geo, nrx, rie, rje, ke_top, rx, ry,
-                     h->d_xp, h->d_yp, (const double2 *)es, (const double2 *)hs, omega, (double2 *)oe, (double2 *)oh,
-                     (double2 *)oz, rho, phi);
+                     h->d_xp, h->d_yp, h->d_zp, (const double2 *)es, (const double2 *)hs, omega, (double2 *)oe,
+                     (double2 *)oh, (double2 *)oz, rho, phi, elev);
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+// nodal Z, rho_a, phase from device nodal fields (2*npt*3 complex each)
+int node_impedance(mvf_handle *h, const double *es, const double *hs, double omega, double *oz, double *rho,
+                   double *phi) {
+  const int64_t npt = h->geo.npt;
+  hipLaunchKernelGGL(k_node_z, dim3(grid1d(npt, 256)), dim3(256), 0, h->stream, npt, (const double2 *)es,
+                     (const double2 *)hs, omega, (double2 *)oz, rho, phi);
   MVF_HIP(h, hipGetLastError());
   return 0;
 }

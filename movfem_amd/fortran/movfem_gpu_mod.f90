@@ -118,6 +118,43 @@ module movfem_gpu
       type(mvf_solver_opts), intent(in) :: o
       type(mvf_solve_info), intent(out) :: info
     end function mvf_solve
+
+    ! node_solution + rx_data + z_rho_phi on the device; pass c_null_ptr for
+    ! outputs not wanted and for x to read frequency f of the last batch
+    integer(c_int) function mvf_readout(h, x, f, ke_top, nrx, rx, ry, esol, hsol, rx_e, rx_h, rx_z, rho, phi, &
+         rx_elev, node_z, node_rho, node_phi) bind(C, name='mvf_readout')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h, x, rx, ry, esol, hsol, rx_e, rx_h, rx_z, rho, phi, rx_elev, node_z, node_rho, node_phi
+      integer(c_int32_t), value :: f, ke_top, nrx
+    end function mvf_readout
+
+    ! the reference's output files (write_solution, write_receiver_data,
+    ! write_surface_data); path is a null-terminated string
+    integer(c_int) function mvf_write_solution(path, nf, append, npt, esol, hsol, node_z, node_rho, node_phi, &
+         nsel, sel) bind(C, name='mvf_write_solution')
+      import :: c_int, c_ptr, c_char, c_int32_t, c_int64_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nf, append
+      integer(c_int64_t), value :: npt, nsel
+      type(c_ptr), value :: esol, hsol, node_z, node_rho, node_phi, sel
+    end function mvf_write_solution
+
+    integer(c_int) function mvf_write_receivers(path, nf, append, nfreq, nrx, rx, ry, elev, rx_e, rx_h, rx_z, &
+         rho, phi) bind(C, name='mvf_write_receivers')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nf, append, nfreq, nrx
+      type(c_ptr), value :: rx, ry, elev, rx_e, rx_h, rx_z, rho, phi
+    end function mvf_write_receivers
+
+    integer(c_int) function mvf_write_surface(path, nf, append, nfreq, nxb, nyb, xs, ys, zs, ids, npt, esol, &
+         hsol, node_z, node_rho, node_phi) bind(C, name='mvf_write_surface')
+      import :: c_int, c_ptr, c_char, c_int32_t, c_int64_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nf, append, nfreq, nxb, nyb
+      integer(c_int64_t), value :: npt
+      type(c_ptr), value :: xs, ys, zs, ids, esol, hsol, node_z, node_rho, node_phi
+    end function mvf_write_surface
   end interface
 
 contains

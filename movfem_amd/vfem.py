@@ -209,11 +209,13 @@ class GpuVfem:
         self._check(self._lib.mvf_get_rhs(self.h, _dp(x.view(np.float64)), ncols))
         return x.reshape(ncols, self.nne)
 
-    def readout(self, x=None, f=0, ke_top=0, rx=(), ry=()):
+    def readout(self, x=None, f=0, ke_top=0, rx=(), ry=(), full=False):
         """node_solution + rx_data on the device.  x: host solution (2*nne,
         polarization-major) at the current frequency, or None for frequency f
         of the last solve_frequencies.  Returns esol, hsol (2, npt, 3) and, with
-        receivers, (E, H (nrx, 2, 3), Z (nrx, 4), rho, phase (nrx, 4))."""
+        receivers, (E, H (nrx, 2, 3), Z (nrx, 4), rho, phase (nrx, 4)).  full=True
+        returns a dict that adds the receivers' heights ("rx_elev") and Z, rho_a,
+        phase at every lattice node ("node_z" (npt, 4), "node_rho", "node_phi")."""
         npt = self.m.npt
         es = np.empty(2 * npt * 3, np.complex128)
         hs = np.empty(2 * npt * 3, np.complex128)
@@ -226,13 +228,87 @@ class GpuVfem:
         rho = np.empty(max(n, 1) * 4)
         phi = np.empty(max(n, 1) * 4)
         xv = None if x is None else np.ascontiguousarray(x, np.complex128)
+        elev = np.empty(max(n, 1))
+        nz = np.empty(npt * 4, np.complex128) if full else None
+        nrho = np.empty(npt * 4) if full else None
+        nphi = np.empty(npt * 4) if full else None
         self._check(self._lib.mvf_readout(self.h, None if xv is None else _dp(xv.view(np.float64)), f, ke_top, n,
                                           _dp(rx) if n else None, _dp(ry) if n else None,
                                           _dp(es.view(np.float64)), _dp(hs.view(np.float64)),
                                           _dp(oe.view(np.float64)), _dp(oh.view(np.float64)),
-                                          _dp(oz.view(np.float64)), _dp(rho), _dp(phi)))
+                                          _dp(oz.view(np.float64)), _dp(rho), _dp(phi), _dp(elev),
+                                          _dp(nz.view(np.float64)) if full else None,
+                                          _dp(nrho) if full else None, _dp(nphi) if full else None))
+        if full:
+            d = {"esol": es.reshape(2, npt, 3), "hsol": hs.reshape(2, npt, 3), "node_z": nz.reshape(npt, 4),
+                 "node_rho": nrho.reshape(npt, 4), "node_phi": nphi.reshape(npt, 4)}
+            if n:
+                d.update(rx_e=oe[:6 * n].reshape(n, 2, 3), rx_h=oh[:6 * n].reshape(n, 2, 3), rx_z=oz[:4 * n].reshape(n, 4),
+                         rx_rho=rho[:4 * n].reshape(n, 4), rx_phi=phi[:4 * n].reshape(n, 4), rx_elev=elev[:n].copy())
+            return d
         out = (es.reshape(2, npt, 3), hs.reshape(2, npt, 3))
         if n:
             out += (oe[:6 * n].reshape(n, 2, 3), oh[:6 * n].reshape(n, 2, 3), oz[:4 * n].reshape(n, 4),
                     rho[:4 * n].reshape(n, 4), phi[:4 * n].reshape(n, 4))
         return out
+
+
+# ---------------------------------------------------------------- text output
+def surface_nodes(m: MTModel, nzl_top: int, nzl_next: int):
+    """The surface points SURFACE.OUT lists (sur_data, receiver_data.f90:145-175):
+    lattice nodes i, j inside the nextd extension cells on the lattice layer
+    nnz - (nzl_top + nzl_next)*(nord-1), y outer and x inner.  Returns nxb, nyb,
+    xs (nxb), ys (nyb), zs (nsp), 0-based node ids (nsp)."""
+    o = m.nord - 1
+    i1, i2 = m.nextd * o, m.nnx - m.nextd * o - 1
+    j1, j2 = m.nextd * o, m.nny - m.nextd * o - 1
+    k = m.nnz - nzl_top * o - nzl_next * o - 1
+    if not (0 <= i1 <= i2 and 0 <= j1 <= j2 and 0 <= k < m.nnz):
+        raise ValueError("surface selection does not fit the lattice")
+    ii, jj = np.meshgrid(np.arange(i1, i2 + 1), np.arange(j1, j2 + 1), indexing="xy")
+    ids = (ii.ravel() * m.nny + jj.ravel()) * m.nnz + k
+    return (i2 - i1 + 1, j2 - j1 + 1, np.ascontiguousarray(m.xp[i1:i2 + 1]), np.ascontiguousarray(m.yp[j1:j2 + 1]),
+            np.ascontiguousarray(m.zp[ids]), np.ascontiguousarray(ids, np.int64))
+
+
+def _wcheck(rc, what):
+    if rc:
+        raise MvfError(rc, f"{what} failed")
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dt)
+
+
+def write_solution(path, nf, esol, hsol, node_z, node_rho, node_phi, sel=None, append=False):
+    """SOLUTION.OUT block of frequency nf (write_solution, solution.f90:73-205)."""
+    es, hs, nz = _c(esol, np.complex128), _c(hsol, np.complex128), _c(node_z, np.complex128)
+    nr, npf = _c(node_rho), _c(node_phi)
+    npt = es.size // 6
+    s = None if sel is None else _c(sel, np.int32)
+    _wcheck(L.lib().mvf_write_solution(str(path).encode(), nf, int(append), npt, _dp(es.view(np.float64)),
+                                       _dp(hs.view(np.float64)), _dp(nz.view(np.float64)), _dp(nr), _dp(npf),
+                                       0 if s is None else len(s), None if s is None else _ip(s)),
+            "mvf_write_solution")
+
+
+def write_receivers(path, nf, nfreq, rx, ry, elev, rx_e, rx_h, rx_z, rho, phi, append=False):
+    """RECEIVER.OUT block of frequency nf (write_receiver_data, receiver_data.f90:25-67)."""
+    rx, ry, el = _c(rx), _c(ry), _c(elev)
+    e, h, z = _c(rx_e, np.complex128), _c(rx_h, np.complex128), _c(rx_z, np.complex128)
+    r, p = _c(rho), _c(phi)
+    _wcheck(L.lib().mvf_write_receivers(str(path).encode(), nf, int(append), nfreq, len(rx), _dp(rx), _dp(ry),
+                                        _dp(el), _dp(e.view(np.float64)), _dp(h.view(np.float64)),
+                                        _dp(z.view(np.float64)), _dp(r), _dp(p)), "mvf_write_receivers")
+
+
+def write_surface(path, nf, nfreq, surface, esol, hsol, node_z, node_rho, node_phi, append=False):
+    """SURFACE.OUT block of frequency nf (write_surface_data, receiver_data.f90:69-109);
+    surface = surface_nodes(...)."""
+    nxb, nyb, xs, ys, zs, ids = surface
+    es, hs, nz = _c(esol, np.complex128), _c(hsol, np.complex128), _c(node_z, np.complex128)
+    nr, npf = _c(node_rho), _c(node_phi)
+    _wcheck(L.lib().mvf_write_surface(str(path).encode(), nf, int(append), nfreq, nxb, nyb, _dp(_c(xs)), _dp(_c(ys)),
+                                      _dp(_c(zs)), _c(ids, np.int64).ctypes.data_as(C.POINTER(C.c_int64)),
+                                      es.size // 6, _dp(es.view(np.float64)), _dp(hs.view(np.float64)),
+                                      _dp(nz.view(np.float64)), _dp(nr), _dp(npf)), "mvf_write_surface")

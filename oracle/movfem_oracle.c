@@ -1309,15 +1309,17 @@ void orc_impedance(const double *e_, const double *h_, double omega, double *z_,
     e[i] = CMPLX(e_[2 * i], e_[2 * i + 1]);
     h[i] = CMPLX(h_[2 * i], h_[2 * i + 1]);
   }
-  zc det = h[0] * h[3] - h[1] * h[2];
-  ih[0] = h[3] / det;
-  ih[1] = -h[1] / det;
-  ih[2] = -h[2] / det;
-  ih[3] = h[0] / det;
-  z[0] = ih[0] * e[0] + ih[2] * e[1];
-  z[1] = ih[1] * e[0] + ih[3] * e[1];
-  z[2] = ih[0] * e[2] + ih[2] * e[3];
-  z[3] = ih[1] * e[2] + ih[3] * e[3];
+  /* inv_matrix, get_impedance (solution.f90:456-469,493-505) in the
+   * reference build's complex arithmetic */
+  zc det = zsub(zmul(h[0], h[3]), zmul(h[1], h[2]));
+  ih[0] = zdiv(h[3], det);
+  ih[1] = zneg(zdiv(h[1], det));
+  ih[2] = zneg(zdiv(h[2], det));
+  ih[3] = zdiv(h[0], det);
+  z[0] = zadd(zmul(ih[0], e[0]), zmul(ih[2], e[1]));
+  z[1] = zadd(zmul(ih[1], e[0]), zmul(ih[3], e[1]));
+  z[2] = zadd(zmul(ih[0], e[2]), zmul(ih[2], e[3]));
+  z[3] = zadd(zmul(ih[1], e[2]), zmul(ih[3], e[3]));
   double mu0 = 4 * ORC_PI * 1.e-7;
   for (int i = 0; i < 4; i++) {
     z_[2 * i] = creal(z[i]);

@@ -105,6 +105,7 @@ class Oracle:
         if L.orc_set_bdmodel(self.h, float(m.ztop), float(m.h_sigma), len(ls), _dp(ls), _dp(ld)) != 0:
             raise ValueError("oracle rejected the boundary model")
         L.orc_set_omega(self.h, m.omega(0))
+        self.omega = m.omega(0)
         self.me = m.edges_per_elem
         self._nne = None
         self._nnze = None
@@ -117,6 +118,7 @@ class Oracle:
             pass
 
     def set_frequency(self, i: int):
+        self.omega = self.m.omega(i)
         L = lib()
         L.orc_set_omega(self.h, self.m.omega(i))
         L.orc_update_sigma(self.h)
@@ -226,6 +228,27 @@ class Oracle:
             raise RuntimeError(f"orc_receivers failed ({rc})")
         return (oe.view(np.complex128).reshape(n, 2, 3), oh.view(np.complex128).reshape(n, 2, 3),
                 oz.view(np.complex128).reshape(n, 4), rho.reshape(n, 4), phi.reshape(n, 4))
+
+    @staticmethod
+    def impedance(e, h, omega):
+        """Z (4), rho_a (4), phase (4) from e, h = (x pol1, x pol2, y pol1, y pol2)."""
+        ev = np.ascontiguousarray(e, np.complex128).view(np.float64)
+        hv = np.ascontiguousarray(h, np.complex128).view(np.float64)
+        z, rho, phi = np.empty(8), np.empty(4), np.empty(4)
+        lib().orc_impedance(_dp(ev), _dp(hv), float(omega), _dp(z), _dp(rho), _dp(phi))
+        return z.view(np.complex128), rho, phi
+
+    def node_impedance(self, esol, hsol):
+        """z_rho_phi (solution.f90:51-69): Z (npt, 4), rho_a, phase (npt, 4) at every
+        lattice node at the current frequency."""
+        npt = self.m.npt
+        z, rho, phi = np.empty((npt, 4), np.complex128), np.empty((npt, 4)), np.empty((npt, 4))
+        es, hs = np.asarray(esol), np.asarray(hsol)
+        for i in range(npt):
+            e = np.array([es[0, i, 0], es[1, i, 0], es[0, i, 1], es[1, i, 1]])
+            h = np.array([hs[0, i, 0], hs[1, i, 0], hs[0, i, 1], hs[1, i, 1]])
+            z[i], rho[i], phi[i] = self.impedance(e, h, self.omega)
+        return z, rho, phi
 
     @staticmethod
     def remove_zeros(ia, ja, a):

@@ -23,8 +23,8 @@ module mvf_ref_harness
   use integration, only: init_integration, int_elem_params, alocal, blocal
   use global_assembly, only: ga_init, ga_assemble_nze, ga_sort_sparse, assign_aij, assign_bi, &
        gne, nzindx, nne, nnze, sym
-  use solution, only: node_solution, esol, hsol
-  use receiver_data, only: write_receiver_data
+  use solution, only: node_solution, esol, hsol, write_solution
+  use receiver_data, only: write_receiver_data, write_surface_data
   implicit none
   integer, save :: h_me = 0
 contains
@@ -224,4 +224,25 @@ contains
     g_nf = 1
     call write_receiver_data(1)
   end subroutine ref_receivers
+
+  ! write_solution(1, inner) (solution.f90:73-205): SOLUTION.OUT in the
+  ! working directory.  Must follow ref_node_solution; frees z, rho, phi.
+  subroutine ref_write_solution(inner) bind(C, name='ref_write_solution')
+    integer(c_int), value :: inner
+    g_nf = 1
+    call write_solution(1, inner)
+  end subroutine ref_write_solution
+
+  ! write_surface_data(1) (receiver_data.f90:69-109): SURFACE.OUT for the
+  ! earth-surface lattice nodes inside the extension zones; nzl_a/nzl_b as
+  ! for ref_receivers.  Must come last: it frees esol and hsol.
+  subroutine ref_write_surface(nzl_a, nzl_b) bind(C, name='ref_write_surface')
+    use geometry, only: x0, y0, z0
+    integer(c_int), value :: nzl_a, nzl_b
+    x0 = 0.d0; y0 = 0.d0; z0 = 0.d0
+    g_nzl(g_nsf) = nzl_a
+    g_nzl(g_nsf - 1) = nzl_b
+    g_nf = 1
+    call write_surface_data(1)
+  end subroutine ref_write_surface
 end module mvf_ref_harness

@@ -22,7 +22,8 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 
 
-def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=None, receivers=None):
+def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=None, receivers=None,
+               write_files=False):
     rx = np.zeros((0, 2)) if receivers is None else np.asarray(receivers["xy"], np.float64)
     rz = np.zeros(2, np.int32) if receivers is None else np.asarray(receivers["nzl"], np.int32)
     np.savez(path, nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp,
@@ -32,7 +33,7 @@ def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=No
              l_dz=np.asarray(m.l_dz, np.float64), elements=np.asarray(elements, np.int32).reshape(-1, 3),
              vfem_freqs=np.asarray(vfem_freqs, np.int32), sigma_after=np.asarray(sigma_after, np.int32),
              readout_x=np.zeros(0, complex) if readout_x is None else np.asarray(readout_x, np.complex128),
-             rx_xy=rx, rx_nzl=rz)
+             rx_xy=rx, rx_nzl=rz, write_files=int(write_files))
 
 
 def parse_receiver_out(path, nsr):
@@ -58,13 +59,16 @@ def parse_receiver_out(path, nsr):
     return {"rx_e": e, "rx_h": hh, "rx_z": z, "rx_rho": rp[:, 2:6], "rx_phi": rp[:, 6:10]}
 
 
-def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, readout_x=None, receivers=None):
-    """Returns a dict of reference outputs for model m (fresh subprocess)."""
+def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, readout_x=None, receivers=None,
+                  write_files=False):
+    """Returns a dict of reference outputs for model m (fresh subprocess).  write_files:
+    also run the reference's SOLUTION.OUT and SURFACE.OUT writers after node_solution;
+    their text is returned under "files" with RECEIVER.OUT's."""
     import os
 
     with tempfile.TemporaryDirectory() as td:
         fin, fout = pathlib.Path(td) / "in.npz", pathlib.Path(td) / "out.npz"
-        dump_model(m, fin, elements, vfem_freqs, sigma_after, readout_x, receivers)
+        dump_model(m, fin, elements, vfem_freqs, sigma_after, readout_x, receivers, write_files)
         env = dict(os.environ)
         env["PYTHONPATH"] = str(HERE.parent) + os.pathsep + env.get("PYTHONPATH", "")
         r = subprocess.run([sys.executable, "-m", "oracle.refrun", str(fin), str(fout)],
@@ -75,6 +79,9 @@ def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, re
             out = {k: z[k] for k in z.files}
         if receivers is not None:
             out.update(parse_receiver_out(pathlib.Path(td) / "RECEIVER.OUT", len(receivers["xy"])))
+        if write_files:
+            out["files"] = {n: (pathlib.Path(td) / n).read_text()
+                            for n in ("SOLUTION.OUT", "RECEIVER.OUT", "SURFACE.OUT") if (pathlib.Path(td) / n).exists()}
         return out
 
 
@@ -94,6 +101,8 @@ def _main(fin, fout):
     L.ref_global_vfem.argtypes = [ip, ip, dp, dp]
     L.ref_node_solution.argtypes = [dp, dp, dp]
     L.ref_receivers.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int]
+    L.ref_write_solution.argtypes = [C.c_int]
+    L.ref_write_surface.argtypes = [C.c_int, C.c_int]
 
     def d(a):
         return a.ctypes.data_as(dp)
@@ -162,10 +171,14 @@ def _main(fin, fout):
         out["esol"] = eo.view(np.complex128).reshape(3, 2, npt).transpose(1, 2, 0)
         out["hsol"] = ho.view(np.complex128).reshape(3, 2, npt).transpose(1, 2, 0)
         rxy = z["rx_xy"]
+        if int(z["write_files"]):
+            L.ref_write_solution(0)
         if len(rxy):
             xs = np.ascontiguousarray(rxy[:, 0])
             ys = np.ascontiguousarray(rxy[:, 1])
             L.ref_receivers(len(xs), d(xs), d(ys), int(z["rx_nzl"][0]), int(z["rx_nzl"][1]))
+        if int(z["write_files"]):
+            L.ref_write_surface(int(z["rx_nzl"][0]), int(z["rx_nzl"][1]))
     np.savez(fout, **out)
 
 

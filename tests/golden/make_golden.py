@@ -51,7 +51,31 @@ def readout_vector(nne, seed=77):
     return rng.standard_normal(2 * nne) + 1j * rng.standard_normal(2 * nne)
 
 
+def write_outfiles_golden():
+    """out_hex8_small.npz: the reference's SOLUTION.OUT, RECEIVER.OUT and
+    SURFACE.OUT for the 8-node Dirichlet readout case, with the nodal fields
+    they were written from."""
+    from oracle.oracle import Oracle
+    from tests.test_oracle_readout import CASES, air_layers, receivers_for, solve_oracle
+
+    m = CASES["hex8_dirichlet"]()
+    o = Oracle(m)
+    o.set_frequency(0)
+    x = solve_oracle(o)
+    xs, ys = receivers_for(m)
+    nza = air_layers(m)
+    nzl_a, nzl_b = m.nzl_top, nza - m.nzl_top
+    r = run_reference(m, readout_x=x, receivers={"xy": np.stack([xs, ys], 1), "nzl": [nzl_a, nzl_b]},
+                      write_files=True)
+    files = {n.replace(".", "_"): np.frombuffer(r["files"][n].encode(), np.uint8) for n in r["files"]}
+    np.savez_compressed(HERE / "out_hex8_small.npz", esol=r["esol"], hsol=r["hsol"], zp=m.zp, nzl_a=nzl_a,
+                        nzl_b=nzl_b, **files)
+    print("out_hex8_small", {k: len(v) for k, v in files.items()})
+
+
 def main(names=()):
+    if not names or "out_hex8_small" in names:
+        write_outfiles_golden()
     for name, m in golden_cases().items():
         if names and name not in names:
             continue

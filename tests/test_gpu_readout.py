@@ -68,3 +68,40 @@ def test_readout_after_device_solve(need_gpu):
     ke_top = m.nz - air_layers(m)
     out = g.readout(None, f=0, ke_top=ke_top, rx=xs, ry=ys)
     compare(m, x, out[0], out[1], out[2:], xs, ys, ke_top)
+
+
+@pytest.mark.parametrize("name", ["hex8_dirichlet", "hex20_gpml_aniso"])
+def test_output_files_from_device(name, need_gpu):
+    """SOLUTION.OUT, RECEIVER.OUT and SURFACE.OUT from the device readout
+    (nodal Z, rho_a, phase and receiver heights included) against the same
+    files from the oracle's fields: equal to print precision."""
+    import pathlib
+    import tempfile
+
+    from movfem_amd import vfem
+    from tests.outfiles import compare_files
+    from tests.test_output_files import write_ours
+
+    m = CASES[name]()
+    print(f"output files {name}", flush=True)
+    g = GpuVfem(m)
+    g.set_frequency(0)
+    x = host_solution(g)
+    xs, ys = receivers_for(m)
+    nza = air_layers(m)
+    nzl_a, nzl_b = m.nzl_top, nza - m.nzl_top
+    ke_top = m.nz - nza
+    d = g.readout(x, ke_top=ke_top, rx=xs, ry=ys, full=True)
+    o = Oracle(m)
+    o.set_frequency(0)
+    oes, ohs = o.node_solution(x)
+    with tempfile.TemporaryDirectory() as td:
+        td = pathlib.Path(td)
+        ref = write_ours(m, oes, ohs, xs, ys, nzl_a, nzl_b, td)
+        vfem.write_solution(td / "S", 1, d["esol"], d["hsol"], d["node_z"], d["node_rho"], d["node_phi"])
+        vfem.write_receivers(td / "R", 1, 1, xs, ys, d["rx_elev"], d["rx_e"], d["rx_h"], d["rx_z"], d["rx_rho"],
+                             d["rx_phi"])
+        vfem.write_surface(td / "F", 1, 1, vfem.surface_nodes(m, nzl_a, nzl_b), d["esol"], d["hsol"], d["node_z"],
+                           d["node_rho"], d["node_phi"])
+        for ours, n in (("S", "SOLUTION.OUT"), ("R", "RECEIVER.OUT"), ("F", "SURFACE.OUT")):
+            compare_files((td / ours).read_text(), ref[n], rel_tol=2.5e-5, abs_tol=2.5e-5, phase_branch=True)

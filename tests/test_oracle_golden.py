@@ -9,7 +9,8 @@ import pytest
 from oracle.oracle import Oracle
 from tests.golden.make_golden import load_model
 
-GOLD = sorted((pathlib.Path(__file__).parent / "golden").glob("*.npz"))
+# model fixtures (out_*.npz hold output files, tests/test_output_files.py)
+GOLD = sorted(p for p in (pathlib.Path(__file__).parent / "golden").glob("*.npz") if not p.stem.startswith("out_"))
 
 
 def _rel(a, b):
