@@ -13,7 +13,8 @@
  *   RHS/solution  column-major (nne, ndir), polarization d at offset (d-1)*nne
  *                                                               (global_assembly.f90:70-74)
  * Errors: every entry point returns 0 or a negative MVF_E_* code and never
- * aborts the process; mvf_last_error(h) holds the message (the reference
+ * aborts the process; mvf_last_error(h) holds the message (NULL: the calling
+ * thread's last handle-less call, i.e. the file readers and mvf_grid3d; the reference
  * prints and stops: global_assembly.f90:55-57, n_fem.f90:374-377,
  * problem.f90:260-271).
  * Threading: one handle per host thread; calls on one handle are serialized
@@ -280,6 +281,86 @@ int mvf_write_surface(const char *path, int32_t nf, int32_t append, int32_t nfre
                       const double *xs, const double *ys, const double *zs, const int64_t *ids, int64_t npt,
                       const double *esol, const double *hsol, const double *node_z, const double *node_rho,
                       const double *node_phi);
+
+/* ---- input files and lattice generation (host only; no device use) --------
+ * Failures return MVF_E_ARG; mvf_last_error(NULL) holds the message of the
+ * calling thread's last failing call of this group.  Inputs on which the
+ * reference's own routines run into undefined values are rejected here: a
+ * Shepard fit that fails (fewer than 20 points on an interface, duplicate or
+ * collinear points; the reference ignores qshep2's ier, geometry.f90:458-459).
+ * For a domain-length / skin-depth ratio in [1.5, 2.5) the reference leaves
+ * its extension depth factor unset (geometry.f90:245-251); its build takes 3
+ * there, and so does this library. */
+
+/* grid_3d's arguments (geometry.f90:48-62) in the reference's array layouts,
+ * so that the arrays read by input() (read_input.f90:14-20) pass unchanged.
+ * Interfaces occupy slots 2..nsf-2 of nsf = (input interfaces) + 3; slots 1,
+ * nsf-1 and nsf (bottom, air and top) are generated. */
+typedef struct mvf_mesh_input {
+  double dx, dy, dz;      /* inner cell sizes (in_dx, in_dy, in_dz)                    */
+  double high, depth;     /* air above / earth below the interface points (in_high...) */
+  double sigma_bg;        /* conductivity for the skin depth (in_bcksigma)              */
+  int32_t mn;             /* nodes per element: 8, 20 or 27                             */
+  int32_t nsr;            /* source/receiver sites                                      */
+  int32_t nsf;            /* interface slots (in_nsf)                                   */
+  int32_t mx, my, mz;     /* inner model lattice                                        */
+  int32_t isigma, imu;    /* conductivity / permeability components; imu = 0: mu_r = 1 */
+  int32_t nf;             /* frequencies                                                */
+  int32_t pad_;
+  const double *xsr, *ysr, *zsr;   /* (nsr)                                             */
+  const int32_t *nsp;              /* (nsf) points per slot; slots 2..nsf-2 are read     */
+  const double *xto, *yto, *zto;   /* (nsf, *) column-major: point j of slot l at        */
+                                   /* (j-1)*nsf + (l-1)                                  */
+  const double *xm, *ym;           /* (mx), (my)                                        */
+  const double *zm;                /* (mx*my*mz): point (i,j,k) at (i-1)*my*mz+(j-1)*mz+k-1 */
+  const int32_t *ijsigma;          /* (isigma, 2) column-major tensor index pairs       */
+  const double *sigma;             /* (isigma, mx*my*mz) column-major                   */
+  const int32_t *ijmu;             /* (imu, 2) column-major, NULL when imu = 0          */
+  const double *mu;                /* (imu, mx*my*mz) column-major, NULL when imu = 0   */
+  const double *freqs;             /* (nf) Hz; freqs[0] gives sigma's i*omega*eps term  */
+} mvf_mesh_input;
+
+/* PARAM.INP (MoVFEM_3DMT.f90:302-334).  gpml_sch, a0, b0, nn are read only
+ * for boundary = 0 (GPML) and are 0 otherwise. */
+typedef struct mvf_params {
+  int32_t nf, boundary, inimod, nl, gpml_sch, pad_;
+  double h_sigma, a0, b0, nn;
+  const double *freqs;   /* (nf) Hz                    */
+  const double *l_sigma; /* (nl)                       */
+  const double *l_dz;    /* (nl-1)                     */
+} mvf_params;
+
+typedef struct mvf_mesh mvf_mesh;
+typedef struct mvf_inputs mvf_inputs;
+
+/* Replaces input() (read_input.f90:37-43) and the PARAM.INP reads of
+ * initialize_vfem (MoVFEM_3DMT.f90:302-334): either path may be NULL.
+ * Fortran list-directed rules: blanks, tabs or commas separate values, r*c
+ * repeats, a READ continues over records until its list is filled and
+ * leaves the rest of its last record unread, an empty list consumes one
+ * record; the conductivity/permeability names are read as (a5,i1,i1) and
+ * (a2,i1,i1). */
+int mvf_read_inputs(const char *inp_path, const char *param_path, mvf_inputs **out);
+/* Pointers into the handle (valid until mvf_inputs_free); nf and freqs of
+ * mesh come from PARAM.INP when it was read. */
+int mvf_inputs_view(const mvf_inputs *h, mvf_mesh_input *mesh, mvf_params *params);
+void mvf_inputs_free(mvf_inputs *h);
+
+/* Replaces grid_3d (geometry.f90:48-122; iout = 0): the node lattice, the
+ * model on it and the layer counts, equal to the reference's bit for bit. */
+int mvf_grid3d(const mvf_mesh_input *in, mvf_mesh **out);
+/* dims[12] = nx, ny, nz (elements: g_nx-1, g_ny-1, g_nz-1), nodes_per_elem,
+ * nnx, nny, nnz, npt, nextd, nsf, nsr, nodes per element edge (2 or 3);
+ * vals[6] = g_ztop, x0, y0, z0 (the shift applied to every coordinate),
+ * omega of freqs[0], skin depth at the lowest frequency.  Either may be NULL. */
+int mvf_mesh_sizes(const mvf_mesh *m, int64_t *dims, double *vals);
+/* Copies g_xp (nnx), g_yp (nny), g_zp (npt), g_sigma (complex (6, npt)),
+ * g_mu ((6, npt)), g_nzl (nsf) and the shifted site coordinates g_xsr,
+ * g_ysr, g_zsr (nsr); any pointer may be NULL.  The lattice, sigma and mu
+ * feed mvf_grid / mvf_model directly. */
+int mvf_mesh_arrays(const mvf_mesh *m, double *xp, double *yp, double *zp, double *sigma, double *mu, int32_t *nzl,
+                    double *xsr, double *ysr, double *zsr);
+void mvf_mesh_free(mvf_mesh *m);
 
 /* Host-only symbolic analysis of the sparse direct factorization for a
  * numbering (gne, ne x me, 1-based, <= 0 removed) and the full CSR pattern

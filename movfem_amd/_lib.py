@@ -28,7 +28,9 @@ ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR"
 EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne", "mvf_set_frequency",
            "mvf_integrate", "mvf_global_vfem", "mvf_remove_zeros", "mvf_default_solver_opts", "mvf_solve",
            "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
-           "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface")
+           "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface",
+           "mvf_read_inputs", "mvf_inputs_view", "mvf_inputs_free", "mvf_grid3d", "mvf_mesh_sizes",
+           "mvf_mesh_arrays", "mvf_mesh_free")
 
 
 class Grid(C.Structure):
@@ -67,6 +69,24 @@ class SolveInfo(C.Structure):
                 ("factorizations", C.c_int32), ("static_pivots", C.c_int32), ("precond_applies", C.c_int32),
                 ("pad2_", C.c_int32), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
                 ("gemm_launches", C.c_int64)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class MeshInput(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in ("dx", "dy", "dz", "high", "depth", "sigma_bg")]
+                + [(n, C.c_int32) for n in ("mn", "nsr", "nsf", "mx", "my", "mz", "isigma", "imu", "nf", "pad_")]
+                + [("xsr", _dp), ("ysr", _dp), ("zsr", _dp), ("nsp", _ip), ("xto", _dp), ("yto", _dp),
+                   ("zto", _dp), ("xm", _dp), ("ym", _dp), ("zm", _dp), ("ijsigma", _ip), ("sigma", _dp),
+                   ("ijmu", _ip), ("mu", _dp), ("freqs", _dp)])
+
+
+class Params(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("nf", "boundary", "inimod", "nl", "gpml_sch", "pad_")]
+                + [(n, C.c_double) for n in ("h_sigma", "a0", "b0", "nn")]
+                + [("freqs", _dp), ("l_sigma", _dp), ("l_dz", _dp)])
 
 
 _L = None
@@ -108,8 +128,17 @@ def lib():
     L.mvf_write_solution.argtypes = [cp, C.c_int32, C.c_int32, C.c_int64] + [dp] * 5 + [C.c_int64, ip]
     L.mvf_write_receivers.argtypes = [cp] + [C.c_int32] * 4 + [dp] * 8
     L.mvf_write_surface.argtypes = [cp] + [C.c_int32] * 5 + [dp] * 3 + [i64p, C.c_int64] + [dp] * 5
+    L.mvf_read_inputs.argtypes = [cp, cp, C.POINTER(vp)]
+    L.mvf_inputs_view.argtypes = [vp, C.POINTER(MeshInput), C.POINTER(Params)]
+    L.mvf_inputs_free.restype = None
+    L.mvf_inputs_free.argtypes = [vp]
+    L.mvf_grid3d.argtypes = [C.POINTER(MeshInput), C.POINTER(vp)]
+    L.mvf_mesh_sizes.argtypes = [vp, i64p, dp]
+    L.mvf_mesh_arrays.argtypes = [vp] + [dp] * 5 + [ip] + [dp] * 3
+    L.mvf_mesh_free.restype = None
+    L.mvf_mesh_free.argtypes = [vp]
     for name in EXPORTS:
-        if name not in ("mvf_free", "mvf_last_error", "mvf_default_solver_opts"):
+        if name not in ("mvf_free", "mvf_last_error", "mvf_default_solver_opts", "mvf_inputs_free", "mvf_mesh_free"):
             getattr(L, name).restype = C.c_int
     _L = L
     return L

@@ -52,22 +52,24 @@ def build(verbose: bool = False) -> pathlib.Path:
 
 FDIR = PKG / "fortran"
 FDRIVER = FDIR / "movfem_driver"
+FRUNCASE = FDIR / "run_case"
 
 
 def build_fortran(verbose: bool = False) -> pathlib.Path | None:
-    """The ISO_C_BINDING module and the Fortran driver (needs amdflang)."""
+    """The ISO_C_BINDING module, the Fortran driver and run_case (needs amdflang)."""
     fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
     if not fc:
         return None
     mod = FDIR / "_mod"
     mod.mkdir(exist_ok=True)
-    cmd = [fc, "-O2", "-module-dir", str(mod), str(FDIR / "movfem_gpu_mod.f90"), str(FDIR / "driver.f90"),
-           "-L", str(PKG), "-lmovfem_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", str(FDRIVER)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("amdflang failed:\n" + r.stdout[-2000:] + r.stderr[-4000:])
+    for prog, out in (("driver.f90", FDRIVER), ("run_case.f90", FRUNCASE)):
+        cmd = [fc, "-O2", "-module-dir", str(mod), str(FDIR / "movfem_gpu_mod.f90"), str(FDIR / prog),
+               "-L", str(PKG), "-lmovfem_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", str(out)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stdout[-2000:] + r.stderr[-4000:])
     return FDRIVER
 
 

@@ -102,7 +102,7 @@ void mvf_default_solver_opts(mvf_solver_opts *o) {
   o->use_graph = 0;
 }
 
-const char *mvf_last_error(const mvf_handle *h) { return h ? h->err.c_str() : "null handle"; }
+const char *mvf_last_error(const mvf_handle *h) { return h ? h->err.c_str() : mvf::g_host_err.c_str(); }
 
 void mvf_free(mvf_handle *h) {
   if (!h) return;

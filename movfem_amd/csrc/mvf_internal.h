@@ -17,6 +17,10 @@
 
 namespace mvf {
 
+// message of the last failing handle-less call (file readers, mesh
+// generation) on this thread; mvf_last_error(NULL) returns it
+inline thread_local std::string g_host_err;
+
 struct Gpml {
   // zone limits in element indices (1-based) and coordinates (init_gpml,
   // boundary_conds.f90:51-70); side 0 = "-", side 1 = "+"

@@ -56,7 +56,65 @@ module movfem_gpu
     integer(c_int64_t) :: gemm_launches
   end type mvf_solve_info
 
+  ! grid_3d's arguments in the reference's layouts (geometry.f90:48-62)
+  type, bind(C) :: mvf_mesh_input
+    real(c_double) :: dx, dy, dz, high, depth, sigma_bg
+    integer(c_int32_t) :: mn, nsr, nsf, mx, my, mz, isigma, imu, nf, pad_
+    type(c_ptr) :: xsr, ysr, zsr, nsp, xto, yto, zto, xm, ym, zm, ijsigma, sigma, ijmu, mu, freqs
+  end type mvf_mesh_input
+
+  ! PARAM.INP (MoVFEM_3DMT.f90:302-334)
+  type, bind(C) :: mvf_params
+    integer(c_int32_t) :: nf, boundary, inimod, nl, gpml_sch, pad_
+    real(c_double) :: h_sigma, a0, b0, nn
+    type(c_ptr) :: freqs, l_sigma, l_dz
+  end type mvf_params
+
   interface
+    ! input() + the PARAM.INP reads; paths are null-terminated (c_null_ptr: skip)
+    integer(c_int) function mvf_read_inputs(inp_path, param_path, h) bind(C, name='mvf_read_inputs')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: inp_path, param_path
+      type(c_ptr), intent(out) :: h
+    end function mvf_read_inputs
+
+    integer(c_int) function mvf_inputs_view(h, mesh, params) bind(C, name='mvf_inputs_view')
+      import :: c_int, c_ptr, mvf_mesh_input, mvf_params
+      type(c_ptr), value :: h
+      type(mvf_mesh_input), intent(out) :: mesh
+      type(mvf_params), intent(out) :: params
+    end function mvf_inputs_view
+
+    subroutine mvf_inputs_free(h) bind(C, name='mvf_inputs_free')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end subroutine mvf_inputs_free
+
+    ! grid_3d
+    integer(c_int) function mvf_grid3d(mesh_in, m) bind(C, name='mvf_grid3d')
+      import :: c_int, c_ptr, mvf_mesh_input
+      type(mvf_mesh_input), intent(in) :: mesh_in
+      type(c_ptr), intent(out) :: m
+    end function mvf_grid3d
+
+    integer(c_int) function mvf_mesh_sizes(m, dims, vals) bind(C, name='mvf_mesh_sizes')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: m
+      integer(c_int64_t), intent(out) :: dims(12)
+      real(c_double), intent(out) :: vals(6)
+    end function mvf_mesh_sizes
+
+    integer(c_int) function mvf_mesh_arrays(m, xp, yp, zp, sigma, mu, nzl, xsr, ysr, zsr) &
+         bind(C, name='mvf_mesh_arrays')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m, xp, yp, zp, sigma, mu, nzl, xsr, ysr, zsr
+    end function mvf_mesh_arrays
+
+    subroutine mvf_mesh_free(m) bind(C, name='mvf_mesh_free')
+      import :: c_ptr
+      type(c_ptr), value :: m
+    end subroutine mvf_mesh_free
+
     integer(c_int) function mvf_init(grid, model, bc, opts, h) bind(C, name='mvf_init')
       import :: c_int, c_ptr, mvf_grid, mvf_model, mvf_bc, mvf_options
       type(mvf_grid), intent(in) :: grid
@@ -158,6 +216,28 @@ module movfem_gpu
   end interface
 
 contains
+
+  ! mvf_read_inputs with Fortran file names; an empty name skips that file
+  integer function mvf_read_files(inp, param, h) result(ierr)
+    character(len=*), intent(in) :: inp, param
+    type(c_ptr), intent(out) :: h
+    character(kind=c_char), target :: ci(len_trim(inp) + 1), cp(len_trim(param) + 1)
+    type(c_ptr) :: pi, pp
+    integer :: i
+    do i = 1, len_trim(inp)
+      ci(i) = inp(i:i)
+    end do
+    ci(len_trim(inp) + 1) = c_null_char
+    do i = 1, len_trim(param)
+      cp(i) = param(i:i)
+    end do
+    cp(len_trim(param) + 1) = c_null_char
+    pi = c_null_ptr
+    pp = c_null_ptr
+    if (len_trim(inp) > 0) pi = c_loc(ci)
+    if (len_trim(param) > 0) pp = c_loc(cp)
+    ierr = mvf_read_inputs(pi, pp, h)
+  end function mvf_read_files
 
   ! the message of the last error on a handle, as a Fortran string
   function mvf_error_string(h) result(s)

@@ -245,4 +245,81 @@ contains
     g_nf = 1
     call write_surface_data(1)
   end subroutine ref_write_surface
+  ! ---- input() and grid_3d (read_input.f90:37-43, geometry.f90:48-122) ----
+  ! ref_read_input reads 3DEM_FEM.INP from the working directory with the
+  ! reference's reader; ref_input_sizes / ref_input_arrays hand its arrays
+  ! over before ref_grid3d (whose grid_3d modifies them) runs on them.
+  subroutine ref_read_input() bind(C, name='ref_read_input')
+    use read_input, only: input
+    call input()
+  end subroutine ref_read_input
+
+  ! isz = nsr, nsf, size(in_xto,2), imu, isigma, mx, my, mz, mn;
+  ! rsz = dx, dy, dz, high, depth, bcksigma
+  subroutine ref_input_sizes(isz, rsz) bind(C, name='ref_input_sizes')
+    use read_input
+    integer(c_int), intent(out) :: isz(9)
+    real(c_double), intent(out) :: rsz(6)
+    isz = (/ in_nsr, in_nsf, size(in_xto, 2), in_imu, in_isigma, in_mx, in_my, in_mz, in_mn /)
+    rsz = (/ in_dx, in_dy, in_dz, in_high, in_depth, in_bcksigma /)
+  end subroutine ref_input_sizes
+
+  subroutine ref_input_arrays(xsr, ysr, zsr, nsp, xto, yto, zto, xm, ym, zm, sig, mu, ijs, ijm) &
+       bind(C, name='ref_input_arrays')
+    use read_input
+    real(c_double), intent(out) :: xsr(*), ysr(*), zsr(*), xto(*), yto(*), zto(*), xm(*), ym(*), zm(*)
+    real(c_double), intent(out) :: sig(*), mu(*)
+    integer(c_int), intent(out) :: nsp(*), ijs(*), ijm(*)
+    integer :: n
+    xsr(1:in_nsr) = in_xsr; ysr(1:in_nsr) = in_ysr; zsr(1:in_nsr) = in_zsr
+    nsp(1:in_nsf) = in_nsp
+    n = size(in_xto)
+    xto(1:n) = reshape(in_xto, (/ n /)); yto(1:n) = reshape(in_yto, (/ n /))
+    zto(1:n) = reshape(in_zto, (/ n /))
+    xm(1:in_mx) = in_xm; ym(1:in_my) = in_ym
+    n = in_mx * in_my * in_mz
+    zm(1:n) = in_zm
+    sig(1:in_isigma * n) = reshape(in_sigma, (/ in_isigma * n /))
+    mu(1:size(in_mu)) = reshape(in_mu, (/ size(in_mu) /))
+    ijs(1:2 * in_isigma) = reshape(in_ijsigma, (/ 2 * in_isigma /))
+    if (in_imu > 0) ijm(1:2 * in_imu) = reshape(in_ijmu, (/ 2 * in_imu /))
+  end subroutine ref_input_arrays
+
+  ! grid_3d on the arrays input() read, as initialize_vfem calls it
+  ! (MoVFEM_3DMT.f90:337-341) but with iout = 0 (no grid files)
+  subroutine ref_grid3d(nf, freqs) bind(C, name='ref_grid3d')
+    use read_input
+    use geometry, only: grid_3d
+    integer(c_int), value :: nf
+    real(c_double), intent(in) :: freqs(nf)
+    real(kind=double), allocatable :: f(:)
+    allocate(f(nf))
+    f = freqs(1:nf)
+    call grid_3d(in_dx, in_dy, in_dz, in_high, in_depth, in_mn, in_bcksigma, in_nsf, &
+         in_nsp, in_xto, in_yto, in_zto, in_nsr, in_xsr, in_ysr, in_zsr, in_mx, in_my, &
+         in_mz, in_xm, in_ym, in_zm, in_isigma, in_ijsigma, in_sigma, in_imu, in_ijmu, in_mu, &
+         nf, f, 0, 0)
+  end subroutine ref_grid3d
+
+  ! isz = g_nx, g_ny, g_nz, g_nnx, g_nny, g_nnz, g_npt, nextd, g_nsf, g_nordx, g_nsr;
+  ! rsz = g_ztop, x0, y0, z0, omega
+  subroutine ref_grid_sizes(isz, rsz) bind(C, name='ref_grid_sizes')
+    use geometry, only: x0, y0, z0, g_nsr
+    integer(c_int), intent(out) :: isz(11)
+    real(c_double), intent(out) :: rsz(5)
+    isz = (/ g_nx, g_ny, g_nz, g_nnx, g_nny, g_nnz, g_npt, nextd, g_nsf, g_nordx, g_nsr /)
+    rsz = (/ g_ztop, x0, y0, z0, omega /)
+  end subroutine ref_grid_sizes
+
+  subroutine ref_grid_arrays(xp, yp, zp, sig, mu, nzl, xsr, ysr, zsr) bind(C, name='ref_grid_arrays')
+    use geometry, only: g_xsr, g_ysr, g_zsr, g_nsr
+    real(c_double), intent(out) :: xp(*), yp(*), zp(*), mu(*), xsr(*), ysr(*), zsr(*)
+    complex(c_double_complex), intent(out) :: sig(*)
+    integer(c_int), intent(out) :: nzl(*)
+    xp(1:g_nnx) = g_xp; yp(1:g_nny) = g_yp; zp(1:g_npt) = g_zp
+    sig(1:6 * g_npt) = reshape(g_sigma, (/ 6 * g_npt /))
+    mu(1:6 * g_npt) = reshape(g_mu, (/ 6 * g_npt /))
+    nzl(1:g_nsf) = g_nzl
+    xsr(1:g_nsr) = g_xsr; ysr(1:g_nsr) = g_ysr; zsr(1:g_nsr) = g_zsr
+  end subroutine ref_grid_arrays
 end module mvf_ref_harness

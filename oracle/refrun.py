@@ -85,9 +85,77 @@ def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, re
         return out
 
 
+def run_reference_grid(inp_text, freqs, timeout=600):
+    """The reference's input() on a 3DEM_FEM.INP holding inp_text, then its
+    grid_3d with these frequencies (fresh subprocess).  Returns the reader's
+    arrays under in_* and the lattice under g_*."""
+    import os
+
+    with tempfile.TemporaryDirectory() as td:
+        (pathlib.Path(td) / "3DEM_FEM.INP").write_text(inp_text)
+        fin, fout = pathlib.Path(td) / "in.npz", pathlib.Path(td) / "out.npz"
+        np.savez(fin, mode="grid", freqs=np.asarray(freqs, np.float64))
+        env = dict(os.environ)
+        env["PYTHONPATH"] = str(HERE.parent) + os.pathsep + env.get("PYTHONPATH", "")
+        r = subprocess.run([sys.executable, "-m", "oracle.refrun", str(fin), str(fout)],
+                           cwd=td, env=env, capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError("reference run failed:\n" + r.stdout[-2000:] + r.stderr[-2000:])
+        with np.load(fout) as z:
+            return {k: z[k] for k in z.files}
+
+
+def _grid_main(z, fout, L):
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    L.ref_input_sizes.argtypes = [ip, dp]
+    L.ref_input_arrays.argtypes = [dp] * 3 + [ip] + [dp] * 8 + [ip, ip]
+    L.ref_grid3d.argtypes = [C.c_int, dp]
+    L.ref_grid_sizes.argtypes = [ip, dp]
+    L.ref_grid_arrays.argtypes = [dp] * 5 + [ip] + [dp] * 3
+    L.ref_read_input()
+    isz = np.zeros(9, np.int32)
+    rsz = np.zeros(6)
+    L.ref_input_sizes(isz.ctypes.data_as(ip), rsz.ctypes.data_as(dp))
+    nsr, nsf, nmax, imu, isig, mx, my, mz, mn = (int(v) for v in isz)
+    n = mx * my * mz
+    out = {"in_isz": isz, "in_rsz": rsz}
+    arr = {"in_xsr": np.zeros(nsr), "in_ysr": np.zeros(nsr), "in_zsr": np.zeros(nsr),
+           "in_nsp": np.zeros(nsf, np.int32), "in_xto": np.zeros(nsf * nmax), "in_yto": np.zeros(nsf * nmax),
+           "in_zto": np.zeros(nsf * nmax), "in_xm": np.zeros(mx), "in_ym": np.zeros(my), "in_zm": np.zeros(n),
+           "in_sigma": np.zeros(isig * n), "in_mu": np.zeros(max(imu, 1) * n),
+           "in_ijsigma": np.zeros(2 * isig, np.int32), "in_ijmu": np.zeros(max(2 * imu, 1), np.int32)}
+
+    def p(a):
+        return a.ctypes.data_as(ip if a.dtype == np.int32 else dp)
+
+    L.ref_input_arrays(*(p(arr[k]) for k in ("in_xsr", "in_ysr", "in_zsr", "in_nsp", "in_xto", "in_yto", "in_zto",
+                                              "in_xm", "in_ym", "in_zm", "in_sigma", "in_mu", "in_ijsigma",
+                                              "in_ijmu")))
+    out.update(arr)
+    fr = np.ascontiguousarray(z["freqs"], np.float64)
+    L.ref_grid3d(len(fr), fr.ctypes.data_as(dp))
+    gsz = np.zeros(11, np.int32)
+    gr = np.zeros(5)
+    L.ref_grid_sizes(gsz.ctypes.data_as(ip), gr.ctypes.data_as(dp))
+    nnx, nny, npt, gnsf, gnsr = int(gsz[3]), int(gsz[4]), int(gsz[6]), int(gsz[8]), int(gsz[10])
+    g = {"g_xp": np.zeros(nnx), "g_yp": np.zeros(nny), "g_zp": np.zeros(npt), "g_sigma": np.zeros(12 * npt),
+         "g_mu": np.zeros(6 * npt), "g_nzl": np.zeros(gnsf, np.int32), "g_xsr": np.zeros(gnsr),
+         "g_ysr": np.zeros(gnsr), "g_zsr": np.zeros(gnsr)}
+    L.ref_grid_arrays(*(p(g[k]) for k in ("g_xp", "g_yp", "g_zp", "g_sigma", "g_mu", "g_nzl", "g_xsr", "g_ysr",
+                                          "g_zsr")))
+    g["g_sigma"] = g["g_sigma"].view(np.complex128).reshape(npt, 6)
+    g["g_mu"] = g["g_mu"].reshape(npt, 6)
+    out.update(g)
+    out["g_isz"], out["g_rsz"] = gsz, gr
+    np.savez(fout, **out)
+
+
 def _main(fin, fout):
     z = np.load(fin)
     L = C.CDLL(str(HERE / "_ref" / "libmovfem_ref.so"))
+    if "mode" in z.files and str(z["mode"]) == "grid":
+        return _grid_main(z, fout, L)
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)
     L.ref_setup.argtypes = ([C.c_int] * 4 + [dp] * 5 + [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] * 2

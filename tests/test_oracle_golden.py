@@ -9,8 +9,10 @@ import pytest
 from oracle.oracle import Oracle
 from tests.golden.make_golden import load_model
 
-# model fixtures (out_*.npz hold output files, tests/test_output_files.py)
-GOLD = sorted(p for p in (pathlib.Path(__file__).parent / "golden").glob("*.npz") if not p.stem.startswith("out_"))
+# model fixtures (out_*.npz hold output files, tests/test_output_files.py;
+# mesh_*.npz the lattice generator's, tests/test_mesh.py)
+GOLD = sorted(p for p in (pathlib.Path(__file__).parent / "golden").glob("*.npz")
+              if not p.stem.startswith(("out_", "mesh_")))
 
 
 def _rel(a, b):
