@@ -235,7 +235,11 @@ int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms);
  * element layer ke_top (1-based: the reference's top earth layer) inside the
  * region ie, je in [nextd+1, n-nextd] (get_rx_element); rx_e/rx_h (nrx, 2, 3)
  * complex, rx_z (nrx, 4) complex [Zxx Zxy Zyx Zyy], rho/phi (nrx, 4), degrees.
- * A receiver outside that region is MVF_E_ARG.  rx_elev (nrx): the receiver's
+ * A receiver outside those cells keeps the previous receiver's element index
+ * along each axis not found (y is searched only when x is found; the
+ * reference's loop variables, receiver_data.f90:121-124,186-209) and is
+ * extrapolated from that element; the first receiver outside them is
+ * MVF_E_ARG (the reference's indices are undefined there).  rx_elev (nrx): the receiver's
  * height on that face (interpolate_rx overwrites g_zsr with it,
  * receiver_data.f90:222-225).  node_z (npt, 4) complex, node_rho/node_phi
  * (npt, 4): Z, rho_a and phase at every lattice node (z_rho_phi,
@@ -281,6 +285,24 @@ int mvf_write_surface(const char *path, int32_t nf, int32_t append, int32_t nfre
                       const double *xs, const double *ys, const double *zs, const int64_t *ids, int64_t npt,
                       const double *esol, const double *hsol, const double *node_z, const double *node_rho,
                       const double *node_phi);
+
+/* PROBLEM.OUT (write_problem_ugrid, problem.f90:188-240), one frequency:
+ * for polarizations 1 and 2 the header (node count, nf), the column line and
+ * per node the list-directed record psigma, ep(3), hp(3), src(3), complex
+ * values as (re,im).  psigma is the real part of pe_psigma(1) = (0, omega*eps)
+ * (pset_pmodel, problem.f90:247-252): 0 at every node.  ep, hp, src:
+ * mvf_problem_fields' arrays (npt nodes); sel: nsel 1-based node ids (the
+ * reference's g_nb for inner = 1) or NULL for every node. */
+int mvf_write_problem(const char *path, int32_t nf, int32_t append, int64_t npt, const double *ep, const double *hp,
+                      const double *src, int64_t nsel, const int32_t *sel);
+
+/* The fields PROBLEM.OUT prints (set_write_problem / write_problem_elem,
+ * problem.f90:459-507) at the current frequency (mvf_set_frequency): at every
+ * lattice node the interpolated primary E and H (get_ep / get_hp, :155-186)
+ * and the source term p_source (:91-128), each complex (2, npt, 3)
+ * polarization-major, taken from the last element (reference order) holding
+ * the node.  Computed on the device; host output buffers, any may be NULL. */
+int mvf_problem_fields(mvf_handle *h, double *ep, double *hp, double *src);
 
 /* ---- input files and lattice generation (host only; no device use) --------
  * Failures return MVF_E_ARG; mvf_last_error(NULL) holds the message of the

@@ -30,7 +30,7 @@ EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne",
            "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
            "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface",
            "mvf_read_inputs", "mvf_inputs_view", "mvf_inputs_free", "mvf_grid3d", "mvf_mesh_sizes",
-           "mvf_mesh_arrays", "mvf_mesh_free")
+           "mvf_mesh_arrays", "mvf_mesh_free", "mvf_problem_fields", "mvf_write_problem")
 
 
 class Grid(C.Structure):
@@ -128,6 +128,8 @@ def lib():
     L.mvf_write_solution.argtypes = [cp, C.c_int32, C.c_int32, C.c_int64] + [dp] * 5 + [C.c_int64, ip]
     L.mvf_write_receivers.argtypes = [cp] + [C.c_int32] * 4 + [dp] * 8
     L.mvf_write_surface.argtypes = [cp] + [C.c_int32] * 5 + [dp] * 3 + [i64p, C.c_int64] + [dp] * 5
+    L.mvf_problem_fields.argtypes = [vp, dp, dp, dp]
+    L.mvf_write_problem.argtypes = [cp, C.c_int32, C.c_int32, C.c_int64, dp, dp, dp, C.c_int64, ip]
     L.mvf_read_inputs.argtypes = [cp, cp, C.POINTER(vp)]
     L.mvf_inputs_view.argtypes = [vp, C.POINTER(MeshInput), C.POINTER(Params)]
     L.mvf_inputs_free.restype = None

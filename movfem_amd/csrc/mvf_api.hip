@@ -657,15 +657,26 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
     MVF_HIP(h, hipMemcpy(hxp.data(), h->d_xp, 8 * g.nnx, hipMemcpyDeviceToHost));
     MVF_HIP(h, hipMemcpy(hyp.data(), h->d_yp, 8 * g.nny, hipMemcpyDeviceToHost));
     const int nd = h->bc.nextd;
+    // y is searched only inside the x cell holding the site; an index not
+    // found keeps the previous receiver's (rx_data's loop variables,
+    // receiver_data.f90:121-124), and that element extrapolates the site
+    int ie = -1, je = -1;
     for (int ir = 0; ir < nrx; ir++) {
-      int ie = -1, je = -1;
       for (int a = nd + 1; a <= g.nx - nd; a++)
-        if (rx[ir] >= hxp[(a - 1) * o] && rx[ir] < hxp[a * o]) ie = a;
-      for (int b = nd + 1; b <= g.ny - nd; b++)
-        if (ry[ir] >= hyp[(b - 1) * o] && ry[ir] < hyp[b * o]) je = b;
+        if (rx[ir] >= hxp[(a - 1) * o] && rx[ir] < hxp[a * o]) {
+          ie = a;
+          for (int b = nd + 1; b <= g.ny - nd; b++)
+            if (ry[ir] >= hyp[(b - 1) * o] && ry[ir] < hyp[b * o]) {
+              je = b;
+              break;
+            }
+        }
       if (ie < 0 || je < 0) {
         dfree(dx);
-        return set_err(h, MVF_E_ARG, "mvf_readout: receiver %d (%g, %g) outside the inner region", ir, rx[ir], ry[ir]);
+        return set_err(h, MVF_E_ARG,
+                       "mvf_readout: receiver %d (%g, %g) lies outside the inner cells and no earlier receiver's "
+                       "element is there to inherit (the reference's indices are undefined then)",
+                       ir, rx[ir], ry[ir]);
       }
       rie[ir] = ie;
       rje[ir] = je;
@@ -735,6 +746,29 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
   dfree(es);
   dfree(hs);
   dfree(dx);
+  return rc;
+}
+
+int mvf_problem_fields(mvf_handle *h, double *ep, double *hp, double *src) {
+  if (!h) return MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_problem_fields: no current frequency");
+  const size_t nf = (size_t)2 * h->geo.npt * 3 * 2;
+  double *de = nullptr, *dh = nullptr, *ds = nullptr;
+  int rc = dalloc(h, &de, nf);
+  if (!rc) rc = dalloc(h, &dh, nf);
+  if (!rc) rc = dalloc(h, &ds, nf);
+  if (!rc) rc = problem_fields(h, h->omega, de, dh, ds);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && ep) e = hipMemcpy(ep, de, nf * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && hp) e = hipMemcpy(hp, dh, nf * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && src) e = hipMemcpy(src, ds, nf * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_problem_fields: %s", hipGetErrorString(e));
+  }
+  dfree(de);
+  dfree(dh);
+  dfree(ds);
   return rc;
 }
 

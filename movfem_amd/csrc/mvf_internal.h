@@ -201,7 +201,9 @@ int element_rhs_range(mvf_handle *h, double omega, int first_flags, int64_t e0, 
 int assemble_rhs(mvf_handle *h, const double *d_be, double *d_b, int ncol, int col0);
 int update_sigma(mvf_handle *h, double omega);
 void direct_free(mvf_handle *h);
+int node_table(mvf_handle *h);
 int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega, double *es, double *hs);
+int problem_fields(mvf_handle *h, double omega, double *ep, double *hp, double *src);
 int receivers(mvf_handle *h, const double *es, const double *hs, int ke_top, int nrx, const int32_t *rie,
               const int32_t *rje, const double *rx, const double *ry, double omega, double *oe, double *oh, double *oz,
               double *rho, double *phi, double *elev);

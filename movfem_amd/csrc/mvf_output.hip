@@ -1,14 +1,15 @@
 // Text writers for the reference's output files (host only): SOLUTION.OUT
 // (write_solution, solution.f90:73-205), RECEIVER.OUT (write_receiver_data,
-// receiver_data.f90:25-67) and SURFACE.OUT (write_surface_data, :69-109).
+// receiver_data.f90:25-67), SURFACE.OUT (write_surface_data, :69-109) and
+// PROBLEM.OUT (write_problem_ugrid, problem.f90:188-240).
 //
 // Column formats are the reference's edit descriptors.  Its list-directed
 // records (write(u,*)) take the reference build's list-directed form: a
 // leading blank, one blank before every item, a new record before the line
 // would pass 79 columns, integers in minimal width and reals as the shortest
-// decimal that reads back to the same double, printed positionally (".1",
-// "1000.") when the value rounded to one significant digit lies in
-// [0.1, 1e15) and as d.dddE+xx otherwise.
+// decimal that reads back to the same double (the middle one when several
+// do), printed positionally (".1", "1000.") when the value rounded to one
+// significant digit lies in [0.1, 1e15) and as d.dddE+xx otherwise.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -54,25 +55,90 @@ std::string fmt_f(double v, int w, int d) {
   return rjust(buf, w);
 }
 
+// k * 10^ss == m (a long double) exactly?  k * 5^ss * 2^ss against
+// M * 2^E with m = M * 2^E, M a 64-bit integer, in base-2^32 integers.
+bool decimal_equals(unsigned long long k, int ss, long double m) {
+  int em;
+  const long double fr = frexpl(m, &em);
+  const unsigned long long M = (unsigned long long)ldexpl(fr, 64);
+  em -= 64;
+  typedef std::vector<uint32_t> Big;
+  auto from = [](unsigned long long v) { return Big{(uint32_t)v, (uint32_t)(v >> 32)}; };
+  auto mul = [](Big &b, uint32_t f) {
+    unsigned long long c = 0;
+    for (auto &w : b) {
+      c += (unsigned long long)w * f;
+      w = (uint32_t)c;
+      c >>= 32;
+    }
+    if (c) b.push_back((uint32_t)c);
+  };
+  auto shl = [&](Big &b, int n) {
+    for (; n >= 31; n -= 31) mul(b, 1u << 31);
+    if (n > 0) mul(b, 1u << n);
+  };
+  auto trim = [](Big &b) {
+    while (!b.empty() && b.back() == 0) b.pop_back();
+  };
+  Big l = from(k), r = from(M);
+  for (int i = 0; i < std::abs(ss); i++) mul(ss >= 0 ? l : r, 5);
+  const int sh = ss - em;
+  shl(sh >= 0 ? l : r, std::abs(sh));
+  trim(l);
+  trim(r);
+  return l == r;
+}
+
 // list-directed REAL(8)
 std::string ld_real(double x) {
   if (x == 0.0) return std::signbit(x) ? "-0." : "0.";
   if (std::isnan(x)) return "NaN";
   if (std::isinf(x)) return x < 0 ? "-Inf" : "Inf";
   const double a = std::fabs(x);
-  // shortest digit string that reads back exactly: a = 0.D x 10^E
+  // Shortest digit strings k * 10^sc (p digits) strictly inside the interval
+  // of reals that round to a.  The reference build excludes the interval's
+  // ends (where a decimal reaches a only by the ties-to-even rule) and, when
+  // several p-digit strings qualify, prints the middle one,
+  // floor((kmin + kmax) / 2), not the nearest: measured on its list-directed
+  // output, every case agreeing (tests/golden/ld_reals.npz).  The ends are
+  // exact in 80-bit long double; a decimal that reads back as one is compared
+  // with it exactly.
+  const long double lo_end = ((long double)a + (long double)std::nextafter(a, 0.0)) / 2;
+  const long double hi_end = ((long double)a + (long double)std::nextafter(a, (double)INFINITY)) / 2;
   char buf[64];
-  int p = 1;
-  for (; p < 17; p++) {
+  long long k = 0;
+  int p = 1, sc = 0;
+  auto reads_back = [&](long long kk, int ss) {
+    char b[48];
+    snprintf(b, sizeof(b), "%llde%d", kk, ss);
+    if (strtod(b, nullptr) != a) return false;
+    const long double c = strtold(b, nullptr);
+    if (c == lo_end) return !decimal_equals((unsigned long long)kk, ss, lo_end);
+    if (c == hi_end) return !decimal_equals((unsigned long long)kk, ss, hi_end);
+    return true;
+  };
+  for (;; p++) {
     snprintf(buf, sizeof(buf), "%.*e", p - 1, a);
-    if (strtod(buf, nullptr) == a) break;
+    k = 0;
+    for (const char *c = buf; *c && *c != 'e'; c++)
+      if (*c >= '0' && *c <= '9') k = 10 * k + (*c - '0');
+    sc = atoi(strchr(buf, 'e') + 1) - (p - 1);
+    if (p >= 17 || reads_back(k, sc)) break;
+    if (reads_back(k + 1, sc)) {  // the nearest lies outside a one-sided interval
+      k = k + 1;
+      break;
+    }
+    if (k > 1 && reads_back(k - 1, sc)) {
+      k = k - 1;
+      break;
+    }
   }
-  snprintf(buf, sizeof(buf), "%.*e", p - 1, a);
-  std::string D;
-  for (const char *c = buf; *c && *c != 'e'; c++)
-    if (*c >= '0' && *c <= '9') D += *c;
+  long long lo = k, hi = k;
+  while (lo > 1 && reads_back(lo - 1, sc)) lo--;
+  while (reads_back(hi + 1, sc)) hi++;
+  std::string D = std::to_string((lo + hi) / 2);
+  const int E = sc + (int)D.size();  // a = 0.D x 10^E
   while (D.size() > 1 && D.back() == '0') D.pop_back();
-  const int E = atoi(strchr(buf, 'e') + 1) + 1;
   // positional form when the value rounded to one digit lies in [0.1, 1e15)
   char r1[32];
   snprintf(r1, sizeof(r1), "%.0e", a);
@@ -109,6 +175,17 @@ struct ListOut {
     line += t;
   }
   void real(double v) { item(ld_real(v)); }
+  // complex: " (re," and "im)" are placed separately, so a record may end
+  // after the comma and the next one then starts with the imaginary part
+  void cplx(double re, double im) {
+    item("(" + ld_real(re) + ",");
+    const std::string t = ld_real(im) + ")";
+    if (line.size() + t.size() > 79) {
+      fprintf(f, "%s\n", line.c_str());
+      line.clear();
+    }
+    line += t;
+  }
   void integer(long long v) { item(std::to_string(v)); }
   void end() {
     fprintf(f, "%s\n", line.c_str());
@@ -285,6 +362,42 @@ int mvf_write_surface(const char *path, int32_t nf, int32_t append, int32_t nfre
     for (int c = 0; c < 4; c++) s += fmt_f(node_rho[4 * ids[i] + c], 20, 5);
     for (int c = 0; c < 4; c++) s += fmt_f(node_phi[4 * ids[i] + c], 20, 5);
     fprintf(f, "%s\n", s.c_str());
+  }
+  const int bad = ferror(f);
+  return (fclose(f) != 0 || bad) ? MVF_E_ARG : 0;
+}
+
+// write_problem_ugrid (problem.f90:188-240), ndir = 2: per polarization the
+// node count and frequency index, the column line, and per node the
+// list-directed record dble(psigma), ep(1:3), hp(1:3), src(1:3).  psigma is
+// pe_psigma(1) = (0, omega*eps) at every node (pset_pmodel, :247-252), so its
+// real part, 0, is printed.
+int mvf_write_problem(const char *path, int32_t nf, int32_t append, int64_t npt, const double *ep, const double *hp,
+                      const double *src, int64_t nsel, const int32_t *sel) {
+  if (!path || npt <= 0 || !ep || !hp || !src || nsel < 0 || (nsel > 0 && !sel)) return MVF_E_ARG;
+  for (int64_t i = 0; i < nsel; i++)
+    if (sel[i] < 1 || sel[i] > npt) return MVF_E_ARG;
+  FILE *f = open_out(path, append);
+  if (!f) return MVF_E_ARG;
+  const int64_t nrow = sel ? nsel : npt;
+  for (int d = 0; d < 2; d++) {
+    ListOut lo{f, ""};
+    lo.item("problem main functions in structured grid");
+    lo.integer(nrow);
+    lo.item("f= ");
+    lo.integer(nf);
+    lo.end();
+    lo.item("--psigma--epx--epy--epz--hpx--hpy--hpz--srcx--srcy--srcz");
+    lo.end();
+    for (int64_t r = 0; r < nrow; r++) {
+      const int64_t id = sel ? sel[r] - 1 : r;
+      lo.real(0.0);
+      for (const double *a : {ep, hp, src}) {
+        const double *v = a + 2 * ((d * npt + id) * 3);
+        for (int c = 0; c < 3; c++) lo.cplx(v[2 * c], v[2 * c + 1]);
+      }
+      lo.end();
+    }
   }
   const int bad = ferror(f);
   return (fclose(f) != 0 || bad) ? MVF_E_ARG : 0;

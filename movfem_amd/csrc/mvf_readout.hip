@@ -126,6 +126,82 @@ __global__ __launch_bounds__(256) void k_node_h(const BasisTab *ntab, Geom g, co
     hsol[(d * g.npt + id) * 3 + c] = make_double2(0.0 * tr[c] - mi * ti[c], 0.0 * ti[c] + mi * tr[c]);
 }
 
+// PROBLEM.OUT's fields (set_write_problem / write_problem_elem,
+// problem.f90:459-507): the reference's element loop writes every lattice
+// point of each element in turn, so the last element containing a node sets
+// its values: max over the elements along each axis, min(n, a/o + 1).  At
+// that point: the interpolated primary fields (get_ep / get_hp, :155-186) and
+// the source p_source (:91-128) of polarization d.
+template <int NN>
+__global__ __launch_bounds__(256) void k_problem(const BasisTab *ntab, Geom g, const double *xp, const double *yp,
+                                                 const double *zp, const double *sig, const double *inmu,
+                                                 const double *dmu, double omega, double2 *ep, double2 *hp,
+                                                 double2 *src) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 2 * g.npt) return;
+  const int d = (int)(t / g.npt);
+  const int64_t id = t % g.npt;
+  const int o = g.nord - 1;
+  const int a = (int)(id / g.nyz), b = (int)((id / g.nnz) % g.nny), c = (int)(id % g.nnz);
+  const int ie = min(g.nx, a / o + 1), je = min(g.ny, b / o + 1), ke = min(g.nz, c / o + 1);
+  const int p = ((a - (ie - 1) * o) * g.nord + (b - (je - 1) * o)) * g.nord + (c - (ke - 1) * o);
+  double xs[NN][3];
+  int64_t nid[NN];
+  for (int n = 0; n < NN; n++) {
+    int ix, jy;
+    load_node(ntab, g, ie, je, ke, n, nid[n], ix, jy);
+    xs[n][0] = xp[ix];
+    xs[n][1] = yp[jy];
+    xs[n][2] = zp[nid[n]];
+  }
+  double Ji[9];
+  jacobian<NN>(ntab, p, xs, Ji);
+  const double w32 = (double)(float)omega;
+  const double psig = (double)(float)(omega * MVF_EPS0);
+  const double hh = (double)(float)MVF_B0 / (4 * MVF_PI * 1.e-7);
+  // Ep has one imaginary component per node (x: -i zz for d = 0, y: +i zz
+  // for d = 1), Hp one real component (y for d = 0, x for d = 1)
+  const int ce = d == 0 ? 0 : 1, chh = d == 0 ? 1 : 0;
+  static const int T6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  double epi = 0.0, hpr = 0.0, sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};
+  for (int n = 0; n < NN; n++) {
+    const double N = ntab->N[p][n];
+    const double zz = (double)(float)(omega * MVF_B0 * xs[n][2]);
+    const double e = d == 0 ? -zz : zz;
+    epi += e * N;
+    hpr += hh * N;
+    // p_dmpf: N (dsigma . Ep)
+    for (int q = 0; q < 3; q++) {
+      const int k6 = T6[q][ce];
+      double dr = sig[2 * (nid[n] * 6 + k6)], di = sig[2 * (nid[n] * 6 + k6) + 1];
+      if (k6 == 0 || k6 == 3 || k6 == 5) di -= psig;
+      sr[q] += N * (-di * e);
+      si[q] += N * (dr * e);
+    }
+    // p_pcurl: grad N x (mu^-1 dmu Hp)
+    const double *im = inmu + nid[n] * 6, *dm = dmu + nid[n] * 6;
+    double v[3];
+    for (int q = 0; q < 3; q++) {
+      double m = 0.0;
+      for (int r = 0; r < 3; r++) m += im[T6[q][r]] * dm[T6[r][chh]];
+      v[q] = m * hh;
+    }
+    double dn[3];
+    for (int q = 0; q < 3; q++)
+      dn[q] = Ji[3 * q] * ntab->dN[p][0][n] + Ji[3 * q + 1] * ntab->dN[p][1][n] + Ji[3 * q + 2] * ntab->dN[p][2][n];
+    sr[0] += v[2] * dn[1] - v[1] * dn[2];
+    sr[1] += v[0] * dn[2] - v[2] * dn[0];
+    sr[2] += v[1] * dn[0] - v[0] * dn[1];
+  }
+  const int64_t base = (d * g.npt + id) * 3;
+  for (int q = 0; q < 3; q++) {
+    ep[base + q] = make_double2(0.0, q == ce ? epi : 0.0);
+    hp[base + q] = make_double2(q == chh ? hpr : 0.0, 0.0);
+    // times -i w32: (a + i b)(-i w) = b w - i a w
+    src[base + q] = make_double2(si[q] * w32, -sr[q] * w32);
+  }
+}
+
 __device__ inline double2 zmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 zdiv(double2 a, double2 b) {
   const double d = b.x * b.x + b.y * b.y;
@@ -223,9 +299,9 @@ static void launch_node_fields(mvf_handle *h, const double2 *x, int ncol, int co
                      h->d_yp, h->d_zp, h->d_inmu, omega, es, hs);
 }
 
-// nodal E and H (device arrays of 2*npt*3 complex) for solution columns
-// (col0, col0+1) of x ([nne][ncol] complex, device) at angular frequency omega
-int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega, double *es, double *hs) {
+// basis functions and derivatives at the nord^3 lattice points of an element
+// (uploaded once)
+int node_table(mvf_handle *h) {
   if (!h->d_ntab) {
     const int nord = h->et.nord;
     double pts[27][3];
@@ -249,6 +325,13 @@ int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega
     delete t;
     if (rc) return rc;
   }
+  return 0;
+}
+
+// nodal E and H (device arrays of 2*npt*3 complex) for solution columns
+// (col0, col0+1) of x ([nne][ncol] complex, device) at angular frequency omega
+int node_fields(mvf_handle *h, const double *x, int ncol, int col0, double omega, double *es, double *hs) {
+  MVF_TRY(node_table(h));
   MVF_TRY(bd_values(h, omega));
   const double2 *xv = (const double2 *)x;
   double2 *e2 = (double2 *)es, *h2 = (double2 *)hs;
@@ -267,6 +350,25 @@ int receivers(mvf_handle *h, const double *es, const double *hs, int ke_top, int
   hipLaunchKernelGGL(k_receivers, dim3(grid1d(nrx, 64)), dim3(64), 0, h->stream, h->geo, nrx, rie, rje, ke_top, rx, ry,
                      h->d_xp, h->d_yp, h->d_zp, (const double2 *)es, (const double2 *)hs, omega, (double2 *)oe,
                      (double2 *)oh, (double2 *)oz, rho, phi, elev);
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+// PROBLEM.OUT's fields at the current frequency into device arrays of
+// 2*npt*3 complex each
+int problem_fields(mvf_handle *h, double omega, double *ep, double *hp, double *src) {
+  MVF_TRY(node_table(h));
+  const int64_t n = 2 * h->geo.npt;
+  double2 *e2 = (double2 *)ep, *h2 = (double2 *)hp, *s2 = (double2 *)src;
+#define MVF_PROBLEM(NN)                                                                                            \
+  hipLaunchKernelGGL((k_problem<NN>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, h->d_ntab, h->geo, h->d_xp,   \
+                     h->d_yp, h->d_zp, h->d_sig, h->d_inmu, h->d_dmu, omega, e2, h2, s2)
+  switch (h->nodes) {
+    case 8: MVF_PROBLEM(8); break;
+    case 20: MVF_PROBLEM(20); break;
+    default: MVF_PROBLEM(27); break;
+  }
+#undef MVF_PROBLEM
   MVF_HIP(h, hipGetLastError());
   return 0;
 }

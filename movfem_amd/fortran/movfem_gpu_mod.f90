@@ -13,6 +13,7 @@ module movfem_gpu
   integer(c_int), parameter :: MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1
   integer(c_int), parameter :: MVF_MATRIX_C64REF = 0, MVF_MATRIX_FP64 = 1
   integer(c_int), parameter :: MVF_PC_NONE = 0, MVF_PC_JACOBI = 1, MVF_PC_DIRECT = 2
+  integer(c_int), parameter :: MVF_E_ARG = -1, MVF_E_DEVICE = -2, MVF_E_NOCONV = -6
 
   type, bind(C) :: mvf_grid
     integer(c_int32_t) :: nx, ny, nz, nodes_per_elem
@@ -204,6 +205,22 @@ module movfem_gpu
       integer(c_int32_t), value :: nf, append, nfreq, nrx
       type(c_ptr), value :: rx, ry, elev, rx_e, rx_h, rx_z, rho, phi
     end function mvf_write_receivers
+
+    ! PROBLEM.OUT's fields at the current frequency (complex (2, npt, 3) each)
+    integer(c_int) function mvf_problem_fields(h, ep, hp, src) bind(C, name='mvf_problem_fields')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, ep, hp, src
+    end function mvf_problem_fields
+
+    ! write_problem_ugrid; sel: nsel 1-based node ids (g_nb) or c_null_ptr
+    integer(c_int) function mvf_write_problem(path, nf, append, npt, ep, hp, src, nsel, sel) &
+         bind(C, name='mvf_write_problem')
+      import :: c_int, c_ptr, c_char, c_int32_t, c_int64_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nf, append
+      integer(c_int64_t), value :: npt, nsel
+      type(c_ptr), value :: ep, hp, src, sel
+    end function mvf_write_problem
 
     integer(c_int) function mvf_write_surface(path, nf, append, nfreq, nxb, nyb, xs, ys, zs, ids, npt, esol, &
          hsol, node_z, node_rho, node_phi) bind(C, name='mvf_write_surface')

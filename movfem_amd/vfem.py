@@ -253,6 +253,16 @@ class GpuVfem:
         return out
 
 
+    def problem_fields(self):
+        """PROBLEM.OUT's fields at the current frequency (write_problem_elem,
+        problem.f90:459-507): primary E, primary H and the source term, each
+        complex (2, npt, 3), from the last element holding each node."""
+        npt = self.m.npt
+        out = [np.empty(2 * npt * 3, np.complex128) for _ in range(3)]
+        self._check(self._lib.mvf_problem_fields(self.h, *(_dp(a.view(np.float64)) for a in out)))
+        return tuple(a.reshape(2, npt, 3) for a in out)
+
+
 # ---------------------------------------------------------------- text output
 def surface_nodes(m: MTModel, nzl_top: int, nzl_next: int):
     """The surface points SURFACE.OUT lists (sur_data, receiver_data.f90:145-175):
@@ -312,3 +322,14 @@ def write_surface(path, nf, nfreq, surface, esol, hsol, node_z, node_rho, node_p
                                       _dp(_c(zs)), _c(ids, np.int64).ctypes.data_as(C.POINTER(C.c_int64)),
                                       es.size // 6, _dp(es.view(np.float64)), _dp(hs.view(np.float64)),
                                       _dp(nz.view(np.float64)), _dp(nr), _dp(npf)), "mvf_write_surface")
+
+
+def write_problem(path, nf, ep, hp, src, sel=None, append=False):
+    """PROBLEM.OUT block of frequency nf (write_problem_ugrid, problem.f90:188-240)."""
+    e, h, s = (_c(a, np.complex128) for a in (ep, hp, src))
+    npt = e.size // 6
+    q = None if sel is None else _c(sel, np.int32)
+    _wcheck(L.lib().mvf_write_problem(str(path).encode(), nf, int(append), npt, _dp(e.view(np.float64)),
+                                      _dp(h.view(np.float64)), _dp(s.view(np.float64)),
+                                      0 if q is None else len(q), None if q is None else _ip(q)),
+            "mvf_write_problem")

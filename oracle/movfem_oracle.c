@@ -626,6 +626,51 @@ static void orc_get_pml(orc_state *s, int i, int j, int k) {
   if (k >= s->el_za[1] && k <= s->el_zb[1]) s->in_pml[2] = 1;
 }
 
+/* p_source (problem.f90:91-128) at reference point x of the element in w
+ * (Ji: inverse Jacobian there), both polarizations, into src[d][g][c]:
+ * (sum_n grad N_n x (mu_n^-1 dmu_n) Hp_n + sum_n N_n dsigma_n Ep_n) (-i fl32(w)) */
+static void orc_psource(const orc_state *s, const orc_ws *w, const double *x, double Ji[3][3], double w32,
+                        zc src[][27][3], int g) {
+  int mn = s->mn;
+  for (int d = 0; d < 2; d++) {
+    zc pc[3] = {0, 0, 0}, dm[3] = {0, 0, 0};
+    for (int n = 0; n < mn; n++) {
+      /* p_pcurl: grad N x (mu^-1 dmu) Hp */
+      double dn[3];
+      for (int a = 0; a < 3; a++) {
+        double acc = 0.0;
+        for (int b = 0; b < 3; b++) acc += Ji[a][b] * nf_dln(s, b, n + 1, x);
+        dn[a] = acc;
+      }
+      const double *im = w->inmu[n], *dmu = w->dmu[n];
+      double M[3][3] = {
+          {im[0] * dmu[0] + im[1] * dmu[1] + im[2] * dmu[2], im[0] * dmu[1] + im[1] * dmu[3] + im[2] * dmu[4],
+           im[0] * dmu[2] + im[1] * dmu[4] + im[2] * dmu[5]},
+          {im[1] * dmu[0] + im[3] * dmu[1] + im[4] * dmu[2], im[1] * dmu[1] + im[3] * dmu[3] + im[4] * dmu[4],
+           im[1] * dmu[2] + im[3] * dmu[4] + im[4] * dmu[5]},
+          {im[2] * dmu[0] + im[4] * dmu[1] + im[5] * dmu[2], im[2] * dmu[1] + im[4] * dmu[3] + im[5] * dmu[4],
+           im[2] * dmu[2] + im[4] * dmu[4] + im[5] * dmu[5]}};
+      zc v[3];
+      for (int a = 0; a < 3; a++) v[a] = M[a][0] * w->hp[d][n][0] + M[a][1] * w->hp[d][n][1] + M[a][2] * w->hp[d][n][2];
+      pc[0] += v[2] * dn[1] - v[1] * dn[2];
+      pc[1] += v[0] * dn[2] - v[2] * dn[0];
+      pc[2] += v[1] * dn[0] - v[0] * dn[1];
+      /* p_dmpf: N * (dsigma . Ep) */
+      const zc *ds = w->dsig[n];
+      const zc *e = w->ep[d][n];
+      zc u0 = ds[0] * e[0] + ds[1] * e[1] + ds[2] * e[2];
+      zc u1 = ds[1] * e[0] + ds[3] * e[1] + ds[4] * e[2];
+      zc u2 = ds[2] * e[0] + ds[4] * e[1] + ds[5] * e[2];
+      double N = nf_ln(s, n + 1, x);
+      dm[0] += N * u0;
+      dm[1] += N * u1;
+      dm[2] += N * u2;
+    }
+    zc mw = CMPLX(0.0, -w32);
+    for (int c = 0; c < 3; c++) src[d][g][c] = (dm[c] + pc[c]) * mw;
+  }
+}
+
 /* nf_get_r + p_elem_fields + int_elem_params for element (ie,je,ke), 1-based */
 static int orc_elem_params(orc_state *s, orc_ws *w, int ie, int je, int ke) {
   int o = s->nord - 1, mn = s->mn, G = s->gpt, me = s->me;
@@ -701,43 +746,7 @@ static int orc_elem_params(orc_state *s, orc_ws *w, int ie, int je, int ke) {
       w->h[g][0] = w->h[g][1] = w->h[g][2] = 1.0;
     }
     /* p_source (problem.f90:91-128) */
-    for (int d = 0; d < 2; d++) {
-      zc pc[3] = {0, 0, 0}, dm[3] = {0, 0, 0};
-      for (int n = 0; n < mn; n++) {
-        /* p_pcurl: grad N x (mu^-1 dmu) Hp */
-        double dn[3];
-        for (int a = 0; a < 3; a++) {
-          double acc = 0.0;
-          for (int b = 0; b < 3; b++) acc += Ji[a][b] * nf_dln(s, b, n + 1, x);
-          dn[a] = acc;
-        }
-        const double *im = w->inmu[n], *dmu = w->dmu[n];
-        double M[3][3] = {
-            {im[0] * dmu[0] + im[1] * dmu[1] + im[2] * dmu[2], im[0] * dmu[1] + im[1] * dmu[3] + im[2] * dmu[4],
-             im[0] * dmu[2] + im[1] * dmu[4] + im[2] * dmu[5]},
-            {im[1] * dmu[0] + im[3] * dmu[1] + im[4] * dmu[2], im[1] * dmu[1] + im[3] * dmu[3] + im[4] * dmu[4],
-             im[1] * dmu[2] + im[3] * dmu[4] + im[4] * dmu[5]},
-            {im[2] * dmu[0] + im[4] * dmu[1] + im[5] * dmu[2], im[2] * dmu[1] + im[4] * dmu[3] + im[5] * dmu[4],
-             im[2] * dmu[2] + im[4] * dmu[4] + im[5] * dmu[5]}};
-        zc v[3];
-        for (int a = 0; a < 3; a++) v[a] = M[a][0] * w->hp[d][n][0] + M[a][1] * w->hp[d][n][1] + M[a][2] * w->hp[d][n][2];
-        pc[0] += v[2] * dn[1] - v[1] * dn[2];
-        pc[1] += v[0] * dn[2] - v[2] * dn[0];
-        pc[2] += v[1] * dn[0] - v[0] * dn[1];
-        /* p_dmpf: N * (dsigma . Ep) */
-        const zc *ds = w->dsig[n];
-        const zc *e = w->ep[d][n];
-        zc u0 = ds[0] * e[0] + ds[1] * e[1] + ds[2] * e[2];
-        zc u1 = ds[1] * e[0] + ds[3] * e[1] + ds[4] * e[2];
-        zc u2 = ds[2] * e[0] + ds[4] * e[1] + ds[5] * e[2];
-        double N = nf_ln(s, n + 1, x);
-        dm[0] += N * u0;
-        dm[1] += N * u1;
-        dm[2] += N * u2;
-      }
-      zc mw = CMPLX(0.0, -w32);
-      for (int c = 0; c < 3; c++) w->src[d][g][c] = (dm[c] + pc[c]) * mw;
-    }
+    orc_psource(s, w, x, Ji, w32, w->src, g);
     /* int_edges (integration.f90:139-152) */
     for (int j = 0; j < me; j++) {
       int node = s->ed[j][0], dir = s->ed[j][1] - 1;
@@ -1191,6 +1200,64 @@ int orc_global_vfem(orc_state *s, int32_t *ia, int32_t *ja, double *a, double *b
   return 0;
 }
 
+/* ------------------------- PROBLEM.OUT fields ----------------------------- */
+/* set_write_problem + write_problem_elem (problem.f90:459-507): the element
+ * loop visits every element in reference order and writes, at each of its
+ * nord^3 lattice points r, the interpolated primary fields get_ep/get_hp
+ * (problem.f90:155-186) and the source p_source(r) (:91-128) of polarization
+ * d, so the last element containing a node sets its values.  psigma is
+ * pe_psigma(1) = (0, omega*eps) (pset_pmodel, :247-252), whose real part
+ * PROBLEM.OUT prints.  ep/hp/src: complex (2*npt, 3), polarization-major. */
+int orc_problem_fields(orc_state *s, double *ep, double *hp, double *src) {
+  int o = s->nord - 1, mn = s->mn, nord = s->nord;
+  int64_t npt = s->npt;
+  double as[3] = {-1.0, 0.0, 1.0};
+  if (nord == 2) as[1] = 1.0;
+  orc_ws *w = (orc_ws *)malloc(sizeof(orc_ws));
+  const double w32 = (double)(float)(s->omega);
+  for (int ie = 1; ie <= s->nx; ie++)
+    for (int je = 1; je <= s->ny; je++)
+      for (int ke = 1; ke <= s->nz; ke++) {
+        int64_t eno = (int64_t)(ie - 1) * s->nyz * o + (int64_t)(je - 1) * s->nnz * o + (int64_t)(ke - 1) * o;
+        int rc = orc_elem_params(s, w, ie, je, ke);
+        if (rc) {
+          free(w);
+          return rc;
+        }
+        for (int i1 = 0; i1 < nord; i1++)
+          for (int j1 = 0; j1 < nord; j1++)
+            for (int k1 = 0; k1 < nord; k1++) {
+              int64_t id = eno + (int64_t)i1 * s->nyz + (int64_t)j1 * s->nnz + k1;
+              double r[3] = {as[i1], as[j1], as[k1]};
+              double J[3][3], Ji[3][3], det;
+              orc_jac(s, w->re, r, J, Ji, &det);
+              zc sr[2][27][3];
+              orc_psource(s, w, r, Ji, w32, sr, 0);
+              for (int d = 0; d < 2; d++) {
+                zc e[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+                for (int n = 0; n < mn; n++) {
+                  double N = nf_ln(s, n + 1, r);
+                  for (int c = 0; c < 3; c++) {
+                    e[c] += w->ep[d][n][c] * N;
+                    h[c] += w->hp[d][n][c] * N;
+                  }
+                }
+                int64_t idd = id + d * npt;
+                for (int c = 0; c < 3; c++) {
+                  ep[2 * (idd * 3 + c)] = creal(e[c]);
+                  ep[2 * (idd * 3 + c) + 1] = cimag(e[c]);
+                  hp[2 * (idd * 3 + c)] = creal(h[c]);
+                  hp[2 * (idd * 3 + c) + 1] = cimag(h[c]);
+                  src[2 * (idd * 3 + c)] = creal(sr[d][0][c]);
+                  src[2 * (idd * 3 + c) + 1] = cimag(sr[d][0][c]);
+                }
+              }
+            }
+      }
+  free(w);
+  return 0;
+}
+
 /* ------------------------------ readout ---------------------------------- */
 /* node_solution (solution.f90:18-49) with get_elem_sol/get_er/get_hr
  * (:207-438) for pe_sch = 1: total E at every lattice node (secondary from the
@@ -1342,13 +1409,22 @@ void orc_impedance(const double *e_, const double *h_, double omega, double *z_,
 int orc_receivers(orc_state *s, const double *esol, const double *hsol, int ke_top, int nrx, const double *rx,
                   const double *ry, double *out_e, double *out_h, double *out_z, double *rho, double *phi) {
   int o = s->nord - 1;
+  /* get_rx_element (receiver_data.f90:186-209): y is searched only inside the
+   * x cell that holds the site; an index not found keeps the previous
+   * receiver's value (rx_data's loop variables, :121-124), and the site is
+   * then extrapolated from that element. */
+  int ie = -1, je = -1;
   for (int ir = 0; ir < nrx; ir++) {
-    int ie = -1, je = -1;
     for (int a = s->nextd + 1; a <= s->nx - s->nextd; a++)
-      if (rx[ir] >= s->xp[(a - 1) * o] && rx[ir] < s->xp[a * o]) ie = a;
-    for (int b = s->nextd + 1; b <= s->ny - s->nextd; b++)
-      if (ry[ir] >= s->yp[(b - 1) * o] && ry[ir] < s->yp[b * o]) je = b;
-    if (ie < 0 || je < 0) return -5;
+      if (rx[ir] >= s->xp[(a - 1) * o] && rx[ir] < s->xp[a * o]) {
+        ie = a;
+        for (int b = s->nextd + 1; b <= s->ny - s->nextd; b++)
+          if (ry[ir] >= s->yp[(b - 1) * o] && ry[ir] < s->yp[b * o]) {
+            je = b;
+            break;
+          }
+      }
+    if (ie < 0 || je < 0) return -5; /* no earlier receiver to inherit from */
     int ke = ke_top;
     int64_t eno = (int64_t)(ie - 1) * o * s->nyz + (int64_t)(je - 1) * o * s->nnz + (int64_t)(ke - 1) * o;
     double x1 = s->xp[(ie - 1) * o], x2 = s->xp[(ie - 1) * o + o];

@@ -61,6 +61,8 @@ def lib():
         L.orc_remove_zeros.argtypes = [C.c_int64, ip, ip, dp]
         L.orc_node_solution.restype = C.c_int
         L.orc_node_solution.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_problem_fields.restype = C.c_int
+        L.orc_problem_fields.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_receivers.restype = C.c_int
         L.orc_receivers.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]
         L.orc_set_bdmodel.restype = C.c_int
@@ -200,6 +202,17 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_global_vfem failed ({rc})")
         return ia, ja, a.view(np.complex128), b.view(np.complex128)
+
+    def problem_fields(self):
+        """write_problem_elem's values at every lattice node (last element in
+        reference order wins): primary E, primary H and the source term, each
+        complex (2, npt, 3) for polarizations 1, 2 at the current frequency."""
+        npt = self.m.npt
+        out = [np.zeros(2 * npt * 3, np.complex128) for _ in range(3)]
+        rc = lib().orc_problem_fields(self.h, *(_dp(a.view(np.float64)) for a in out))
+        if rc:
+            raise RuntimeError(f"orc_problem_fields failed ({rc})")
+        return tuple(a.reshape(2, npt, 3) for a in out)
 
     def node_solution(self, x):
         """esol, hsol (2, npt, 3) complex: total E and H at every lattice node for

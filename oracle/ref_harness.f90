@@ -245,6 +245,15 @@ contains
     g_nf = 1
     call write_surface_data(1)
   end subroutine ref_write_surface
+  ! write_problem_ugrid(1, 0) (problem.f90:188-240): PROBLEM.OUT in the
+  ! working directory, every lattice node, both polarizations, at the current
+  ! frequency.  g_nf = 1 so that the reference closes the file.
+  subroutine ref_write_problem() bind(C, name='ref_write_problem')
+    use problem, only: write_problem_ugrid
+    g_nf = 1
+    call write_problem_ugrid(1, 0)
+  end subroutine ref_write_problem
+
   ! ---- input() and grid_3d (read_input.f90:37-43, geometry.f90:48-122) ----
   ! ref_read_input reads 3DEM_FEM.INP from the working directory with the
   ! reference's reader; ref_input_sizes / ref_input_arrays hand its arrays

@@ -23,7 +23,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 
 
 def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=None, receivers=None,
-               write_files=False):
+               write_files=False, write_problem=False):
     rx = np.zeros((0, 2)) if receivers is None else np.asarray(receivers["xy"], np.float64)
     rz = np.zeros(2, np.int32) if receivers is None else np.asarray(receivers["nzl"], np.int32)
     np.savez(path, nx=m.nx, ny=m.ny, nz=m.nz, mn=m.nodes_per_elem, xp=m.xp, yp=m.yp, zp=m.zp,
@@ -33,7 +33,7 @@ def dump_model(m, path, elements=(), vfem_freqs=(), sigma_after=(), readout_x=No
              l_dz=np.asarray(m.l_dz, np.float64), elements=np.asarray(elements, np.int32).reshape(-1, 3),
              vfem_freqs=np.asarray(vfem_freqs, np.int32), sigma_after=np.asarray(sigma_after, np.int32),
              readout_x=np.zeros(0, complex) if readout_x is None else np.asarray(readout_x, np.complex128),
-             rx_xy=rx, rx_nzl=rz, write_files=int(write_files))
+             rx_xy=rx, rx_nzl=rz, write_files=int(write_files), write_problem=int(write_problem))
 
 
 def parse_receiver_out(path, nsr):
@@ -60,15 +60,17 @@ def parse_receiver_out(path, nsr):
 
 
 def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, readout_x=None, receivers=None,
-                  write_files=False):
+                  write_files=False, write_problem=False):
     """Returns a dict of reference outputs for model m (fresh subprocess).  write_files:
     also run the reference's SOLUTION.OUT and SURFACE.OUT writers after node_solution;
-    their text is returned under "files" with RECEIVER.OUT's."""
+    their text is returned under "files" with RECEIVER.OUT's.  write_problem: run
+    write_problem_ugrid at the first frequency; PROBLEM.OUT's text is returned under
+    "problem_out"."""
     import os
 
     with tempfile.TemporaryDirectory() as td:
         fin, fout = pathlib.Path(td) / "in.npz", pathlib.Path(td) / "out.npz"
-        dump_model(m, fin, elements, vfem_freqs, sigma_after, readout_x, receivers, write_files)
+        dump_model(m, fin, elements, vfem_freqs, sigma_after, readout_x, receivers, write_files, write_problem)
         env = dict(os.environ)
         env["PYTHONPATH"] = str(HERE.parent) + os.pathsep + env.get("PYTHONPATH", "")
         r = subprocess.run([sys.executable, "-m", "oracle.refrun", str(fin), str(fout)],
@@ -79,6 +81,8 @@ def run_reference(m, elements=(), vfem_freqs=(), sigma_after=(), timeout=600, re
             out = {k: z[k] for k in z.files}
         if receivers is not None:
             out.update(parse_receiver_out(pathlib.Path(td) / "RECEIVER.OUT", len(receivers["xy"])))
+        if write_problem:
+            out["problem_out"] = (pathlib.Path(td) / "PROBLEM.OUT").read_text()
         if write_files:
             out["files"] = {n: (pathlib.Path(td) / n).read_text()
                             for n in ("SOLUTION.OUT", "RECEIVER.OUT", "SURFACE.OUT") if (pathlib.Path(td) / n).exists()}
@@ -200,6 +204,8 @@ def _main(fin, fout):
     out["nzindx"] = nz.reshape(ne, me, me)
     npt = len(zp)
     L.ref_set_frequency(1)
+    if "write_problem" in z.files and int(z["write_problem"]):
+        L.ref_write_problem()
     els = z["elements"]
     if len(els):
         A = np.empty((len(els), me * me * 2))

@@ -3,7 +3,8 @@ the reference's own Fortran modules by oracle/Makefile).
 
 Run in the container that holds the reference sources:
     make -C oracle ref && python -m tests.golden.make_golden
-Each fixture stores the model inputs (lattice, sigma, mu, boundary settings)
+out_problem_hex8_mu.npz holds the reference's PROBLEM.OUT for one parity case.
+Each model fixture stores the model inputs (lattice, sigma, mu, boundary settings)
 and the reference's outputs: gne, nzindx, alocal/blocal of every element and
 global_vfem (ia, ja, a, b) for the first two frequencies, plus g_sigma after
 update_sigma for frequency 2.  The Dirichlet inimod 2/3 fixtures add the
@@ -73,9 +74,23 @@ def write_outfiles_golden():
     print("out_hex8_small", {k: len(v) for k, v in files.items()})
 
 
+def write_problem_golden():
+    """out_problem_hex8_mu.npz: the reference's PROBLEM.OUT (write_problem_ugrid,
+    every node, both polarizations) for the 8-node Dirichlet case with topography
+    and a permeability contrast (tests/parity_cases.py), at its first frequency."""
+    from tests.parity_cases import cases
+
+    r = run_reference(cases()["hex8_dirichlet_topo_mu"], write_problem=True)
+    np.savez_compressed(HERE / "out_problem_hex8_mu.npz",
+                        PROBLEM_OUT=np.frombuffer(r["problem_out"].encode(), np.uint8))
+    print("out_problem_hex8_mu", len(r["problem_out"]))
+
+
 def main(names=()):
     if not names or "out_hex8_small" in names:
         write_outfiles_golden()
+    if not names or "out_problem_hex8_mu" in names:
+        write_problem_golden()
     for name, m in golden_cases().items():
         if names and name not in names:
             continue

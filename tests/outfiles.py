@@ -3,9 +3,9 @@ SURFACE.OUT) written by two sources.
 
 Lines must have the same length and tokens.  A token pair passes when the
 strings are equal, or when both read as numbers and agree to rel_tol (0: the
-same double).  The reference build picks, among the shortest decimals that
-read back to a double, not always the one nearest to it; the library writes
-the nearest, so list-directed reals are compared by value."""
+same double).  Equal doubles give identical text (the library prints, among
+the shortest decimals that read back, the middle one, as the reference build
+does); the tolerances serve files written from device-computed values."""
 import numpy as np
 
 
@@ -62,3 +62,30 @@ def receiver_heights(m, xs, ys, ke_top):
         zx2 = ((x2 - x) / (x2 - x1)) * z12 + ((x - x1) / (x2 - x1)) * z22
         out.append(((y2 - y) / (y2 - y1)) * zx1 + ((y - y1) / (y2 - y1)) * zx2)
     return np.array(out)
+
+
+def parse_problem(text):
+    """PROBLEM.OUT blocks (write_problem_ugrid, problem.f90:188-240): a list of
+    (header line, complex array (nrow, 10)): psigma, ep(3), hp(3), src(3) per
+    row, records joined over their line breaks."""
+    import re
+
+    blocks, lines, i = [], text.split("\n"), 0
+    while i < len(lines):
+        if not lines[i].strip().startswith("problem main functions"):
+            i += 1
+            continue
+        head, i = lines[i], i + 2
+        body = []
+        while i < len(lines) and not lines[i].strip().startswith("problem main functions"):
+            body.append(lines[i])
+            i += 1
+        vals = []
+        for t in re.findall(r"\([^)]*\)|[^\s()]+", "\n".join(body)):
+            if t.startswith("("):
+                a, b = t[1:-1].replace("\n", "").split(",")
+                vals.append(complex(float(a), float(b)))
+            else:
+                vals.append(complex(float(t), 0.0))
+        blocks.append((head, np.array(vals).reshape(-1, 10)))
+    return blocks

@@ -12,7 +12,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from movfem_amd import model as M
-from movfem_amd.vfem import GpuVfem
+from movfem_amd.vfem import GpuVfem, MvfError
 from oracle.oracle import Oracle
 from tests.test_oracle_readout import CASES, air_layers, receivers_for
 
@@ -54,6 +54,25 @@ def test_readout_matches_oracle(name, need_gpu):
     ke_top = m.nz - air_layers(m)
     out = g.readout(x, ke_top=ke_top, rx=xs, ry=ys)
     compare(m, x, out[0], out[1], out[2:], xs, ys, ke_top)
+
+
+def test_edge_receivers(need_gpu):
+    """Sites on and beyond the inner cells' edges inherit the previous
+    receiver's element, as the reference's rx_data does (the oracle's rule is
+    pinned to the reference build in test_oracle_readout.py)."""
+    from tests.test_oracle_readout import edge_sites
+
+    m = CASES["hex8_dirichlet"]()
+    print("edge receivers", flush=True)
+    g = GpuVfem(m)
+    g.set_frequency(0)
+    x = host_solution(g)
+    xs, ys = edge_sites(m)
+    ke_top = m.nz - air_layers(m)
+    out = g.readout(x, ke_top=ke_top, rx=xs, ry=ys)
+    compare(m, x, out[0], out[1], out[2:], xs, ys, ke_top)
+    with pytest.raises(MvfError):
+        g.readout(x, ke_top=ke_top, rx=xs[[3, 0]], ry=ys[[3, 0]])
 
 
 def test_readout_after_device_solve(need_gpu):

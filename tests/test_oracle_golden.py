@@ -9,10 +9,11 @@ import pytest
 from oracle.oracle import Oracle
 from tests.golden.make_golden import load_model
 
-# model fixtures (out_*.npz hold output files, tests/test_output_files.py;
-# mesh_*.npz the lattice generator's, tests/test_mesh.py)
+# model fixtures (out_*.npz and ld_reals.npz hold output files and printed
+# reals, tests/test_output_files.py; mesh_*.npz the lattice generator's,
+# tests/test_mesh.py)
 GOLD = sorted(p for p in (pathlib.Path(__file__).parent / "golden").glob("*.npz")
-              if not p.stem.startswith(("out_", "mesh_")))
+              if not p.stem.startswith(("out_", "mesh_", "ld_")))
 
 
 def _rel(a, b):

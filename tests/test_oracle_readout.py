@@ -87,3 +87,41 @@ def test_readout_matches_reference(name):
     assert np.allclose(z, r["rx_z"], rtol=2e-5, atol=1e-5 * np.abs(r["rx_z"]).max())
     assert np.allclose(rho, r["rx_rho"], rtol=1e-5, atol=2e-5)
     assert np.allclose(phi, r["rx_phi"], rtol=1e-5, atol=2e-5)
+
+
+def edge_sites(m):
+    """Sites on the inner cells' lower and upper edges and beyond them, after
+    one inside."""
+    q = m.nord - 1
+    xa, xb = m.xp[m.nextd * q], m.xp[(m.nx - m.nextd) * q]
+    ya, yb = m.yp[m.nextd * q], m.yp[(m.ny - m.nextd) * q]
+    xm, ym = 0.5 * (xa + xb) + 1.0, 0.5 * (ya + yb) + 1.0
+    return (np.array([xm, xa, xb, xb + 300.0, xm, xm, xa - 200.0, xm]),
+            np.array([ym, ym, ym, ya, yb, ya - 50.0, yb + 10.0, ya]))
+
+
+@needs_ref
+def test_edge_receivers_match_reference():
+    """Sites on and beyond the inner cells' edges: get_rx_element finds no
+    cell for x (then y is not searched) or for y, and rx_data goes on with the
+    previous receiver's indices, extrapolating from that element (the
+    reference's example input has such sites at its model's edges)."""
+    from oracle.refrun import run_reference
+
+    m = CASES["hex8_dirichlet"]()
+    o = Oracle(m)
+    o.set_frequency(0)
+    x = solve_oracle(o)
+    es, hs = o.node_solution(x)
+    xs, ys = edge_sites(m)
+    nza = air_layers(m)
+    nzl_a, nzl_b = m.nzl_top, nza - m.nzl_top
+    r = run_reference(m, vfem_freqs=[1], readout_x=x, receivers={"xy": np.stack([xs, ys], 1), "nzl": [nzl_a, nzl_b]})
+    e, h, z, rho, phi = o.receivers(es, hs, m.nz - nza, xs, ys)
+    assert np.allclose(e, r["rx_e"], rtol=2e-5, atol=1e-5 * np.abs(r["rx_e"]).max())
+    assert np.allclose(h, r["rx_h"], rtol=2e-5, atol=1e-5 * np.abs(r["rx_h"]).max())
+    assert np.allclose(z, r["rx_z"], rtol=2e-5, atol=1e-5 * np.abs(r["rx_z"]).max())
+    assert np.allclose(rho, r["rx_rho"], rtol=1e-5, atol=2e-5)
+    # the first site outside every inner cell has nothing to inherit
+    with pytest.raises(RuntimeError):
+        o.receivers(es, hs, m.nz - nza, xs[[3, 0]], ys[[3, 0]])
