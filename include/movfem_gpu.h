@@ -162,7 +162,8 @@ typedef struct mvf_solver_opts {
   int32_t gmres_m;      /* GMRES restart length (MVF_PC_DIRECT), 0 = 30 */
   int32_t freqs_per_factor; /* MVF_PC_DIRECT in mvf_solve_frequencies: consecutive frequencies sharing one
                                factorization (at their geometric-mean frequency), 0 or 1 = one each */
-  int32_t direct_tile;  /* MVF_PC_DIRECT: GEMM tile of the factorization, 0 = by front size, 64 or 128 = forced */
+  int32_t direct_tile;  /* MVF_PC_DIRECT: GEMM tiles of the factorization, 0 = by front size, 64 = 64 x 64
+                           (four real products per complex one), 128 = 128 x 64 (three) forced */
 } mvf_solver_opts;
 
 typedef struct mvf_solve_info {

@@ -107,9 +107,11 @@ struct GemmTimer {
   static constexpr int kRing = 512;
   hipEvent_t ev[kRing][2];
   double fl[kRing];
+  int md[kRing];
   int used = 0;
   double ms = 0.0, flops = 0.0;
   int64_t launches = 0;
+  double mode_ms[6] = {0, 0, 0, 0, 0, 0}, mode_fl[6] = {0, 0, 0, 0, 0, 0};  // per GEMM mode
   void init() {
     for (int i = 0; i < kRing; i++) {
       (void)hipEventCreate(&ev[i][0]);
@@ -126,12 +128,15 @@ struct GemmTimer {
         ms += t;
         flops += fl[i];
         launches++;
+        mode_ms[md[i]] += t;
+        mode_fl[md[i]] += fl[i];
       }
     }
     for (int i = count; i < used; i++) {
       std::swap(ev[i - count][0], ev[i][0]);
       std::swap(ev[i - count][1], ev[i][1]);
       fl[i - count] = fl[i];
+      md[i - count] = md[i];
     }
     used -= count;
   }
@@ -829,57 +834,62 @@ __device__ __forceinline__ void mfma_tiles(const double2 *A, int64_t lda, const 
   }
 }
 
-// C -= A B over 128x128 complex tiles (large fronts): eight waves of 64x32,
-// each 4x2 blocks of 16x16 accumulated in the matrix cores (128 accumulator
-// registers per lane, two waves per SIMD), 16-deep K chunks staged through
-// two LDS buffers with the next chunk's global loads in flight while the
-// current one is consumed.  A is staged transposed (k-major) so that both
-// operands are read as 256-byte rows per 16-lane group.  Blocks are
-// renumbered so that each XCD (blocks are dealt round-robin to the 8 XCDs)
-// walks a contiguous range of tiles and shares the A row panels in its L2.
-constexpr int kBig = 128;
+// C -= A B over 128 x 64 complex tiles (large fronts) with three real
+// products per complex one (Gauss): t1 = Ar Br, t2 = Ai Bi, t3 = (Ar + Ai)
+// (Br + Bi), Re = t1 - t2, Im = t3 - t1 - t2: three quarters of the matrix-core
+// work of the direct form.  The imaginary part's rounding is relative to
+// (|Ar| + |Ai|)(|Br| + |Bi|) rather than |Ar||Bi| + |Ai||Br|; the factor only
+// preconditions GMRES, whose iteration counts it leaves unchanged within one.
+// Eight waves of 32 x 32 (2 x 2 blocks of 16 x 16, three accumulators each:
+// 96 registers per lane, two waves per SIMD), 16-deep K chunks through two
+// LDS buffers with the next chunk's global loads in flight.  A is staged
+// transposed (k-major) so that both operands are read as 256-byte rows per
+// 16-lane group.  Blocks are renumbered so that each XCD (blocks are dealt
+// round-robin to the 8 XCDs) walks a contiguous range of tiles and shares the
+// A row panels in its L2.
+constexpr int kBig = 128;        // tile rows of the large-front GEMM
+constexpr int kBigN = 64;        // tile columns
 constexpr int kBigThreads = 512;
-__device__ __forceinline__ void mfma_tiles128(const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
-                                              double2 *C, int64_t ldc, int64_t Mr, int64_t Nc, int64_t Kd,
-                                              bool lower) {
-  const int64_t tn = (Nc + kBig - 1) / kBig, tm = (Mr + kBig - 1) / kBig;
+__device__ __forceinline__ void mfma_tiles3m(const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                                             double2 *C, int64_t ldc, int64_t Mr, int64_t Nc, int64_t Kd,
+                                             bool lower) {
+  const int64_t tn = (Nc + kBigN - 1) / kBigN, tm = (Mr + kBig - 1) / kBig;
   __shared__ double2 As[2][16][kBig + 1];
-  __shared__ double2 Bs[2][16][kBig];
+  __shared__ double2 Bs[2][16][kBigN];
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
-  const int wr = (wv >> 2) * 64, wc = (wv & 3) * 32;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   const int li = l & 15, lk = l >> 4;
   const unsigned G = gridDim.x;
   const int64_t b0 = (G % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
-  // global -> register staging: A rows ar + 32 j at column aq; B rows bq + 4 j at column bc
-  const int aq = tid & 15, ar = tid >> 4, bc = tid & 127, bq = tid >> 7;
+  // global -> register staging: A rows ar + 32 j at column aq; B rows bq + 8 j at column bc
+  const int aq = tid & 15, ar = tid >> 4, bc = tid & 63, bq = tid >> 6;
   for (int64_t tile = b0; tile < tm * tn; tile += G) {
     const int64_t ti = tile / tn, tj = tile % tn;
-    if (lower && tj > ti) continue;
-    const int64_t i0 = ti * kBig, j0 = tj * kBig;
-    mvf_d4 cre[4][2], cim[4][2];
+    const int64_t i0 = ti * kBig, j0 = tj * kBigN;
+    if (lower && j0 > i0 + kBig - 1) continue;  // wholly above the diagonal
+    mvf_d4 t1[2][2], t2[2][2], t3[2][2];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < 2; r++)
 #pragma unroll
-      for (int c = 0; c < 2; c++) {
-        cre[r][c] = mvf_d4{0.0, 0.0, 0.0, 0.0};
-        cim[r][c] = mvf_d4{0.0, 0.0, 0.0, 0.0};
-      }
-    double2 ra[4], rb[4];
+      for (int c = 0; c < 2; c++) t1[r][c] = t2[r][c] = t3[r][c] = mvf_d4{0.0, 0.0, 0.0, 0.0};
+    double2 ra[4], rb[2];
     auto fetch = [&](int64_t kk) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int64_t r = i0 + ar + 32 * j, q = kk + aq;
         ra[j] = (r < Mr && q < Kd) ? A[r * lda + q] : make_double2(0.0, 0.0);
-        const int64_t qb = kk + bq + 4 * j, c = j0 + bc;
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int64_t qb = kk + bq + 8 * j, c = j0 + bc;
         rb[j] = (qb < Kd && c < Nc) ? B[qb * ldb + c] : make_double2(0.0, 0.0);
       }
     };
     auto stash = [&](int buf) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        As[buf][aq][ar + 32 * j] = ra[j];
-        Bs[buf][bq + 4 * j][bc] = rb[j];
-      }
+      for (int j = 0; j < 4; j++) As[buf][aq][ar + 32 * j] = ra[j];
+#pragma unroll
+      for (int j = 0; j < 2; j++) Bs[buf][bq + 8 * j][bc] = rb[j];
     };
     const int64_t nch = (Kd + 15) / 16;
     fetch(0);
@@ -890,31 +900,32 @@ __device__ __forceinline__ void mfma_tiles128(const double2 *A, int64_t lda, con
       if (ch + 1 < nch) fetch((ch + 1) * 16);
 #pragma unroll
       for (int ks = 0; ks < 16; ks += 4) {
-        double2 a[4], b[2];
+        double2 a[2], b[2];
+        double sa[2], sb[2];
 #pragma unroll
-        for (int r = 0; r < 4; r++) a[r] = As[buf][ks + lk][wr + r * 16 + li];
+        for (int r = 0; r < 2; r++) {
+          a[r] = As[buf][ks + lk][wr + r * 16 + li];
+          sa[r] = a[r].x + a[r].y;
+        }
 #pragma unroll
-        for (int c = 0; c < 2; c++) b[c] = Bs[buf][ks + lk][wc + c * 16 + li];
+        for (int c = 0; c < 2; c++) {
+          b[c] = Bs[buf][ks + lk][wc + c * 16 + li];
+          sb[c] = b[c].x + b[c].y;
+        }
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int r = 0; r < 2; r++)
 #pragma unroll
           for (int c = 0; c < 2; c++) {
-            cre[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, cre[r][c], 0, 0, 0);
-            cim[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].y, cim[r][c], 0, 0, 0);
-          }
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-          for (int c = 0; c < 2; c++) {
-            cre[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[r].y, b[c].y, cre[r][c], 0, 0, 0);
-            cim[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].x, cim[r][c], 0, 0, 0);
+            t1[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, t1[r][c], 0, 0, 0);
+            t2[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, t2[r][c], 0, 0, 0);
+            t3[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[r], sb[c], t3[r][c], 0, 0, 0);
           }
       }
       if (ch + 1 < nch) stash(buf ^ 1);
       __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < 2; r++)
 #pragma unroll
       for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -922,8 +933,8 @@ __device__ __forceinline__ void mfma_tiles128(const double2 *A, int64_t lda, con
           const int64_t i = i0 + wr + r * 16 + lk + 4 * v, j = j0 + wc + c * 16 + li;
           if (i < Mr && j < Nc) {
             double2 &o = C[i * ldc + j];
-            o.x -= cre[r][c][v];
-            o.y -= cim[r][c][v];
+            o.x -= t1[r][c][v] - t2[r][c][v];
+            o.y -= t3[r][c][v] - t1[r][c][v] - t2[r][c][v];
           }
         }
   }
@@ -1022,7 +1033,7 @@ __global__ __launch_bounds__(BIG ? kBigThreads : 256, 2) void k_fa_gemm_mfma(con
   const int64_t m = nd.m;
   double2 *T = fac + nd.fac;
   double2 *AC = (MODE == 2) ? pool + nd.pool : T;  // the Schur update reads F21 and writes F22 in the pool band
-  if constexpr (BIG) mfma_tiles128(AC + g.a, m, T + g.b, m, AC + g.c, m, g.Mr, g.Nc, g.Kd, MODE == 2);
+  if constexpr (BIG) mfma_tiles3m(AC + g.a, m, T + g.b, m, AC + g.c, m, g.Mr, g.Nc, g.Kd, MODE == 2);
   else mfma_tiles<false>(AC + g.a, m, T + g.b, m, AC + g.c, m, g.Mr, g.Nc, g.Kd, MODE == 2);
 }
 
@@ -1427,9 +1438,10 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   MVF_HIP(h, hipMemsetAsync(P->d_npert, 0, sizeof(int), st));
   GemmTimer *gt_ = new GemmTimer();
   gt_->init();
-  auto gemm_begin = [&](double flops) {
+  auto gemm_begin = [&](double flops, int mode) {
     if (gt_->used == GemmTimer::kRing) gt_->drain(GemmTimer::kRing / 2);
     gt_->fl[gt_->used] = flops;
+    gt_->md[gt_->used] = mode;
     (void)hipEventRecord(gt_->ev[gt_->used][0], st);
   };
   auto gemm_end = [&]() {
@@ -1456,6 +1468,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
     MVF_HIP(h, hipGetLastError());
     const bool big = P->tile ? P->tile == kBig : mm >= kBigFront;
     const int64_t tile = big ? kBig : kTile;
+    const int64_t tile_n = big ? kBigN : kTile;
     if (mn1 > 0) {
       hipLaunchKernelGGL(k_fa_sym, dim3(cap_grid(((mn1 + 31) / 32) * ((mm + 31) / 32)), nl), dim3(256), 0, st,
                          P->d_nodes, lev, P->d_fac, pool);
@@ -1467,12 +1480,12 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
         for (int t : P->levels[d]) {
           const GemmArgs g = gemm_args(MODE, P->nodes[t], step, K0);
           if (!g.ok) continue;
-          gt = std::max(gt, ((g.Mr + tile - 1) / tile) * ((g.Nc + tile - 1) / tile));
+          gt = std::max(gt, ((g.Mr + tile - 1) / tile) * ((g.Nc + tile_n - 1) / tile_n));
           gfl += MODE == 2 ? 8.0 * (double)g.Mr * (double)(g.Mr + 1) / 2.0 * (double)g.Kd
                            : 8.0 * (double)g.Mr * (double)g.Nc * (double)g.Kd;
         }
         if (gt == 0) return;
-        gemm_begin(gfl);
+        gemm_begin(gfl, MODE);
         launch_gemm<MODE>(big, cap_grid(gt), nl, st, P->d_nodes, lev, step, K0, P->d_fac, pool);
         gemm_end();
       };
@@ -1555,6 +1568,11 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   }
   MVF_HIP(h, hipStreamSynchronize(st));
   gt_->destroy();
+  if (h->verbose)
+    for (int q = 0; q < 6; q++)
+      if (gt_->mode_ms[q] > 0)
+        fprintf(stderr, "[direct] GEMM mode %d: %.1f ms, %.3e flop, %.1f TFLOP/s\n", q, gt_->mode_ms[q],
+                gt_->mode_fl[q], gt_->mode_fl[q] / (gt_->mode_ms[q] * 1e-3) / 1e12);
   P->gemm_ms = gt_->ms;
   P->gemm_flops = gt_->flops;
   P->gemm_launches = gt_->launches;
