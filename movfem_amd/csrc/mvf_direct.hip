@@ -844,12 +844,16 @@ __device__ __forceinline__ void mfma_tiles(const double2 *A, int64_t lda, const 
 // 96 registers per lane, two waves per SIMD), 16-deep K chunks through two
 // LDS buffers with the next chunk's global loads in flight.  A is staged
 // transposed (k-major) so that both operands are read as 256-byte rows per
-// 16-lane group.  Blocks are renumbered so that each XCD (blocks are dealt
-// round-robin to the 8 XCDs) walks a contiguous range of tiles and shares the
-// A row panels in its L2.
+// 16-lane group.  With lower set only the tiles that reach the lower triangle
+// are enumerated (row ti holds min(2 ti + 2, tn) of them; this needs Mr == Nc),
+// so no workgroup is spent on tiles above the diagonal.  When every workgroup
+// takes one tile, workgroups are renumbered so that each XCD (workgroups are
+// dealt round-robin to the 8 XCDs) walks an equal contiguous range of the
+// front's tiles and shares the A row panels in its L2.
 constexpr int kBig = 128;        // tile rows of the large-front GEMM
 constexpr int kBigN = 64;        // tile columns
 constexpr int kBigThreads = 512;
+static_assert(kBig == 2 * kBigN, "the lower-tile count assumes two tile columns per tile row");
 __device__ __forceinline__ void mfma_tiles3m(const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                                              double2 *C, int64_t ldc, int64_t Mr, int64_t Nc, int64_t Kd,
                                              bool lower) {
@@ -860,13 +864,26 @@ __device__ __forceinline__ void mfma_tiles3m(const double2 *A, int64_t lda, cons
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   const int li = l & 15, lk = l >> 4;
   const unsigned G = gridDim.x;
-  const int64_t b0 = (G % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int64_t ntiles = lower ? (tm - 1) * tm + tn : tm * tn;
   // global -> register staging: A rows ar + 32 j at column aq; B rows bq + 8 j at column bc
   const int aq = tid & 15, ar = tid >> 4, bc = tid & 63, bq = tid >> 6;
-  for (int64_t tile = b0; tile < tm * tn; tile += G) {
-    const int64_t ti = tile / tn, tj = tile % tn;
+  for (int64_t it = blockIdx.x; it < ntiles; it += G) {
+    int64_t tile = it;
+    if (ntiles <= (int64_t)G) {
+      const int64_t q = ntiles / 8, rem = ntiles % 8, x = it % 8;
+      tile = x * q + (x < rem ? x : rem) + it / 8;
+    }
+    int64_t ti, tj;
+    if (lower) {  // rows before ti hold ti (ti + 1) lower tiles
+      ti = (int64_t)((sqrt(4.0 * (double)tile + 1.0) - 1.0) * 0.5);
+      while (ti > 0 && ti * (ti + 1) > tile) ti--;
+      while (ti + 1 < tm && (ti + 1) * (ti + 2) <= tile) ti++;
+      tj = tile - ti * (ti + 1);
+    } else {
+      ti = tile / tn;
+      tj = tile % tn;
+    }
     const int64_t i0 = ti * kBig, j0 = tj * kBigN;
-    if (lower && j0 > i0 + kBig - 1) continue;  // wholly above the diagonal
     mvf_d4 t1[2][2], t2[2][2], t3[2][2];
 #pragma unroll
     for (int r = 0; r < 2; r++)
@@ -1206,43 +1223,46 @@ __global__ __launch_bounds__(256) void k_sv_lupd(const DNode *nodes, const int32
   }
 }
 
-// back substitution (upper U) inside panel block b = nb-1-step
+// back substitution (upper U) inside panel block b = nb-1-step: one wave per
+// front, lane r owns row r.  The strict upper block is staged column-major in
+// LDS by one batch of independent loads; each step scales row q by its
+// precomputed diagonal reciprocal, broadcasts it across the wave and updates
+// the rows above it, with no barrier inside the substitution.
 template <int NC, int NB>
 __global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32_t *lev, int64_t step,
                                                  const double2 *fac, double2 *zb) {
+  static_assert(NB == 64, "one lane per row of the diagonal block");
   const DNode nd = nodes[lev[blockIdx.x]];
   const int64_t b = (nd.n1 + NB - 1) / NB - 1 - step;
   if (b < 0) return;
   const int64_t r0 = b * NB, r1 = r0 + NB < nd.n1 ? r0 + NB : nd.n1;
   const int w = (int)(r1 - r0);
   const double2 *T = fac + nd.fac;
-  __shared__ double2 U[NB][NB + 1];
-  __shared__ double2 z[NB][NC];
-  for (int e = threadIdx.x; e < NB * NB; e += 64) {
-    const int r = e / NB, c = e % NB;
-    U[r][c] = (r < w && c < w && c >= r) ? T[(r0 + r) * nd.m + r0 + c] : make_double2(0.0, 0.0);
+  __shared__ double2 S[NB][NB + 1];  // S[col][row] = U[row][col], row < col; padded for conflict-free writes
+  const int r = threadIdx.x;
+#pragma unroll 16
+  for (int q = 0; q < NB; q++) {
+    // lane r reads U[q][r] of row q (coalesced)
+    const double2 u = (q < w && r < w && r > q) ? T[(r0 + q) * nd.m + r0 + r] : make_double2(0.0, 0.0);
+    S[r][q] = u;
   }
-  for (int e = threadIdx.x; e < NB * NC; e += 64) {
-    const int r = e / NC, c = e % NC;
-    z[r][c] = r < w ? zb[(nd.pos0 + r0 + r) * NC + c] : make_double2(0.0, 0.0);
-  }
+  const double2 dinv = r < w ? cinv(T[(r0 + r) * nd.m + r0 + r]) : make_double2(0.0, 0.0);
+  double2 z[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) z[c] = r < w ? zb[(nd.pos0 + r0 + r) * NC + c] : make_double2(0.0, 0.0);
   __syncthreads();
   for (int q = w - 1; q >= 0; q--) {
-    for (int e = threadIdx.x; e < NB * NC; e += 64) {
-      const int r = e % NB, c = e / NB;
-      if (r == q) z[r][c] = cmul(z[r][c], cinv(U[q][q]));
+    const double2 u = S[q][r];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      if (r == q) z[c] = cmul(z[c], dinv);
+      const double2 zq = make_double2(__shfl(z[c].x, q, 64), __shfl(z[c].y, q, 64));
+      if (r < q) cfms(z[c], u, zq);
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NB * NC; e += 64) {
-      const int r = e % NB, c = e / NB;
-      if (r < q) cfms(z[r][c], U[r][q], z[q][c]);
-    }
-    __syncthreads();
   }
-  for (int e = threadIdx.x; e < w * NC; e += 64) {
-    const int r = e / NC, c = e % NC;
-    zb[(nd.pos0 + r0 + r) * NC + c] = z[r][c];
-  }
+  if (r < w)
+#pragma unroll
+    for (int c = 0; c < NC; c++) zb[(nd.pos0 + r0 + r) * NC + c] = z[c];
 }
 
 // rows above block b = nb-1-step: z[r] -= U[r][r0:r1] z[r0:r1]
@@ -1480,7 +1500,9 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
         for (int t : P->levels[d]) {
           const GemmArgs g = gemm_args(MODE, P->nodes[t], step, K0);
           if (!g.ok) continue;
-          gt = std::max(gt, ((g.Mr + tile - 1) / tile) * ((g.Nc + tile_n - 1) / tile_n));
+          const int64_t tm = (g.Mr + tile - 1) / tile, tn = (g.Nc + tile_n - 1) / tile_n;
+          // the large-front Schur update enumerates its lower tiles only
+          gt = std::max(gt, (MODE == 2 && big) ? (tm - 1) * tm + tn : tm * tn);
           gfl += MODE == 2 ? 8.0 * (double)g.Mr * (double)(g.Mr + 1) / 2.0 * (double)g.Kd
                            : 8.0 * (double)g.Mr * (double)g.Nc * (double)g.Kd;
         }
