@@ -439,6 +439,14 @@ __device__ __forceinline__ void cfmac(double2 &c, double2 a, double2 b) {
   c.x += a.x * b.x + a.y * b.y;
   c.y += a.x * b.y - a.y * b.x;
 }
+__device__ __forceinline__ int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
+// value of v in lane q (q the same in every lane) through two scalar reads
+__device__ __forceinline__ double lane_bcast(double v, int q) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, q);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), q);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 __device__ __forceinline__ double2 cinv(double2 a) {
   const double d = a.x * a.x + a.y * a.y;
   return make_double2(a.x / d, -a.y / d);
@@ -488,7 +496,27 @@ __global__ __launch_bounds__(256) void k_fa_extend(const DNode *nodes, const int
     const int64_t pi = rel[cn.rel + i];
     const double2 *src = cpool + cn.pool + i * cn.m + cn.n1;
     double2 *drow = pi < nd.n1 ? fac + nd.fac + pi * nd.m : pool + nd.pool + (pi - nd.n1) * nd.m;
-    for (int64_t j = lane; j <= i; j += 64) {
+    // four columns per lane in flight together; the destinations of one row
+    // are distinct, so the batched read-modify-writes do not overlap
+    int64_t j = lane;
+    for (; j + 192 <= i; j += 256) {
+      int32_t pj[4];
+      double2 v[4], d[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        pj[u] = rel[cn.rel + j + 64 * u];
+        v[u] = src[j + 64 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) d[u] = drow[pj[u]];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        d[u].x += v[u].x;
+        d[u].y += v[u].y;
+        drow[pj[u]] = d[u];
+      }
+    }
+    for (; j <= i; j += 64) {
       const int32_t pj = rel[cn.rel + j];
       const double2 v = src[j];
       drow[pj].x += v.x;
@@ -531,6 +559,7 @@ __global__ __launch_bounds__(256) void k_fa_sym(const DNode *nodes, const int32_
 template <int NB>
 __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
                                                  double2 *inv, int32_t *piv, int *npert, double big, double tiny) {
+  static_assert(NB == 64, "one lane per column of the diagonal block");
   const DNode nd = nodes[lev[blockIdx.x]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
@@ -539,9 +568,15 @@ __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32
   __shared__ double2 a[NB][NB + 1];
   __shared__ int sp;
   const int t = threadIdx.x;
-  for (int e = t; e < w * w; e += 64) {
-    const int r = e / w, c = e % w;
-    a[r][c] = T[(k + r) * m + k + c];
+  // lane t reads column t of every row: independent coalesced loads from
+  // clamped addresses, padding selected afterwards
+  {
+    const int tc = t < w ? t : w - 1;
+#pragma unroll 16
+    for (int r = 0; r < NB; r++) {
+      const double2 v = T[(k + (r < w ? r : w - 1)) * m + k + tc];
+      a[r][t] = (r < w && t < w) ? v : make_double2(0.0, 0.0);
+    }
   }
   __syncthreads();
   for (int q = 0; q < w; q++) {
@@ -578,32 +613,60 @@ __global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32
     }
     __syncthreads();
     const double2 inv = cinv(a[q][q]);
-    if (t > q && t < w) a[t][q] = cmul(a[t][q], inv);
-    __syncthreads();
-    if (t > q && t < w)
-      for (int c = q + 1; c < w; c++) cfms(a[t][c], a[t][q], a[q][c]);
+    if (t > q && t < w) {
+      const double2 l = cmul(a[t][q], inv);
+      a[t][q] = l;
+      // row t -= l * row q, eight columns per batch: the batch's LDS reads are
+      // issued together instead of each waiting behind the previous write
+      for (int c0 = q + 1; c0 < w; c0 += 8) {
+        double2 x[8], y[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (c0 + j < w) {
+            x[j] = a[t][c0 + j];
+            y[j] = a[q][c0 + j];
+          }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (c0 + j < w) {
+            cfms(x[j], l, y[j]);
+            a[t][c0 + j] = x[j];
+          }
+      }
+    }
     __syncthreads();
   }
-  for (int e = t; e < w * w; e += 64) {
-    const int r = e / w, c = e % w;
-    T[(k + r) * m + k + c] = a[r][c];
-  }
+#pragma unroll 16
+  for (int r = 0; r < NB; r++)
+    if (r < w && t < w) T[(k + r) * m + k + t] = a[r][t];
   // inverse of the unit lower L_kk (thread t owns column t).  L's multipliers
   // are bounded by 1, so the explicit inverse is safe; U_kk can hold pivots
   // ~1e-11 of the largest (the air shift) and is only ever used by
   // substitution.
+  // Row r of X = L_kk^-1 is delta(r, t) - sum_{p<r} L[r][p] X[p][t]: every lane
+  // runs the same r and p (X[p][t] = 0 for p < t adds exact zeros), reading
+  // L[r][p] as a broadcast, eight terms per batch.
   __shared__ double2 x[NB][NB + 1];
   double2 *Li = inv + nd.inv + (int64_t)NB * NB * (k / NB);
-  if (t < w) {
-    for (int r = 0; r < w; r++) x[r][t] = make_double2(r == t ? 1.0 : 0.0, 0.0);
-    for (int r = t + 1; r < w; r++) {
-      double2 v = make_double2(0.0, 0.0);
-      for (int p = t; p < r; p++) cfms(v, a[r][p], x[p][t]);
-      x[r][t] = v;
+  for (int r = 0; r < w; r++) {
+    double2 v = make_double2(r == t ? 1.0 : 0.0, 0.0);
+    int p = 0;
+    for (; p + 8 <= r; p += 8) {
+      double2 l[8], y[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        l[j] = a[r][p + j];
+        y[j] = x[p + j][t];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) cfms(v, l[j], y[j]);
     }
+    for (; p < r; p++) cfms(v, a[r][p], x[p][t]);
+    x[r][t] = v;
   }
-  __syncthreads();
-  for (int e = t; e < w * w; e += 64) Li[e] = x[e / w][e % w];
+#pragma unroll 16
+  for (int r = 0; r < NB; r++)
+    if (r < w && t < w) Li[r * w + t] = x[r][t];
 }
 
 constexpr int kTrsmThreads = 256;
@@ -784,15 +847,23 @@ __device__ __forceinline__ void mfma_tiles(const double2 *A, int64_t lda, const 
         cim[a][b] = mvf_d4{0.0, 0.0, 0.0, 0.0};
       }
     for (int64_t kk = 0; kk < Kd; kk += 16) {
+      // every load is issued (from a clamped in-bounds address) and the
+      // padding is selected afterwards, so that the eight loads are in flight
+      // together rather than each behind a branch and a wait
+      double2 va[kTile * 16 / 256], vb[kTile * 16 / 256];
 #pragma unroll
-      for (int e = tid; e < kTile * 16; e += 256) {
-        const int r = e >> 4, q = e & 15;
-        As[r][q] = (i0 + r < Mr && kk + q < Kd) ? A[(i0 + r) * lda + kk + q] : make_double2(0.0, 0.0);
+      for (int j = 0; j < kTile * 16 / 256; j++) {
+        const int e = tid + 256 * j, r = e >> 4, q = e & 15;
+        va[j] = A[imin(i0 + r, Mr - 1) * lda + imin(kk + q, Kd - 1)];
+        const int qb = e >> 6, c = e & 63;
+        vb[j] = B[imin(kk + qb, Kd - 1) * ldb + imin(j0 + c, Nc - 1)];
       }
 #pragma unroll
-      for (int e = tid; e < kTile * 16; e += 256) {
-        const int q = e >> 6, c = e & 63;
-        Bs[q][c] = (kk + q < Kd && j0 + c < Nc) ? B[(kk + q) * ldb + j0 + c] : make_double2(0.0, 0.0);
+      for (int j = 0; j < kTile * 16 / 256; j++) {
+        const int e = tid + 256 * j, r = e >> 4, q = e & 15;
+        As[r][q] = (i0 + r < Mr && kk + q < Kd) ? va[j] : make_double2(0.0, 0.0);
+        const int qb = e >> 6, c = e & 63;
+        Bs[qb][c] = (kk + qb < Kd && j0 + c < Nc) ? vb[j] : make_double2(0.0, 0.0);
       }
       __syncthreads();
 #pragma unroll
@@ -815,22 +886,34 @@ __device__ __forceinline__ void mfma_tiles(const double2 *A, int64_t lda, const 
       __syncthreads();
     }
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+    for (int rb = 0; rb < 2; rb++) {
+      // C reads of one row block issued together (clamped addresses), then the updates
+      double2 cv[2][4];
+      if (!OVERWRITE)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int64_t i = i0 + wr + rb * 16 + lk + 4 * v, j = j0 + wc + cb * 16 + li;
+            cv[cb][v] = C[imin(i, Mr - 1) * ldc + imin(j, Nc - 1)];
+          }
 #pragma unroll
       for (int cb = 0; cb < 2; cb++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int64_t i = i0 + wr + rb * 16 + lk + 4 * v, j = j0 + wc + cb * 16 + li;
           if (i < Mr && j < Nc) {
-            double2 &o = C[i * ldc + j];
             if (OVERWRITE) {
-              o = make_double2(cre[rb][cb][v], cim[rb][cb][v]);
+              C[i * ldc + j] = make_double2(cre[rb][cb][v], cim[rb][cb][v]);
             } else {
+              double2 o = cv[cb][v];
               o.x -= cre[rb][cb][v];
               o.y -= cim[rb][cb][v];
+              C[i * ldc + j] = o;
             }
           }
         }
+    }
   }
 }
 
@@ -942,18 +1025,29 @@ __device__ __forceinline__ void mfma_tiles3m(const double2 *A, int64_t lda, cons
       __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 2; r++)
+    for (int r = 0; r < 2; r++) {
+      // C reads of one row block issued together (clamped addresses), then the updates
+      double2 cv[2][4];
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int64_t i = i0 + wr + r * 16 + lk + 4 * v, j = j0 + wc + c * 16 + li;
+          cv[c][v] = C[imin(i, Mr - 1) * ldc + imin(j, Nc - 1)];
+        }
 #pragma unroll
       for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int64_t i = i0 + wr + r * 16 + lk + 4 * v, j = j0 + wc + c * 16 + li;
           if (i < Mr && j < Nc) {
-            double2 &o = C[i * ldc + j];
+            double2 o = cv[c][v];
             o.x -= t1[r][c][v] - t2[r][c][v];
             o.y -= t3[r][c][v] - t1[r][c][v] - t2[r][c][v];
+            C[i * ldc + j] = o;
           }
         }
+    }
   }
 }
 
@@ -1141,7 +1235,17 @@ __global__ __launch_bounds__(256) void k_sv_contrib(const DNode *nodes, const in
       __syncthreads();
       if (j < nd.n2) {
         const int64_t lim = (nd.n1 - i0) < 256 ? nd.n1 - i0 : 256;
-        for (int64_t r = 0; r < lim; r++) {
+        int64_t r = 0;
+        for (; r + 8 <= lim; r += 8) {  // eight rows' loads in flight together
+          double2 y[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) y[u] = T[(i0 + r + u) * nd.m + nd.n1 + j];
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) cfma(acc[c], y[u], s[r + u][c]);
+        }
+        for (; r < lim; r++) {
           const double2 y = T[(i0 + r) * nd.m + nd.n1 + j];
 #pragma unroll
           for (int c = 0; c < NC; c++) cfma(acc[c], y, s[r][c]);
@@ -1240,11 +1344,12 @@ __global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32
   const double2 *T = fac + nd.fac;
   __shared__ double2 S[NB][NB + 1];  // S[col][row] = U[row][col], row < col; padded for conflict-free writes
   const int r = threadIdx.x;
+  const int rc = r < w ? r : w - 1;
 #pragma unroll 16
   for (int q = 0; q < NB; q++) {
-    // lane r reads U[q][r] of row q (coalesced)
-    const double2 u = (q < w && r < w && r > q) ? T[(r0 + q) * nd.m + r0 + r] : make_double2(0.0, 0.0);
-    S[r][q] = u;
+    // lane r reads U[q][r] of row q (coalesced, clamped address, padding selected afterwards)
+    const double2 u = T[(r0 + (q < w ? q : w - 1)) * nd.m + r0 + rc];
+    S[r][q] = (q < w && r < w && r > q) ? u : make_double2(0.0, 0.0);
   }
   const double2 dinv = r < w ? cinv(T[(r0 + r) * nd.m + r0 + r]) : make_double2(0.0, 0.0);
   double2 z[NC];
@@ -1256,7 +1361,7 @@ __global__ __launch_bounds__(64) void k_sv_udiag(const DNode *nodes, const int32
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       if (r == q) z[c] = cmul(z[c], dinv);
-      const double2 zq = make_double2(__shfl(z[c].x, q, 64), __shfl(z[c].y, q, 64));
+      const double2 zq = make_double2(lane_bcast(z[c].x, q), lane_bcast(z[c].y, q));
       if (r < q) cfms(z[c], u, zq);
     }
   }
