@@ -557,116 +557,137 @@ __global__ __launch_bounds__(256) void k_fa_sym(const DNode *nodes, const int32_
 // restricted to the block's rows (kend - k <= NB); one workgroup per front.
 // A pivot below `tiny` (a singular block) is replaced by `big` and counted.
 template <int NB>
-__global__ __launch_bounds__(64) void k_fa_pdiag(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
-                                                 double2 *inv, int32_t *piv, int *npert, double big, double tiny) {
-  static_assert(NB == 64, "one lane per column of the diagonal block");
+__global__ __launch_bounds__(256) void k_fa_pdiag(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac,
+                                                  double2 *inv, int32_t *piv, int *npert, double big, double tiny) {
+  // Four waves: lane t of every wave stands for row t, wave g for the columns
+  // c = g (mod 4) of the rank-1 updates and of the inverse's updates.  The
+  // pivot search runs on wave 0.
+  static_assert(NB == 64, "one lane per row of the diagonal block");
   const DNode nd = nodes[lev[blockIdx.x]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
   const int w = (int)(kend - k);
   double2 *T = fac + nd.fac;
   __shared__ double2 a[NB][NB + 1];
+  __shared__ double2 x[NB][NB + 1];
   __shared__ int sp;
-  const int t = threadIdx.x;
-  // lane t reads column t of every row: independent coalesced loads from
-  // clamped addresses, padding selected afterwards
+  const int tid = threadIdx.x, t = tid & 63, g = tid >> 6;
+  // wave g reads rows g, g + 4, ...; lane t column t (coalesced, clamped
+  // addresses, padding selected afterwards)
   {
     const int tc = t < w ? t : w - 1;
-#pragma unroll 16
-    for (int r = 0; r < NB; r++) {
+#pragma unroll
+    for (int j = 0; j < NB / 4; j++) {
+      const int r = g + 4 * j;
       const double2 v = T[(k + (r < w ? r : w - 1)) * m + k + tc];
       a[r][t] = (r < w && t < w) ? v : make_double2(0.0, 0.0);
     }
   }
   __syncthreads();
   for (int q = 0; q < w; q++) {
-    // pivot: largest |a[r][q]|, r in [q, w); one wave, ties to the lower row
+    // pivot: largest |a[r][q]|, r in [q, w); ties to the lower row
     double v = -1.0;
-    int ri = q;
-    if (t >= q && t < w) {
-      v = a[t][q].x * a[t][q].x + a[t][q].y * a[t][q].y;
-      ri = t;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(v, off, 64);
-      const int oi = __shfl_xor(ri, off, 64);
-      if (ov > v || (ov == v && oi < ri)) {
-        v = ov;
-        ri = oi;
+    if (g == 0) {
+      int ri = q;
+      if (t >= q && t < w) {
+        v = a[t][q].x * a[t][q].x + a[t][q].y * a[t][q].y;
+        ri = t;
       }
-    }
-    if (t == 0) {
-      sp = ri;
-      piv[nd.pos0 + k + q] = (int32_t)(k + ri);
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(ri, off, 64);
+        if (ov > v || (ov == v && oi < ri)) {
+          v = ov;
+          ri = oi;
+        }
+      }
+      if (t == 0) {
+        sp = ri;
+        piv[nd.pos0 + k + q] = (int32_t)(k + ri);
+      }
     }
     __syncthreads();
     const int r = sp;
-    if (r != q && t < w) {
-      const double2 x = a[q][t];
-      a[q][t] = a[r][t];
-      a[r][t] = x;
+    if (r != q && tid < w) {
+      const double2 e = a[q][tid];
+      a[q][tid] = a[r][tid];
+      a[r][tid] = e;
     }
     __syncthreads();
-    if (t == 0 && !(v > tiny * tiny)) {
+    if (tid == 0 && !(v > tiny * tiny)) {
       a[q][q] = make_double2(big, 0.0);
       atomicAdd(npert, 1);
     }
     __syncthreads();
-    const double2 inv = cinv(a[q][q]);
-    if (t > q && t < w) {
-      const double2 l = cmul(a[t][q], inv);
-      a[t][q] = l;
-      // row t -= l * row q, eight columns per batch: the batch's LDS reads are
-      // issued together instead of each waiting behind the previous write
-      for (int c0 = q + 1; c0 < w; c0 += 8) {
-        double2 x[8], y[8];
+    // every wave forms the multiplier of its row before wave 0 stores it
+    const bool act = t > q && t < w;
+    const double2 l = act ? cmul(a[t][q], cinv(a[q][q])) : make_double2(0.0, 0.0);
+    __syncthreads();
+    if (act) {
+      if (g == 0) a[t][q] = l;
+      // row t -= l * row q over this wave's columns, four per batch
+      for (int c0 = q + 1 + g; c0 < w; c0 += 16) {
+        double2 xr[4], yr[4];
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-          if (c0 + j < w) {
-            x[j] = a[t][c0 + j];
-            y[j] = a[q][c0 + j];
+        for (int j = 0; j < 4; j++)
+          if (c0 + 4 * j < w) {
+            xr[j] = a[t][c0 + 4 * j];
+            yr[j] = a[q][c0 + 4 * j];
           }
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-          if (c0 + j < w) {
-            cfms(x[j], l, y[j]);
-            a[t][c0 + j] = x[j];
+        for (int j = 0; j < 4; j++)
+          if (c0 + 4 * j < w) {
+            cfms(xr[j], l, yr[j]);
+            a[t][c0 + 4 * j] = xr[j];
           }
       }
     }
     __syncthreads();
   }
-#pragma unroll 16
-  for (int r = 0; r < NB; r++)
+#pragma unroll
+  for (int j = 0; j < NB / 4; j++) {
+    const int r = g + 4 * j;
     if (r < w && t < w) T[(k + r) * m + k + t] = a[r][t];
-  // inverse of the unit lower L_kk (thread t owns column t).  L's multipliers
-  // are bounded by 1, so the explicit inverse is safe; U_kk can hold pivots
-  // ~1e-11 of the largest (the air shift) and is only ever used by
-  // substitution.
-  // Row r of X = L_kk^-1 is delta(r, t) - sum_{p<r} L[r][p] X[p][t]: every lane
-  // runs the same r and p (X[p][t] = 0 for p < t adds exact zeros), reading
-  // L[r][p] as a broadcast, eight terms per batch.
-  __shared__ double2 x[NB][NB + 1];
-  double2 *Li = inv + nd.inv + (int64_t)NB * NB * (k / NB);
-  for (int r = 0; r < w; r++) {
-    double2 v = make_double2(r == t ? 1.0 : 0.0, 0.0);
-    int p = 0;
-    for (; p + 8 <= r; p += 8) {
-      double2 l[8], y[8];
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        l[j] = a[r][p + j];
-        y[j] = x[p + j][t];
-      }
-#pragma unroll
-      for (int j = 0; j < 8; j++) cfms(v, l[j], y[j]);
-    }
-    for (; p < r; p++) cfms(v, a[r][p], x[p][t]);
-    x[r][t] = v;
   }
-#pragma unroll 16
-  for (int r = 0; r < NB; r++)
+  // inverse X of the unit lower L_kk.  L's multipliers are bounded by 1, so
+  // the explicit inverse is safe; U_kk can hold pivots ~1e-11 of the largest
+  // (the air shift) and is only ever used by substitution.  Column form:
+  // X = I, then for p = 0, 1, ...: X[r][c] -= L[r][p] X[p][c] for r > p and
+  // c <= p (row p of X is final once the steps before p are done); lane t
+  // updates row t, wave g the columns c = g (mod 4).
+#pragma unroll
+  for (int j = 0; j < NB / 4; j++) {
+    const int c = g + 4 * j;
+    x[t][c] = make_double2(t == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int p = 0; p + 1 < w; p++) {
+    if (t > p && t < w) {
+      const double2 lp = a[t][p];
+      for (int c0 = g; c0 <= p; c0 += 16) {
+        double2 xr[4], yr[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (c0 + 4 * j <= p) {
+            xr[j] = x[t][c0 + 4 * j];
+            yr[j] = x[p][c0 + 4 * j];
+          }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (c0 + 4 * j <= p) {
+            cfms(xr[j], lp, yr[j]);
+            x[t][c0 + 4 * j] = xr[j];
+          }
+      }
+    }
+    __syncthreads();
+  }
+  double2 *Li = inv + nd.inv + (int64_t)NB * NB * (k / NB);
+#pragma unroll
+  for (int j = 0; j < NB / 4; j++) {
+    const int r = g + 4 * j;
     if (r < w && t < w) Li[r * w + t] = x[r][t];
+  }
 }
 
 constexpr int kTrsmThreads = 256;
@@ -1295,6 +1316,49 @@ __global__ __launch_bounds__(256) void k_sv_ldiag(const DNode *nodes, const int3
   }
 }
 
+// z[r] -= sum_{q<w} M[r][q] zblk[q] for the rows r in [rb, re) of a front,
+// M[r] = M0 + r m: sixteen lanes per row, lane l taking columns l, l + 16,
+// l + 32, l + 48 (each load instruction reads 256 contiguous bytes per row),
+// then a reduction over the sixteen lanes.  zblk rows past w are zero.
+template <int NC, int NB>
+__device__ __forceinline__ void sv_rows_update(const double2 *M0, int64_t m, int w, int64_t rb, int64_t re,
+                                               const double2 (*zblk)[NC], double2 *zrows) {
+  static_assert(NB == 64, "four columns per lane");
+  const int l16 = threadIdx.x & 15, rr = threadIdx.x >> 4;
+  for (int64_t r0 = rb + (int64_t)blockIdx.x * 16; r0 < re; r0 += (int64_t)gridDim.x * 16) {
+    const int64_t r = r0 + rr;
+    double2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = make_double2(0.0, 0.0);
+    if (r < re) {
+      const double2 *Mr = M0 + r * m;
+      double2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = Mr[l16 + 16 * u < w ? l16 + 16 * u : w - 1];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (l16 + 16 * u < w)
+#pragma unroll
+          for (int c = 0; c < NC; c++) cfma(acc[c], v[u], zblk[l16 + 16 * u][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) {
+        acc[c].x += __shfl_xor(acc[c].x, off, 16);
+        acc[c].y += __shfl_xor(acc[c].y, off, 16);
+      }
+    if (r < re)
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        if (l16 == c) {  // lane c of the group writes column c
+          double2 &o = zrows[r * NC + c];
+          o.x -= acc[c].x;
+          o.y -= acc[c].y;
+        }
+  }
+}
+
 // rows below block b: z[r] -= L[r][b0:b1] z[b0:b1]
 template <int NC, int NB>
 __global__ __launch_bounds__(256) void k_sv_lupd(const DNode *nodes, const int32_t *lev, int64_t b, const double2 *fac,
@@ -1304,27 +1368,14 @@ __global__ __launch_bounds__(256) void k_sv_lupd(const DNode *nodes, const int32
   if (b0 >= nd.n1) return;
   const int64_t b1 = b0 + NB < nd.n1 ? b0 + NB : nd.n1;
   const int w = (int)(b1 - b0);
-  if (b1 + (int64_t)blockIdx.x * 256 >= nd.n1) return;
-  const double2 *T = fac + nd.fac;
+  if (b1 + (int64_t)blockIdx.x * 16 >= nd.n1) return;
   __shared__ double2 z[NB][NC];
   for (int e = threadIdx.x; e < NB * NC; e += 256) {
     const int r = e / NC, c = e % NC;
     z[r][c] = r < w ? zb[(nd.pos0 + b0 + r) * NC + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  for (int64_t r = b1 + (int64_t)blockIdx.x * 256 + threadIdx.x; r < nd.n1; r += (int64_t)gridDim.x * 256) {
-    double2 acc[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) acc[c] = zb[(nd.pos0 + r) * NC + c];
-    const double2 *Lr = T + r * nd.m + b0;
-    for (int q = 0; q < w; q++) {
-      const double2 l = Lr[q];
-#pragma unroll
-      for (int c = 0; c < NC; c++) cfms(acc[c], l, z[q][c]);
-    }
-#pragma unroll
-    for (int c = 0; c < NC; c++) zb[(nd.pos0 + r) * NC + c] = acc[c];
-  }
+  sv_rows_update<NC, NB>(fac + nd.fac + b0, nd.m, w, b1, nd.n1, z, zb + nd.pos0 * NC);
 }
 
 // back substitution (upper U) inside panel block b = nb-1-step: one wave per
@@ -1379,27 +1430,14 @@ __global__ __launch_bounds__(256) void k_sv_uupd(const DNode *nodes, const int32
   if (b <= 0) return;
   const int64_t r0 = b * NB, r1 = r0 + NB < nd.n1 ? r0 + NB : nd.n1;
   const int w = (int)(r1 - r0);
-  if ((int64_t)blockIdx.x * 256 >= r0) return;
-  const double2 *T = fac + nd.fac;
+  if ((int64_t)blockIdx.x * 16 >= r0) return;
   __shared__ double2 z[NB][NC];
   for (int e = threadIdx.x; e < NB * NC; e += 256) {
     const int r = e / NC, c = e % NC;
     z[r][c] = r < w ? zb[(nd.pos0 + r0 + r) * NC + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < r0; r += (int64_t)gridDim.x * 256) {
-    double2 acc[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) acc[c] = zb[(nd.pos0 + r) * NC + c];
-    const double2 *Ur = T + r * nd.m + r0;
-    for (int q = 0; q < w; q++) {
-      const double2 u = Ur[q];
-#pragma unroll
-      for (int c = 0; c < NC; c++) cfms(acc[c], u, z[q][c]);
-    }
-#pragma unroll
-    for (int c = 0; c < NC; c++) zb[(nd.pos0 + r) * NC + c] = acc[c];
-  }
+  sv_rows_update<NC, NB>(fac + nd.fac + r0, nd.m, w, 0, r0, z, zb + nd.pos0 * NC);
 }
 
 // backward: x1[i] = y1[i] - sum_j Y[i][j] x2[j]; one wave per row
@@ -1646,7 +1684,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
             gc = std::max(gc, n.m);
             gr = std::max(gr, n.n1 - std::min(k + kNB, n.n1));
           }
-          hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
+          hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
                              P->d_piv, P->d_npert, big, tiny);
           if (gr > 0)
             hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kTrsmThreads - 1) / kTrsmThreads), nl),
@@ -1751,13 +1789,13 @@ static int direct_apply(mvf_handle *h, const double2 *in, double2 *out) {
     for (int64_t b = 0; b < nb; b++) {
       hipLaunchKernelGGL((k_sv_ldiag<NC, kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, b, P->d_inv, zb);
       if ((b + 1) * kNB < mn1)
-        hipLaunchKernelGGL((k_sv_lupd<NC, kNB>), dim3(cap_grid((mn1 - (b + 1) * kNB + 255) / 256), nl), dim3(256), 0,
+        hipLaunchKernelGGL((k_sv_lupd<NC, kNB>), dim3(cap_grid((mn1 - (b + 1) * kNB + 15) / 16), nl), dim3(256), 0,
                            st, P->d_nodes, lev, b, P->d_fac, zb);
     }
     for (int64_t s = 0; s < nb; s++) {
       hipLaunchKernelGGL((k_sv_udiag<NC, kNB>), dim3(nl), dim3(64), 0, st, P->d_nodes, lev, s, P->d_fac, zb);
       if (s + 1 < nb)
-        hipLaunchKernelGGL((k_sv_uupd<NC, kNB>), dim3(cap_grid((mn1 + 255) / 256), nl), dim3(256), 0, st, P->d_nodes,
+        hipLaunchKernelGGL((k_sv_uupd<NC, kNB>), dim3(cap_grid((mn1 + 15) / 16), nl), dim3(256), 0, st, P->d_nodes,
                            lev, s, P->d_fac, zb);
     }
   }
