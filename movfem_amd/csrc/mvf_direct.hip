@@ -90,6 +90,10 @@ struct DirectPlan {
   int *d_npert = nullptr;
   double2 *d_sv = nullptr, *d_zb = nullptr;
   int sv_cols = 0;
+  // side stream (default priority) for the trailing part of the outer-block
+  // updates, with the events that order it against the handle's stream
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // state
   int factored = 0;
   double omega = -1.0;
@@ -1113,13 +1117,26 @@ __host__ __device__ inline GemmArgs gemm_args(int mode, const DNode &nd, int64_t
       g.Nc = m - K1;
     }
   } else if (mode == 4) {
+    // step selects the part: 0 the whole update; with K1n = min(K1 + kOuter, n1),
+    // 1 the next outer block's columns T[K1:n1, K1:K1n], 2 its rows past them
+    // T[K1:K1n, K1n:m], 3 the trailing rest T[K1n:n1, K1n:m]
     if (K0 >= n1) return g;
-    const int64_t K1 = (K0 + kOuter < n1) ? K0 + kOuter : n1;
-    g.a = K1 * m + K0;
-    g.b = K0 * m + K1;
-    g.c = K1 * m + K1;
-    g.Mr = n1 - K1;
-    g.Nc = m - K1;
+    const int64_t K1 = (K0 + kOuter < n1) ? K0 + kOuter : n1, K1n = (K1 + kOuter < n1) ? K1 + kOuter : n1;
+    int64_t r0 = K1, r1 = n1, c0 = K1, c1 = m;
+    if (step == 1) {
+      c1 = K1n;
+    } else if (step == 2) {
+      r1 = K1n;
+      c0 = K1n;
+    } else if (step == 3) {
+      r0 = K1n;
+      c0 = K1n;
+    }
+    g.a = r0 * m + K0;
+    g.b = K0 * m + c0;
+    g.c = r0 * m + c0;
+    g.Mr = r1 - r0;
+    g.Nc = c1 - c0;
     g.Kd = K1 - K0;
   } else if (mode == 1 || mode == 5) {
     const int64_t nbk = (n1 + kNB - 1) / kNB, bb = nbk - 1 - step;
@@ -1493,6 +1510,9 @@ void direct_free(mvf_handle *h) {
   dfree(P->d_sv);
   dfree(P->d_zb);
   dfree(P->d_arena);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  if (P->st2) (void)hipStreamDestroy(P->st2);
   delete P;
   h->direct = nullptr;
 }
@@ -1601,16 +1621,21 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   MVF_HIP(h, hipMemsetAsync(P->d_npert, 0, sizeof(int), st));
   GemmTimer *gt_ = new GemmTimer();
   gt_->init();
-  auto gemm_begin = [&](double flops, int mode) {
+  auto gemm_begin = [&](double flops, int mode, hipStream_t s) {
     if (gt_->used == GemmTimer::kRing) gt_->drain(GemmTimer::kRing / 2);
     gt_->fl[gt_->used] = flops;
     gt_->md[gt_->used] = mode;
-    (void)hipEventRecord(gt_->ev[gt_->used][0], st);
+    (void)hipEventRecord(gt_->ev[gt_->used][0], s);
   };
-  auto gemm_end = [&]() {
-    (void)hipEventRecord(gt_->ev[gt_->used][1], st);
+  auto gemm_end = [&](hipStream_t s) {
+    (void)hipEventRecord(gt_->ev[gt_->used][1], s);
     gt_->used++;
   };
+  if (!P->st2) {
+    MVF_HIP(h, hipStreamCreate(&P->st2));
+    MVF_HIP(h, hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+    MVF_HIP(h, hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+  }
   double2 *prev = nullptr;
   for (int d = P->maxdepth; d >= 0; d--) {
     const int nl = (int)P->levels[d].size();
@@ -1636,7 +1661,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
       hipLaunchKernelGGL(k_fa_sym, dim3(cap_grid(((mn1 + 31) / 32) * ((mm + 31) / 32)), nl), dim3(256), 0, st,
                          P->d_nodes, lev, P->d_fac, pool);
       // one GEMM launch over the level's fronts: grid from the largest operand, algorithmic flops summed
-      auto gemm = [&](auto mode_tag, int64_t step, int64_t K0) {
+      auto gemm = [&](auto mode_tag, int64_t step, int64_t K0, hipStream_t s) {
         constexpr int MODE = decltype(mode_tag)::value;
         int64_t gt = 0;
         double gfl = 0.0;
@@ -1650,9 +1675,9 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                            : 8.0 * (double)g.Mr * (double)g.Nc * (double)g.Kd;
         }
         if (gt == 0) return;
-        gemm_begin(gfl, MODE);
-        launch_gemm<MODE>(big, cap_grid(gt), nl, st, P->d_nodes, lev, step, K0, P->d_fac, pool);
-        gemm_end();
+        gemm_begin(gfl, MODE, s);
+        launch_gemm<MODE>(big, cap_grid(gt), nl, s, P->d_nodes, lev, step, K0, P->d_fac, pool);
+        gemm_end(s);
       };
       auto urow = [&](int64_t k, int64_t K0, int part) {
         int64_t gu = 0;
@@ -1673,7 +1698,15 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
       using M3 = std::integral_constant<int, 3>;
       using M4 = std::integral_constant<int, 4>;
       using M5 = std::integral_constant<int, 5>;
-      // LU of the top bands, two-level: panels of kNB inside outer blocks of kOuter
+      // LU of the top bands, two-level: panels of kNB inside outer blocks of kOuter.
+      // With more than one outer block, the trailing part of each outer update
+      // (rows and columns past the next outer block) runs on the side stream
+      // while the next block's panels are factored on the handle's stream:
+      // pivoting is restricted to the panel's rows, so those panels touch
+      // neither the trailing part's rows nor its columns.  The next outer
+      // update waits for it.
+      const bool ahead = mn1 > kOuter;
+      bool pending = false;
       for (int64_t K0 = 0; K0 < mn1; K0 += kOuter) {
         const int64_t Kend = std::min(K0 + kOuter, mn1);
         for (int64_t k = K0; k < Kend; k += kNB) {
@@ -1693,15 +1726,27 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes,
                              lev, k, P->d_fac, P->d_piv);
           urow(k, K0, 0);
-          gemm(M0{}, k, K0);
+          gemm(M0{}, k, K0, st);
         }
         // deferred part of the outer block's rows past it, then the rest of the front with K = kOuter
         for (int64_t k = K0; k < Kend; k += kNB) {
           urow(k, K0, 1);
-          gemm(M3{}, k, K0);
+          gemm(M3{}, k, K0, st);
         }
-        gemm(M4{}, 0, K0);
+        if (ahead) {
+          if (pending) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
+          MVF_HIP(h, hipEventRecord(P->ev_fork, st));
+          MVF_HIP(h, hipStreamWaitEvent(P->st2, P->ev_fork, 0));
+          gemm(M4{}, 3, K0, P->st2);
+          MVF_HIP(h, hipEventRecord(P->ev_join, P->st2));
+          pending = true;
+          gemm(M4{}, 1, K0, st);
+          gemm(M4{}, 2, K0, st);
+        } else {
+          gemm(M4{}, 0, K0, st);
+        }
       }
+      if (pending) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
       if (mn2 > 0) {
         // Y = U^-1 (L^-1 P F12), blocks from the bottom, outer groups of four blocks
         const int64_t nsteps = (mn1 + kNB - 1) / kNB;
@@ -1709,10 +1754,10 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kTrsmThreads - 1) / kTrsmThreads), nl),
                              dim3(kTrsmThreads), 0, st,
                              P->d_nodes, lev, s, P->d_fac);
-          gemm(M1{}, s, 0);
-          gemm(M5{}, s, 0);
+          gemm(M1{}, s, 0, st);
+          gemm(M5{}, s, 0, st);
         }
-        gemm(M2{}, 0, 0);
+        gemm(M2{}, 0, 0, st);
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
     }
