@@ -997,57 +997,66 @@ __device__ __forceinline__ void mfma_tiles3m(const double2 *A, int64_t lda, cons
     for (int r = 0; r < 2; r++)
 #pragma unroll
       for (int c = 0; c < 2; c++) t1[r][c] = t2[r][c] = t3[r][c] = mvf_d4{0.0, 0.0, 0.0, 0.0};
-    double2 ra[4], rb[2];
-    auto fetch = [&](int64_t kk) {
+    // two register sets: chunks ch + 1 and ch + 2 are in flight while chunk ch
+    // is multiplied, so each load has two chunk times to arrive
+    double2 ra[2][4], rb[2][2];
+    auto fetch = [&](int set, int64_t kk) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int64_t r = i0 + ar + 32 * j, q = kk + aq;
-        ra[j] = (r < Mr && q < Kd) ? A[r * lda + q] : make_double2(0.0, 0.0);
+        ra[set][j] = (r < Mr && q < Kd) ? A[r * lda + q] : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int j = 0; j < 2; j++) {
         const int64_t qb = kk + bq + 8 * j, c = j0 + bc;
-        rb[j] = (qb < Kd && c < Nc) ? B[qb * ldb + c] : make_double2(0.0, 0.0);
+        rb[set][j] = (qb < Kd && c < Nc) ? B[qb * ldb + c] : make_double2(0.0, 0.0);
       }
     };
-    auto stash = [&](int buf) {
+    auto stash = [&](int set, int buf) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) As[buf][aq][ar + 32 * j] = ra[j];
+      for (int j = 0; j < 4; j++) As[buf][aq][ar + 32 * j] = ra[set][j];
 #pragma unroll
-      for (int j = 0; j < 2; j++) Bs[buf][bq + 8 * j][bc] = rb[j];
+      for (int j = 0; j < 2; j++) Bs[buf][bq + 8 * j][bc] = rb[set][j];
     };
     const int64_t nch = (Kd + 15) / 16;
-    fetch(0);
-    stash(0);
+    fetch(0, 0);
+    stash(0, 0);
+    if (nch > 1) fetch(0, 16);
+    if (nch > 2) fetch(1, 32);
     __syncthreads();
-    for (int64_t ch = 0; ch < nch; ch++) {
+    // register set `set` holds chunk ch + 1; once stashed it is refilled with chunk ch + 3
+    auto step = [&](int64_t ch, int set) {
       const int buf = (int)(ch & 1);
-      if (ch + 1 < nch) fetch((ch + 1) * 16);
 #pragma unroll
       for (int ks = 0; ks < 16; ks += 4) {
-        double2 a[2], b[2];
-        double sa[2], sb[2];
+          double2 a[2], b[2];
+          double sa[2], sb[2];
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-          a[r] = As[buf][ks + lk][wr + r * 16 + li];
-          sa[r] = a[r].x + a[r].y;
-        }
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-          b[c] = Bs[buf][ks + lk][wc + c * 16 + li];
-          sb[c] = b[c].x + b[c].y;
-        }
-#pragma unroll
-        for (int r = 0; r < 2; r++)
+          for (int r = 0; r < 2; r++) {
+            a[r] = As[buf][ks + lk][wr + r * 16 + li];
+            sa[r] = a[r].x + a[r].y;
+          }
 #pragma unroll
           for (int c = 0; c < 2; c++) {
-            t1[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, t1[r][c], 0, 0, 0);
-            t2[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, t2[r][c], 0, 0, 0);
-            t3[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[r], sb[c], t3[r][c], 0, 0, 0);
+            b[c] = Bs[buf][ks + lk][wc + c * 16 + li];
+            sb[c] = b[c].x + b[c].y;
           }
+#pragma unroll
+          for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+              t1[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, t1[r][c], 0, 0, 0);
+              t2[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, t2[r][c], 0, 0, 0);
+              t3[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[r], sb[c], t3[r][c], 0, 0, 0);
+            }
       }
-      if (ch + 1 < nch) stash(buf ^ 1);
+      if (ch + 1 < nch) stash(set, buf ^ 1);
+      if (ch + 3 < nch) fetch(set, (ch + 3) * 16);
       __syncthreads();
+    };
+    for (int64_t ch = 0; ch < nch; ch += 2) {
+      step(ch, 0);
+      if (ch + 1 < nch) step(ch + 1, 1);
     }
 #pragma unroll
     for (int r = 0; r < 2; r++) {
