@@ -1160,12 +1160,15 @@ __host__ __device__ inline GemmArgs gemm_args(int mode, const DNode &nd, int64_t
       g.Mr = r0 - R0;
       g.Kd = r1 - r0;
     } else {
+      // K0 selects the part: 0 all rows [0, r0); 1 the next group's rows
+      // [r0 - kOuter, r0); 2 the rows above them [0, r0 - kOuter)
       if (bb % 4 != 0) return g;
-      const int64_t R1 = (r0 + kOuter < n1) ? r0 + kOuter : n1;
-      g.a = r0;
+      const int64_t R1 = (r0 + kOuter < n1) ? r0 + kOuter : n1, rs = r0 > kOuter ? r0 - kOuter : 0;
+      const int64_t q0 = K0 == 1 ? rs : 0, q1 = K0 == 2 ? rs : r0;
+      g.a = q0 * m + r0;
       g.b = r0 * m + n1;
-      g.c = n1;
-      g.Mr = r0;
+      g.c = q0 * m + n1;
+      g.Mr = q1 - q0;
       g.Kd = R1 - r0;
     }
   } else {
@@ -1759,13 +1762,31 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
       if (mn2 > 0) {
         // Y = U^-1 (L^-1 P F12), blocks from the bottom, outer groups of four blocks
         const int64_t nsteps = (mn1 + kNB - 1) / kNB;
+        // lookahead as in the LU: the outer Y update of the rows above the next
+        // group runs on the side stream while that group is solved
+        bool ypend = false;
         for (int64_t s = 0; s < nsteps; s++) {
           hipLaunchKernelGGL((k_fa_trsm_y<kNB>), dim3(cap_grid((mn2 + kTrsmThreads - 1) / kTrsmThreads), nl),
                              dim3(kTrsmThreads), 0, st,
                              P->d_nodes, lev, s, P->d_fac);
           gemm(M1{}, s, 0, st);
-          gemm(M5{}, s, 0, st);
+          if (ahead) {
+            bool any = false;
+            for (int t : P->levels[d]) any = any || gemm_args(5, P->nodes[t], s, 2).ok;
+            if (any) {
+              if (ypend) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
+              MVF_HIP(h, hipEventRecord(P->ev_fork, st));
+              MVF_HIP(h, hipStreamWaitEvent(P->st2, P->ev_fork, 0));
+              gemm(M5{}, s, 2, P->st2);
+              MVF_HIP(h, hipEventRecord(P->ev_join, P->st2));
+              ypend = true;
+            }
+            gemm(M5{}, s, 1, st);
+          } else {
+            gemm(M5{}, s, 0, st);
+          }
         }
+        if (ypend) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
         gemm(M2{}, 0, 0, st);
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
