@@ -17,7 +17,8 @@
 // the Gauss points are computed once (the reference recomputes the Jacobian
 // for every basis call, v_fem.f90:475-476,516-517), the per-(point, DOF)
 // vectors chat, ve, m1*chat, m2*ve are staged in LDS for a chunk of points, and
-// each thread accumulates its upper-packed (i <= j) pairs over the points.
+// each thread accumulates a 3 x 3 tile of the upper (i <= j) pairs over the
+// points in registers.
 //
 // k_elem_rhs: the per-frequency element RHS blocal (integration.f90:88-106)
 // with the secondary-field source of p_source (problem.f90:91-128), the air
@@ -92,9 +93,12 @@ __device__ inline void sym6_mul(const double *m, const double *v, double *o) {
 template <typename T, int NN>
 __device__ inline T jacobian_t(const BasisTab *tab, int gpt, const T (*xs)[3], T *Ji) {
   T J[3][3];
+#pragma unroll
   for (int m = 0; m < 3; m++)
+#pragma unroll
     for (int n = 0; n < 3; n++) {
       T acc = 0;
+#pragma unroll 4
       for (int l = 0; l < NN; l++) acc += (T)tab->dN[gpt][m][l] * xs[l][n];
       J[m][n] = acc;
     }
@@ -129,7 +133,11 @@ __global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ ta
                                                  Geom g, Gpml gp, int prev, int first_flags, int64_t e0,
                                                  T *__restrict__ Kout, T *__restrict__ Mout, int *bad) {
   constexpr int NP = NE * (NE + 1) / 2;
-  constexpr int PPT = (NP + 255) / 256;
+  // pairs in 3 x 3 tiles (i in 3bi..3bi+2, j in 3bj..3bj+2, bi <= bj): a tile
+  // reads six values of each of its three row DOFs and three column DOFs per
+  // point for 54 products, a third of the LDS traffic of one pair per thread
+  constexpr int TE = NE / 3, NT = TE * (TE + 1) / 2;
+  static_assert(NE % 3 == 0 && NT <= 256, "one 3 x 3 tile per thread");
   const int64_t e = e0 + blockIdx.x;
   const int t = threadIdx.x;
   __shared__ double xs64[NN][3];
@@ -160,8 +168,10 @@ __global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ ta
     if (det == (T)0) atomicExch(bad, 1);
     T wgt = det * (T)tab->w[t];
     T m1[6], m2[6];
+#pragma unroll
     for (int c = 0; c < 6; c++) {
       T a1 = 0, a2 = 0;
+#pragma unroll 4
       for (int n = 0; n < NN; n++) {
         a1 += (T)tab->N[t][n] * nm[n][c];
         a2 += (T)tab->N[t][n] * nm[n][6 + c];
@@ -189,15 +199,20 @@ __global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ ta
     for (int c = 0; c < 9; c++) sJi[t][c] = Ji[c];
   }
   __syncthreads();
-  T kacc[PPT], macc[PPT];
-  int pa[PPT], pb[PPT];
-#pragma unroll
-  for (int s = 0; s < PPT; s++) {
-    kacc[s] = macc[s] = 0;
-    int p = t + s * 256;
-    pa[s] = p < NP ? tab->pi[p] : 0;
-    pb[s] = p < NP ? tab->pj[p] : 0;
+  int bi = 0, bj = 0;
+  {
+    int q = t < NT ? t : 0;  // row-major over the upper tiles
+    while (q >= TE - bi) {
+      q -= TE - bi;
+      bi++;
+    }
+    bj = bi + q;
   }
+  T kacc[3][3], macc[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) kacc[r][c] = macc[r][c] = 0;
   for (int g0 = 0; g0 < NG; g0 += GC) {
     const int gcount = (NG - g0) < GC ? (NG - g0) : GC;
     for (int q = t; q < gcount * NE; q += 256) {
@@ -223,29 +238,39 @@ __global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ ta
       o[9] = tm[0]; o[10] = tm[1]; o[11] = tm[2];
     }
     __syncthreads();
-    for (int gl = 0; gl < gcount; gl++) {
+    if (t < NT)
+      for (int gl = 0; gl < gcount; gl++) {
+        T A[3][6], B[3][6];
 #pragma unroll
-      for (int s = 0; s < PPT; s++) {
-        int p = t + s * 256;
-        if (p < NP) {
-          const T *A = sv[gl][pa[s]];
-          const T *B = sv[gl][pb[s]];
-          kacc[s] += A[0] * B[6] + A[1] * B[7] + A[2] * B[8];
-          macc[s] += A[3] * B[9] + A[4] * B[10] + A[5] * B[11];
-        }
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 6; c++) {
+            A[r][c] = sv[gl][3 * bi + r][c];
+            B[r][c] = sv[gl][3 * bj + r][6 + c];
+          }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            kacc[r][c] += A[r][0] * B[c][0] + A[r][1] * B[c][1] + A[r][2] * B[c][2];
+            macc[r][c] += A[r][3] * B[c][3] + A[r][4] * B[c][4] + A[r][5] * B[c][5];
+          }
       }
-    }
     __syncthreads();
   }
   const int64_t base = (e - e0) * (int64_t)NP;
+  if (t < NT)
 #pragma unroll
-  for (int s = 0; s < PPT; s++) {
-    int p = t + s * 256;
-    if (p < NP) {
-      Kout[base + p] = kacc[s];
-      Mout[base + p] = macc[s];
-    }
-  }
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int ii = 3 * bi + r, jj = 3 * bj + c;
+        if (ii <= jj) {
+          const int p = ii * NE - ii * (ii - 1) / 2 + (jj - ii);  // row-major upper packing
+          Kout[base + p] = kacc[r][c];
+          Mout[base + p] = macc[r][c];
+        }
+      }
 }
 
 // per-frequency element RHS: be[e][im][d] complex, d = polarization
