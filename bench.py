@@ -42,7 +42,7 @@ ELEM_FLOPS = {8: 21768, 20: 376731, 27: 744795}
 # WRITE_SIZE runs, scaled by the same counters on a 1 GiB copy
 # (profiles/k_spmv_halfspace8.md)
 # HBM bytes per launch from FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy's counts (profiles/)
-TRAFFIC_MEASURED = {("halfspace8", "k_spmv<float>"): 11.98e6, ("aniso20", "k_fa_gemm"): 3.59e9}
+TRAFFIC_MEASURED = {("halfspace8", "k_spmv<float>"): 11.98e6, ("aniso20", "k_fa_gemm"): 3.43e9}
 
 
 def workload(name):

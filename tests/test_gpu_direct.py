@@ -103,3 +103,33 @@ def test_direct_frequency_sweep(need_gpu):
             # zones: a 1e-10 residual bounds the difference at ~1e-6 there
             # (observed 0.6-4e-6 depending on the GEMM's rounding), 1e-8 elsewhere
             assert e <= (1e-5 if freqs[i] >= 10 else 1e-6), (i, d, e, rr)
+
+
+def test_direct_lookahead_fronts(need_gpu):
+    """20-node 14 x 14 x 11 GPML model whose largest fronts own 974 DOFs:
+    four outer blocks of 256, so the factorization overlaps the trailing
+    part of each outer LU update and of each outer Y update with the next
+    block's panels on its second stream, and the fronts above 512 run the
+    128 x 64 three-product tiles.  An error in either split or in the stream
+    ordering corrupts the factor that preconditions GMRES: the bar is the
+    true residual on the reference's matrix within the few iterations an
+    exact factor needs, at both ends of the frequency range."""
+    from tests.test_direct_plan import plan_stats
+    from oracle.oracle import Oracle
+    for f in (1e-3, 100.0):  # one frequency per model, as the reference's first frequency
+        m = M.anisotropic_blocks(nx=14, ny=14, nz_earth=8, nz_air=3, nodes=20, freqs=(f,), block=2, seed=5, nextd=2)
+        if f == 1e-3:
+            o = Oracle(m)
+            s = plan_stats(m, o.gne(), o.nne)
+            assert s["max_n1"] > 3 * 256, s
+        g = GpuVfem(m)
+        g.set_frequency(0)
+        A, b = reference_system(g)
+        print(f"lookahead fronts: f={f} nne={g.nne} solving", flush=True)
+        x, info = g.solve(b.ravel(), method="gmres", precond="direct", tol=1e-8)
+        x = x.reshape(2, -1)
+        print(f"lookahead fronts: f={f} its={info['iterations']} pert={info['static_pivots']}", flush=True)
+        assert info["converged"] and info["iterations"] <= 10, info
+        for d in range(2):
+            r = np.linalg.norm(b[d] - A @ x[d]) / np.linalg.norm(b[d])
+            assert r <= 1e-8 * 1.0001, (f, d, r)
