@@ -1973,10 +1973,12 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   info->gemm_ms = P->gemm_ms;
   info->gemm_flops = P->gemm_flops;
   info->gemm_launches = P->gemm_launches;
-  double2 *V, *z, *wv, *r;
+  // Z keeps the preconditioned basis vectors M^-1 V_j, so the update of x needs
+  // no further application of the factor at the end of a cycle
+  double2 *V, *Z, *wv, *r;
   double *part, *dout, *coef, *dnorm;
   MVF_TRY(dalloc(h, &V, (size_t)(mres + 1) * nv));
-  MVF_TRY(dalloc(h, &z, (size_t)nv));
+  MVF_TRY(dalloc(h, &Z, (size_t)mres * nv));
   MVF_TRY(dalloc(h, &wv, (size_t)nv));
   MVF_TRY(dalloc(h, &r, (size_t)nv));
   const int nbk = vec_blocks(n, NC);
@@ -2007,7 +2009,7 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   int applies = 0;
   bool done = false;
   while (!done) {
-    MVF_TRY(true_resid<NC>(S, b, x, (double *)z, part + (size_t)(mres + 1) * std::max(nbk, kRedBlocks) * NC * 2,
+    MVF_TRY(true_resid<NC>(S, b, x, (double *)Z, part + (size_t)(mres + 1) * std::max(nbk, kRedBlocks) * NC * 2,
                            dnorm, res, (double *)r));
     double hn[2 * NC];
     MVF_HIP(h, hipMemcpy(hn, dnorm, sizeof(hn), hipMemcpyDeviceToHost));
@@ -2035,8 +2037,9 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
     int kdone = 0;
     for (int j = 0; j < mres && it < o->maxit; j++) {
       double ta = now_s();
-      MVF_TRY(direct_apply<NC>(h, V + (size_t)j * nv, z));
-      spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, (const double *)z, (double *)wv, nullptr, nullptr);
+      double2 *zj = Z + (size_t)j * nv;
+      MVF_TRY(direct_apply<NC>(h, V + (size_t)j * nv, zj));
+      spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, (const double *)zj, (double *)wv, nullptr, nullptr);
       std::vector<cplx> hv;
       for (int pass = 0; pass < 2; pass++) {
         MVF_TRY(dots(j + 1, wv, hv));
@@ -2108,7 +2111,7 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
       kdone = j + 1;
       if (worst <= 0.3 * o->tol) break;
     }
-    // y = R^-1 g per column; u = V y; x += M^-1 u
+    // y = R^-1 g per column; x += Z y
     std::vector<double> cy((size_t)kdone * NC * 2, 0.0);
     for (int c = 0; c < NC; c++) {
       std::vector<cplx> y(kdone);
@@ -2130,10 +2133,7 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
     }
     if (kdone > 0) {
       MVF_TRY(put_coef(cy));
-      hipLaunchKernelGGL((k_gm_axpy<NC>), dim3(gv), dim3(256), 0, st, n, V, nv, kdone, coef, wv, 1.0, 1);
-      MVF_TRY(direct_apply<NC>(h, wv, z));
-      hipLaunchKernelGGL((k_gm_add<NC>), dim3(gv), dim3(256), 0, st, n, (double2 *)x, z);
-      applies++;
+      hipLaunchKernelGGL((k_gm_axpy<NC>), dim3(gv), dim3(256), 0, st, n, Z, nv, kdone, coef, (double2 *)x, 1.0, 0);
     }
     restarts++;
   }
@@ -2142,7 +2142,7 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   info->converged = rc == 0;
   info->precond_applies += applies;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  dfree(V); dfree(z); dfree(wv); dfree(r); dfree(part); dfree(dout); dfree(coef); dfree(dnorm);
+  dfree(V); dfree(Z); dfree(wv); dfree(r); dfree(part); dfree(dout); dfree(coef); dfree(dnorm);
   return rc;
 }
 
