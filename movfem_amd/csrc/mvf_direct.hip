@@ -1926,14 +1926,6 @@ __global__ void k_gm_axpy(int64_t n, const double2 *V, int64_t vstride, int nv, 
   }
 }
 
-template <int NC>
-__global__ void k_gm_add(int64_t n, double2 *x, const double2 *u) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * NC; e += (int64_t)gridDim.x * blockDim.x) {
-    x[e].x += u[e].x;
-    x[e].y += u[e].y;
-  }
-}
-
 struct cplx {
   double r, i;
 };
