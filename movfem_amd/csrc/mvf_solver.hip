@@ -72,9 +72,8 @@ __global__ __launch_bounds__(kBS) void k_spmv(int64_t nne, const int64_t *__rest
   for (int c = 0; c < NC; c++) ar[c] = ai[c] = 0.0;
   if (row < nne) {
     const int64_t k1 = rowptr[row + 1];
-    for (int64_t k = rowptr[row] + lane; k < k1; k += LPR) {
-      const int64_t cc = col[k];
-      const double kv = (double)K[k], mv = (double)M[k];
+    // one entry of the lane: A_k x_col(k) accumulated into (ar, ai)
+    auto entry = [&](int64_t cc, double kv, double mv) {
       const VT *xp = x + 2 * cc * NC;
 #pragma unroll
       for (int c = 0; c < NC; c++) {
@@ -87,7 +86,16 @@ __global__ __launch_bounds__(kBS) void k_spmv(int64_t nne, const int64_t *__rest
         ar[c] += vr * xr - vi * xi;
         ai[c] += vr * xi + vi * xr;
       }
+    };
+    int64_t k = rowptr[row] + lane;
+    // two entries per lane in flight, accumulated in the same order
+    for (; k + LPR < k1; k += 2 * LPR) {
+      const int64_t c0 = col[k], c1 = col[k + LPR];
+      const double k0v = (double)K[k], m0v = (double)M[k], k1v = (double)K[k + LPR], m1v = (double)M[k + LPR];
+      entry(c0, k0v, m0v);
+      entry(c1, k1v, m1v);
     }
+    if (k < k1) entry(col[k], (double)K[k], (double)M[k]);
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1)
