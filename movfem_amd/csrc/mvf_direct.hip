@@ -565,7 +565,8 @@ __global__ __launch_bounds__(256) void k_fa_pdiag(const DNode *nodes, const int3
                                                   double2 *inv, int32_t *piv, int *npert, double big, double tiny) {
   // Four waves: lane t of every wave stands for row t, wave g for the columns
   // c = g (mod 4) of the rank-1 updates and of the inverse's updates.  The
-  // pivot search runs on wave 0.
+  // pivot search, interchange and multipliers run on wave 0: two barriers per
+  // column.
   static_assert(NB == 64, "one lane per row of the diagonal block");
   const DNode nd = nodes[lev[blockIdx.x]];
   if (k >= nd.n1) return;
@@ -574,7 +575,6 @@ __global__ __launch_bounds__(256) void k_fa_pdiag(const DNode *nodes, const int3
   double2 *T = fac + nd.fac;
   __shared__ double2 a[NB][NB + 1];
   __shared__ double2 x[NB][NB + 1];
-  __shared__ int sp;
   const int tid = threadIdx.x, t = tid & 63, g = tid >> 6;
   // wave g reads rows g, g + 4, ...; lane t column t (coalesced, clamped
   // addresses, padding selected afterwards)
@@ -589,9 +589,12 @@ __global__ __launch_bounds__(256) void k_fa_pdiag(const DNode *nodes, const int3
   }
   __syncthreads();
   for (int q = 0; q < w; q++) {
-    // pivot: largest |a[r][q]|, r in [q, w); ties to the lower row
-    double v = -1.0;
+    // wave 0 alone: pivot search, row interchange, perturbation and the
+    // multipliers of column q (one wave's LDS operations complete in order,
+    // so no barrier is needed between these steps)
     if (g == 0) {
+      // pivot: largest |a[r][q]|, r in [q, w); ties to the lower row
+      double v = -1.0;
       int ri = q;
       if (t >= q && t < w) {
         v = a[t][q].x * a[t][q].x + a[t][q].y * a[t][q].y;
@@ -605,30 +608,21 @@ __global__ __launch_bounds__(256) void k_fa_pdiag(const DNode *nodes, const int3
           ri = oi;
         }
       }
-      if (t == 0) {
-        sp = ri;
-        piv[nd.pos0 + k + q] = (int32_t)(k + ri);
+      if (t == 0) piv[nd.pos0 + k + q] = (int32_t)(k + ri);
+      if (ri != q && t < w) {
+        const double2 e = a[q][t];
+        a[q][t] = a[ri][t];
+        a[ri][t] = e;
       }
+      if (t == 0 && !(v > tiny * tiny)) {
+        a[q][q] = make_double2(big, 0.0);
+        atomicAdd(npert, 1);
+      }
+      if (t > q && t < w) a[t][q] = cmul(a[t][q], cinv(a[q][q]));
     }
     __syncthreads();
-    const int r = sp;
-    if (r != q && tid < w) {
-      const double2 e = a[q][tid];
-      a[q][tid] = a[r][tid];
-      a[r][tid] = e;
-    }
-    __syncthreads();
-    if (tid == 0 && !(v > tiny * tiny)) {
-      a[q][q] = make_double2(big, 0.0);
-      atomicAdd(npert, 1);
-    }
-    __syncthreads();
-    // every wave forms the multiplier of its row before wave 0 stores it
-    const bool act = t > q && t < w;
-    const double2 l = act ? cmul(a[t][q], cinv(a[q][q])) : make_double2(0.0, 0.0);
-    __syncthreads();
-    if (act) {
-      if (g == 0) a[t][q] = l;
+    if (t > q && t < w) {
+      const double2 l = a[t][q];
       // row t -= l * row q over this wave's columns, four per batch
       for (int c0 = q + 1 + g; c0 < w; c0 += 16) {
         double2 xr[4], yr[4];
