@@ -38,11 +38,9 @@ FP64_PEAK_TFS = 78.6
 FP32_PEAK_TFS = 157.3
 # algorithmic flops per element matrix (K, M upper and b), canonical counts
 ELEM_FLOPS = {8: 21768, 20: 376731, 27: 744795}
-# HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs, scaled by the same counters on a 1 GiB copy
-# (profiles/k_spmv_halfspace8.md)
-# HBM bytes per launch from FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy's counts (profiles/)
-TRAFFIC_MEASURED = {("halfspace8", "k_spmv<float>"): 11.98e6, ("aniso20", "k_fa_gemm"): 3.43e9}
+# HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
+# by what the counter reads for a plain copy of known size (profiles/)
+TRAFFIC_MEASURED = {("halfspace8", "k_spmv"): 20.17e6, ("aniso20", "k_fa_gemm"): 3.43e9}
 
 
 def workload(name):
@@ -201,7 +199,8 @@ def main():
     avg_ms = last["spmv_ms"][k] / max(1, last["spmv_launches"][k])
     spmv = {"bound": "hbm", "kernel": "k_spmv<double>", "achieved": last["spmv_bytes"][k] / (avg_ms * 1e-3) / 1e9
             if avg_ms > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s", "avg_launch_ms": avg_ms,
-            "launches": last["spmv_launches"][k], "bytes_per_launch": last["spmv_bytes"][k]}
+            "launches": last["spmv_launches"][k], "bytes_per_launch": last["spmv_bytes"][k],
+            "traffic": TRAFFIC_MEASURED.get((args.workload, "k_spmv"))}
     spmv["frac"] = spmv["achieved"] / HBM_PEAK_GBS
     # element kernel (FP64 and FP32 integration)
     e64 = float(np.median(elem_ms))
