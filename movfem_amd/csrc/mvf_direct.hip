@@ -692,61 +692,56 @@ constexpr int kTrsmThreads = 256;
 constexpr int kTrsmChunk = 16;
 
 // L rows below the diagonal block: T[i, k:kend] = T[i, k:kend] U_kk^-1 for
-// i in [kend, n1).  One thread per row walks the block's columns in 16-wide
-// chunks: a chunk is loaded into registers, reduced by the chunks solved
-// before it (read back from the row, where the thread itself stored them),
-// solved by forward substitution and stored; U_kk and the reciprocal pivots
-// are read from LDS as broadcasts.
+// i in [kend, n1).  Four lanes per row, lane l holding columns 16 l .. 16 l + 15
+// of its row in registers (one batch of loads, 1 KB contiguous per row): for
+// each column c in order the lane that owns it scales it by the reciprocal
+// pivot, passes it to the row's other lanes, and every lane subtracts its
+// multiple of row c of U_kk (read from LDS) from its later columns.
 template <int NB>
-__global__ __launch_bounds__(kTrsmThreads, 2) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k,
-                                                            double2 *fac) {
+__global__ __launch_bounds__(256) void k_fa_prows(const DNode *nodes, const int32_t *lev, int64_t k, double2 *fac) {
+  static_assert(NB == 64, "four lanes of sixteen columns per row");
   const DNode nd = nodes[lev[blockIdx.y]];
   if (k >= nd.n1) return;
   const int64_t m = nd.m, kend = (k + NB < nd.n1) ? k + NB : nd.n1;
-  if (kend + (int64_t)blockIdx.x * kTrsmThreads >= nd.n1) return;
+  if (kend + (int64_t)blockIdx.x * 64 >= nd.n1) return;
   const int w = (int)(kend - k);
   double2 *T = fac + nd.fac;
   __shared__ double2 U[NB][NB];
   __shared__ double2 dinv[NB];
-  for (int e = threadIdx.x; e < NB * NB; e += kTrsmThreads) {
+  for (int e = threadIdx.x; e < NB * NB; e += 256) {
     const int r = e / NB, c = e % NB;
-    U[r][c] = (r < w && c < w && c >= r) ? T[(k + r) * m + k + c] : make_double2(0.0, 0.0);
+    const double2 v = T[(k + (r < w ? r : w - 1)) * m + k + (c < w ? c : w - 1)];
+    U[r][c] = (r < w && c < w && c >= r) ? v : make_double2(0.0, 0.0);
   }
   __syncthreads();
   if (threadIdx.x < NB) dinv[threadIdx.x] = threadIdx.x < w ? cinv(U[threadIdx.x][threadIdx.x]) : make_double2(0, 0);
   __syncthreads();
-  constexpr int C = kTrsmChunk / 2, NQ = NB / C;
-  for (int64_t i0 = kend + (int64_t)blockIdx.x * kTrsmThreads; i0 < nd.n1; i0 += (int64_t)gridDim.x * kTrsmThreads) {
-    const int64_t i = i0 + threadIdx.x;
-    if (i >= nd.n1) continue;
+  const int l = threadIdx.x & 3, rr = threadIdx.x >> 2;
+  for (int64_t i0 = kend + (int64_t)blockIdx.x * 64; i0 < nd.n1; i0 += (int64_t)gridDim.x * 64) {
+    const int64_t i = i0 + rr;
+    if (i >= nd.n1) continue;  // the row's four lanes leave together
     double2 *row = T + i * m + k;
-    #pragma unroll 1
-    for (int q = 0; q < NQ && q * C < w; q++) {
-      double2 x[C];
+    double2 x[16];
 #pragma unroll
-      for (int c = 0; c < C; c++) x[c] = (q * C + c < w) ? row[q * C + c] : make_double2(0.0, 0.0);
-      #pragma unroll 1
-      for (int q2 = 0; q2 < q; q2++) {
-        double2 y[C];
-#pragma unroll
-        for (int p = 0; p < C; p++) y[p] = row[q2 * C + p];
-#pragma unroll
-        for (int p = 0; p < C; p++)
-#pragma unroll
-          for (int c = 0; c < C; c++) cfms(x[c], y[p], U[q2 * C + p][q * C + c]);
-      }
-      // columns past w hold zeros and zero U entries, so they stay zero
-#pragma unroll
-      for (int c = 0; c < C; c++) {
-        double2 v = x[c];
-#pragma unroll
-        for (int p = 0; p < c; p++) cfms(v, x[p], U[q * C + p][q * C + c]);
-        x[c] = cmul(v, dinv[q * C + c]);
-      }
-#pragma unroll
-      for (int c = 0; c < C; c++)
-        if (q * C + c < w) row[q * C + c] = x[c];
+    for (int j = 0; j < 16; j++) {
+      const int c = 16 * l + j;
+      const double2 v = row[c < w ? c : w - 1];
+      x[j] = c < w ? v : make_double2(0.0, 0.0);
     }
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+      if (c < w) {  // uniform; c stays a compile-time register index
+        const int o = c >> 4, jj = c & 15;
+        if (l == o) x[jj] = cmul(x[jj], dinv[c]);
+        const double2 xc = make_double2(__shfl(x[jj].x, o, 4), __shfl(x[jj].y, o, 4));
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+          if (16 * l + j > c) cfms(x[j], xc, U[c][16 * l + j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      if (16 * l + j < w) row[16 * l + j] = x[j];
   }
 }
 
@@ -1726,8 +1721,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           hipLaunchKernelGGL((k_fa_pdiag<kNB>), dim3(nl), dim3(256), 0, st, P->d_nodes, lev, k, P->d_fac, P->d_inv,
                              P->d_piv, P->d_npert, big, tiny);
           if (gr > 0)
-            hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + kTrsmThreads - 1) / kTrsmThreads), nl),
-                               dim3(kTrsmThreads), 0, st,
+            hipLaunchKernelGGL((k_fa_prows<kNB>), dim3(cap_grid((gr + 63) / 64), nl), dim3(256), 0, st,
                                P->d_nodes, lev, k, P->d_fac);
           hipLaunchKernelGGL((k_fa_swap<kNB>), dim3(cap_grid((gc + 255) / 256), nl), dim3(256), 0, st, P->d_nodes,
                              lev, k, P->d_fac, P->d_piv);
