@@ -47,7 +47,7 @@ namespace mvf {
 
 constexpr int kLeafMax = 16;       // elements per leaf box
 constexpr int kNB = 64;            // panel width / triangular block size
-constexpr double kRegEps = 1e-3;   // air regularization relative to the smallest omega*M/K
+constexpr double kRegEps = 1e-4;   // air regularization relative to the smallest omega*M/K
 constexpr int kTile = 64;          // GEMM tile edge
 // threads of the kernels that keep one column (row) per thread in LDS
 constexpr int kColT = kNB <= 32 ? 128 : 64;
