@@ -1079,7 +1079,8 @@ __device__ __forceinline__ void mfma_tiles3m(const double2 *A, int64_t lda, cons
 // reach past the outer block are deferred and applied once per outer block
 // with K = kOuter, so the trailing part of a front is read and written
 // kOuter/kNB times less often than by rank-64 updates.
-constexpr int64_t kOuter = 4 * kNB;
+constexpr int64_t kGroup = 4;  // panels per outer block
+constexpr int64_t kOuter = kGroup * kNB;
 
 // operands of the batched GEMM launches, as offsets into the front's top
 // band T (or its pool band for the Schur update), all with leading dimension m
@@ -1142,7 +1143,7 @@ __host__ __device__ inline GemmArgs gemm_args(int mode, const DNode &nd, int64_t
     const int64_t r0 = bb * kNB, r1 = (r0 + kNB < n1) ? r0 + kNB : n1;
     g.Nc = n2;
     if (mode == 1) {
-      const int64_t R0 = (bb / 4) * 4 * kNB;
+      const int64_t R0 = (bb / kGroup) * kOuter;
       g.a = R0 * m + r0;
       g.b = r0 * m + n1;
       g.c = R0 * m + n1;
@@ -1151,7 +1152,7 @@ __host__ __device__ inline GemmArgs gemm_args(int mode, const DNode &nd, int64_t
     } else {
       // K0 selects the part: 0 all rows [0, r0); 1 the next group's rows
       // [r0 - kOuter, r0); 2 the rows above them [0, r0 - kOuter)
-      if (bb % 4 != 0) return g;
+      if (bb % kGroup != 0) return g;
       const int64_t R1 = (r0 + kOuter < n1) ? r0 + kOuter : n1, rs = r0 > kOuter ? r0 - kOuter : 0;
       const int64_t q0 = K0 == 1 ? rs : 0, q1 = K0 == 2 ? rs : r0;
       g.a = q0 * m + r0;
@@ -1748,7 +1749,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
       }
       if (pending) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
       if (mn2 > 0) {
-        // Y = U^-1 (L^-1 P F12), blocks from the bottom, outer groups of four blocks
+        // Y = U^-1 (L^-1 P F12), blocks from the bottom, outer groups of kGroup blocks
         const int64_t nsteps = (mn1 + kNB - 1) / kNB;
         // lookahead as in the LU: the outer Y update of the rows above the next
         // group runs on the side stream while that group is solved
