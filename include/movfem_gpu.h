@@ -35,7 +35,7 @@ enum {
   MVF_E_DEVICE = -2,     /* HIP runtime error                                      */
   MVF_E_SINGULAR = -3,   /* zero Jacobian or singular sigma/mu tensor at a node   */
   MVF_E_INDEX = -4,      /* slot/index consistency failure                         */
-  MVF_E_UNSUPPORTED = -5,/* option not implemented (e.g. Dirichlet inimod 2/3)     */
+  MVF_E_UNSUPPORTED = -5,/* option or combination this build does not provide      */
   MVF_E_NOCONV = -6,     /* Krylov did not reach the tolerance                     */
   MVF_E_BREAKDOWN = -7,  /* Krylov breakdown (rho or omega vanished)               */
   MVF_E_MEMORY = -8      /* device allocation failed                               */
@@ -187,7 +187,8 @@ typedef struct mvf_solve_info {
   int32_t factorizations;  /* factorizations performed by the handle so far */
   int32_t static_pivots;   /* pivots replaced because the owned block was singular */
   int32_t precond_applies; /* triangular-solve sweeps performed by this call */
-  int32_t pad2_;
+  int32_t operator_fp32;   /* 1: K and M in use were integrated in FP32 (mvf_assemble precision 1);
+                              residuals are then judged on that operator */
   /* batched complex GEMM launches of the last factorization (its dominant
    * kernel): summed HIP-event time, algorithmic real flops, launch count */
   double gemm_ms;
@@ -203,9 +204,9 @@ int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_so
 
 /* Device-resident path: RHS integrated on the device for each of nfreq
  * frequencies (processed in that order, as the reference's frequency loop
- * would), all 2*nfreq columns solved together by one batched Krylov sweep
- * sharing K and M.  Solutions stay on the device; fetch with
- * mvf_get_solution.  nfreq <= 8. */
+ * would), solved in batches of up to 8 frequencies (16 columns) per batched
+ * Krylov sweep sharing K and M.  Solutions stay on the device; fetch with
+ * mvf_get_solution.  1 <= nfreq <= 32 (true_resid holds 64 columns). */
 int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, const mvf_solver_opts *o,
                           mvf_solve_info *info);
 /* column-major (nne, 2*nfreq) complex solution of the last solve_frequencies */
@@ -221,6 +222,9 @@ int mvf_setup_times(const mvf_handle *h, double *t4);
  * and re-assembles them in place (the element loop of global_vfem,
  * MoVFEM_3DMT.f90:193-211, without any host transfer).  precision 0: FP64
  * integration; 1: FP32 integration (K and M rounded from FP32 element sums).
+ * The handle records which: solves report it in mvf_solve_info.operator_fp32,
+ * and mvf_global_vfem (the reference's FP64 matrix) refuses an FP32-integrated
+ * operator with MVF_E_ARG until an FP64 re-assembly.
  * integrate_ms (optional) receives the element kernel's time from HIP events. */
 int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms);
 

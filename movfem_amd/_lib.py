@@ -67,7 +67,7 @@ class SolveInfo(C.Structure):
                 ("spmv_ncol", C.c_int32), ("pad_", C.c_int32),
                 ("factor_seconds", C.c_double), ("factor_flops", C.c_double), ("factor_bytes", C.c_double),
                 ("factorizations", C.c_int32), ("static_pivots", C.c_int32), ("precond_applies", C.c_int32),
-                ("pad2_", C.c_int32), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
+                ("operator_fp32", C.c_int32), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
                 ("gemm_launches", C.c_int64)]
 
 

@@ -290,6 +290,7 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
 }
 
 int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms) {
+  DevScope ds;
   if (!h || (precision != 0 && precision != 1)) return h ? set_err(h, MVF_E_ARG, "precision must be 0 or 1") : MVF_E_ARG;
   MVF_HIP(h, hipSetDevice(h->device));
   const int64_t ne = h->geo.ne, np = (int64_t)h->et.ne * (h->et.ne + 1) / 2;
@@ -310,13 +311,13 @@ int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms) {
     h->d_Ke = h->d_Me = nullptr;
   } else {
     float *k32, *m32;
-    MVF_TRY(dalloc(h, &k32, np * ne));
-    MVF_TRY(dalloc(h, &m32, np * ne));
+    MVF_TRY(ds.alloc(h, &k32, np * ne));
+    MVF_TRY(ds.alloc(h, &m32, np * ne));
     MVF_HIP(h, hipEventRecord(ev[0], h->stream));
     rc = integrate_elements32(h, 0, ne, ff, k32, m32);
     (void)hipEventRecord(ev[1], h->stream);
     if (!rc) rc = assemble_operators32(h, k32, m32);
-    dfree(k32); dfree(m32);
+    ds.drop(k32); ds.drop(m32);
   }
   direct_invalidate(h);
   if (!rc) {
@@ -325,6 +326,7 @@ int mvf_assemble(mvf_handle *h, int32_t precision, double *integrate_ms) {
     (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
     if (integrate_ms) *integrate_ms = ms;
     h->carry_applied = (h->carry_nonzero && h->calls >= 1) ? 1 : 0;
+    h->ops_fp32 = precision;
   }
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
@@ -363,6 +365,7 @@ int mvf_set_frequency(mvf_handle *h, double freq_hz) {
 }
 
 int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me, double *be) {
+  DevScope ds;
   if (!h || e0 < 0 || e1 > h->geo.ne || e1 < e0) return MVF_E_ARG;
   if (be && !h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede element RHS");
   MVF_HIP(h, hipSetDevice(h->device));
@@ -372,28 +375,28 @@ int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me,
   int ff = first_flags_for(h, h->calls);
   if (Ke || Me) {
     double *dK, *dM;
-    MVF_TRY(dalloc(h, &dK, np * n));
-    MVF_TRY(dalloc(h, &dM, np * n));
+    MVF_TRY(ds.alloc(h, &dK, np * n));
+    MVF_TRY(ds.alloc(h, &dM, np * n));
     int rc = integrate_elements(h, e0, e1, ff, dK, dM);
     if (!rc && Ke)
       rc = hipMemcpyAsync(Ke, dK, 8 * np * n, hipMemcpyDeviceToHost, h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
     if (!rc && Me)
       rc = hipMemcpyAsync(Me, dM, 8 * np * n, hipMemcpyDeviceToHost, h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
     if (!rc) rc = hipStreamSynchronize(h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
-    dfree(dK);
-    dfree(dM);
+    ds.drop(dK);
+    ds.drop(dM);
     if (rc) return rc;
   }
   if (be) {
     double *d;
-    MVF_TRY(dalloc(h, &d, (size_t)n * h->et.ne * 4));
+    MVF_TRY(ds.alloc(h, &d, (size_t)n * h->et.ne * 4));
     int rc = element_rhs_range(h, h->omega, ff, e0, e1, d);
     if (!rc)
       rc = hipMemcpyAsync(be, d, 8 * (size_t)n * h->et.ne * 4, hipMemcpyDeviceToHost, h->stream) == hipSuccess
                ? 0
                : MVF_E_DEVICE;
     if (!rc) rc = hipStreamSynchronize(h->stream) == hipSuccess ? 0 : MVF_E_DEVICE;
-    dfree(d);
+    ds.drop(d);
     if (rc) return rc;
   }
   return 0;
@@ -401,26 +404,30 @@ int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me,
 
 // RHS of the current frequency into column pair col0 of an [nne][ncol] device array
 static int build_rhs(mvf_handle *h, double omega, int64_t call, double *d_b, int ncol, int col0) {
+  DevScope ds;
   double *be;
-  MVF_TRY(dalloc(h, &be, (size_t)h->geo.ne * h->et.ne * 4));
+  MVF_TRY(ds.alloc(h, &be, (size_t)h->geo.ne * h->et.ne * 4));
   int rc = element_rhs_range(h, omega, first_flags_for(h, call), 0, h->geo.ne, be);
   if (!rc) rc = bd_lift(h, omega, be);
   if (!rc) rc = assemble_rhs(h, be, d_b, ncol, col0);
-  dfree(be);
+  ds.drop(be);
   return rc;
 }
 
 int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze, int32_t nne,
                     int32_t ndir) {
+  DevScope ds;
   if (!h || !ia || !ja || !a || !b) return MVF_E_ARG;
   if (nnze != h->nnz || nne != h->nne || ndir != 2)
     return set_err(h, MVF_E_ARG, "size mismatch: nnze %lld/%lld nne %d/%d ndir %d", (long long)nnze,
                    (long long)h->nnz, nne, h->nne, ndir);
   if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede mvf_global_vfem");
+  if (h->ops_fp32)
+    return set_err(h, MVF_E_ARG, "K and M were integrated in FP32: mvf_assemble(h, 0, ...) restores the FP64 matrix");
   MVF_HIP(h, hipSetDevice(h->device));
   MVF_TRY(ensure_call_state(h, h->calls));
   double *db;
-  MVF_TRY(dalloc(h, &db, (size_t)nne * 4));
+  MVF_TRY(ds.alloc(h, &db, (size_t)nne * 4));
   MVF_TRY(build_rhs(h, h->omega, h->calls, db, 2, 0));
   std::vector<int64_t> rp(nne + 1);
   std::vector<int32_t> col(h->nnz);
@@ -431,7 +438,7 @@ int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *
   MVF_HIP(h, hipMemcpyAsync(M.data(), h->d_M, 8 * h->nnz, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipMemcpyAsync(bb.data(), db, 32 * (size_t)nne, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
-  dfree(db);
+  ds.drop(db);
   const double w32 = (double)(float)h->omega;
   for (int32_t r = 0; r < nne; r++)
     for (int64_t p = rp[r]; p < rp[r + 1]; p++) {
@@ -475,6 +482,7 @@ int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_
 static int pad_cols(int n) { return n <= 2 ? 2 : (n <= 4 ? 4 : (n <= 8 ? 8 : (n <= 16 ? 16 : -1))); }
 
 int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_solve_info *info) {
+  DevScope ds;
   if (!h || !rhs_inout) return MVF_E_ARG;
   if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede mvf_solve");
   mvf_solver_opts od;
@@ -492,8 +500,8 @@ int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_so
       hb[4 * (size_t)r + 2 * d + 1] = rhs_inout[2 * ((size_t)d * nne + r) + 1];
     }
   double *db, *dx;
-  MVF_TRY(dalloc(h, &db, (size_t)nne * 4));
-  MVF_TRY(dalloc(h, &dx, (size_t)nne * 4));
+  MVF_TRY(ds.alloc(h, &db, (size_t)nne * 4));
+  MVF_TRY(ds.alloc(h, &dx, (size_t)nne * 4));
   MVF_HIP(h, hipMemcpyAsync(db, hb.data(), 32 * (size_t)nne, hipMemcpyHostToDevice, h->stream));
   MVF_HIP(h, hipMemsetAsync(dx, 0, 32 * (size_t)nne, h->stream));
   double om[2] = {h->omega, h->omega};
@@ -501,9 +509,10 @@ int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_so
   int rc = solve_device(h, om, 2, 2, o, db, dx, info);
   (void)hipStreamSynchronize(h->stream);
   info->seconds = now() - t0;
+  info->operator_fp32 = h->ops_fp32;
   if (hipMemcpy(hb.data(), dx, 32 * (size_t)nne, hipMemcpyDeviceToHost) != hipSuccess && !rc) rc = MVF_E_DEVICE;
-  dfree(db);
-  dfree(dx);
+  ds.drop(db);
+  ds.drop(dx);
   for (int32_t r = 0; r < nne; r++)
     for (int d = 0; d < 2; d++) {
       rhs_inout[2 * ((size_t)d * nne + r)] = hb[4 * (size_t)r + 2 * d];
@@ -514,7 +523,8 @@ int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_so
 
 int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, const mvf_solver_opts *o,
                           mvf_solve_info *info) {
-  if (!h || !freqs_hz || nfreq < 1 || nfreq > 8) return h ? set_err(h, MVF_E_ARG, "nfreq must be 1..8") : MVF_E_ARG;
+  DevScope ds;
+  if (!h || !freqs_hz || nfreq < 1 || nfreq > 32) return h ? set_err(h, MVF_E_ARG, "nfreq must be 1..32") : MVF_E_ARG;
   mvf_solver_opts od;
   mvf_default_solver_opts(&od);
   if (!o) o = &od;
@@ -541,15 +551,16 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
   int f = 0;
   int rc = 0;
   while (f < nfreq && !rc) {
-    int nb = nfreq - f;
+    // at most 8 frequencies (16 columns) share one batched sweep
+    int nb = std::min(nfreq - f, 8);
     if (h->carry_nonzero && h->calls == 0) nb = 1;
     // the direct preconditioner serves freqs_per_factor consecutive frequencies
     if (o->precond == MVF_PC_DIRECT) nb = std::min(nb, std::max(1, std::min(o->freqs_per_factor, 8)));
     MVF_TRY(ensure_call_state(h, h->calls));
     int pc = pad_cols(2 * nb);
     double *bb, *xb;
-    MVF_TRY(dalloc(h, &bb, (size_t)nne * pc * 2));
-    MVF_TRY(dalloc(h, &xb, (size_t)nne * pc * 2));
+    MVF_TRY(ds.alloc(h, &bb, (size_t)nne * pc * 2));
+    MVF_TRY(ds.alloc(h, &xb, (size_t)nne * pc * 2));
     MVF_HIP(h, hipMemsetAsync(bb, 0, 16 * (size_t)nne * pc, h->stream));
     MVF_HIP(h, hipMemsetAsync(xb, 0, 16 * (size_t)nne * pc, h->stream));
     double om[16] = {0};
@@ -591,13 +602,14 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
     MVF_HIP(h, hipMemcpy2DAsync(h->d_b + 2 * (size_t)(2 * f), 16 * (size_t)ncol, bb, 16 * (size_t)pc, 16 * (size_t)(2 * nb),
                                 nne, hipMemcpyDeviceToDevice, h->stream));
     MVF_HIP(h, hipStreamSynchronize(h->stream));
-    dfree(bb);
-    dfree(xb);
+    ds.drop(bb);
+    ds.drop(xb);
     h->calls += nb;
     f += nb;
   }
   info->converged = rc == 0;
   info->seconds = now() - t0;
+  info->operator_fp32 = h->ops_fp32;
   return rc;
 }
 
@@ -621,6 +633,7 @@ int mvf_get_rhs(mvf_handle *h, double *b_out, int32_t ncols) { return fetch_cols
 int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32_t nrx, const double *rx,
                 const double *ry, double *esol, double *hsol, double *rx_e, double *rx_h, double *rx_z, double *rho,
                 double *phi, double *rx_elev, double *node_z, double *node_rho, double *node_phi) {
+  DevScope ds;
   if (!h) return MVF_E_ARG;
   MVF_HIP(h, hipSetDevice(h->device));
   const Geom &g = h->geo;
@@ -639,7 +652,7 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
         t[4 * r + 2 * d] = x[2 * ((size_t)d * h->nne + r)];
         t[4 * r + 2 * d + 1] = x[2 * ((size_t)d * h->nne + r) + 1];
       }
-    MVF_TRY(dalloc(h, &dx, t.size()));
+    MVF_TRY(ds.alloc(h, &dx, t.size()));
     MVF_HIP(h, hipMemcpy(dx, t.data(), t.size() * 8, hipMemcpyHostToDevice));
   } else {
     if (!h->d_x || f < 0 || f >= (int)h->last_freqs.size())
@@ -672,7 +685,7 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
             }
         }
       if (ie < 0 || je < 0) {
-        dfree(dx);
+        ds.drop(dx);
         return set_err(h, MVF_E_ARG,
                        "mvf_readout: receiver %d (%g, %g) lies outside the inner cells and no earlier receiver's "
                        "element is there to inherit (the reference's indices are undefined then)",
@@ -719,8 +732,8 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
       if (e == hipSuccess && rx_elev) e = hipMemcpy(rx_elev, delev, 8 * (size_t)nrx, hipMemcpyDeviceToHost);
       if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
     }
-    dfree(drx); dfree(dry); dfree(die); dfree(dje); dfree(doe); dfree(doh); dfree(doz); dfree(drho); dfree(dphi);
-    dfree(delev);
+    ds.drop(drx); ds.drop(dry); ds.drop(die); ds.drop(dje); ds.drop(doe); ds.drop(doh); ds.drop(doz); ds.drop(drho); ds.drop(dphi);
+    ds.drop(delev);
   }
   if (!rc && (node_z || node_rho || node_phi)) {
     double *dz = nullptr, *drh = nullptr, *dph = nullptr;
@@ -735,7 +748,7 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
       if (e == hipSuccess && node_phi) e = hipMemcpy(node_phi, dph, 8 * (size_t)g.npt * 4, hipMemcpyDeviceToHost);
       if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
     }
-    dfree(dz); dfree(drh); dfree(dph);
+    ds.drop(dz); ds.drop(drh); ds.drop(dph);
   }
   if (!rc) {
     hipError_t e = hipStreamSynchronize(h->stream);
@@ -743,9 +756,9 @@ int mvf_readout(mvf_handle *h, const double *x, int32_t f, int32_t ke_top, int32
     if (e == hipSuccess && hsol) e = hipMemcpy(hsol, hs, nf * 8, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = set_err(h, MVF_E_DEVICE, "mvf_readout: %s", hipGetErrorString(e));
   }
-  dfree(es);
-  dfree(hs);
-  dfree(dx);
+  ds.drop(es);
+  ds.drop(hs);
+  ds.drop(dx);
   return rc;
 }
 

@@ -1927,6 +1927,7 @@ static inline double cabs_(cplx a) { return std::hypot(a.r, a.i); }
 // reorthogonalization pass, Hermitian inner products, FP64 throughout.
 template <int NC>
 static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  DevScope ds;
   mvf_handle *h = S.h;
   const int64_t n = S.n;
   const int64_t nv = n * NC;  // complex entries per vector
@@ -1958,15 +1959,15 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   // no further application of the factor at the end of a cycle
   double2 *V, *Z, *wv, *r;
   double *part, *dout, *coef, *dnorm;
-  MVF_TRY(dalloc(h, &V, (size_t)(mres + 1) * nv));
-  MVF_TRY(dalloc(h, &Z, (size_t)mres * nv));
-  MVF_TRY(dalloc(h, &wv, (size_t)nv));
-  MVF_TRY(dalloc(h, &r, (size_t)nv));
+  MVF_TRY(ds.alloc(h, &V, (size_t)(mres + 1) * nv));
+  MVF_TRY(ds.alloc(h, &Z, (size_t)mres * nv));
+  MVF_TRY(ds.alloc(h, &wv, (size_t)nv));
+  MVF_TRY(ds.alloc(h, &r, (size_t)nv));
   const int nbk = vec_blocks(n, NC);
-  MVF_TRY(dalloc(h, &part, (size_t)(mres + 1) * std::max(nbk, kRedBlocks) * NC * 2 + (size_t)kRedBlocks * 3 * NC * 2));
-  MVF_TRY(dalloc(h, &dout, (size_t)(mres + 1) * NC * 2));
-  MVF_TRY(dalloc(h, &coef, (size_t)(mres + 1) * NC * 2));
-  MVF_TRY(dalloc(h, &dnorm, 4 * NC));
+  MVF_TRY(ds.alloc(h, &part, (size_t)(mres + 1) * std::max(nbk, kRedBlocks) * NC * 2 + (size_t)kRedBlocks * 3 * NC * 2));
+  MVF_TRY(ds.alloc(h, &dout, (size_t)(mres + 1) * NC * 2));
+  MVF_TRY(ds.alloc(h, &coef, (size_t)(mres + 1) * NC * 2));
+  MVF_TRY(ds.alloc(h, &dnorm, 4 * NC));
   hipStream_t st = h->stream;
   const unsigned gv = cap_grid((nv + 255) / 256);
   auto dots = [&](int nvec, const double2 *w, std::vector<cplx> &outv) -> int {
@@ -2125,7 +2126,7 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   info->converged = rc == 0;
   info->precond_applies += applies;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  dfree(V); dfree(Z); dfree(wv); dfree(r); dfree(part); dfree(dout); dfree(coef); dfree(dnorm);
+  ds.drop(V); ds.drop(Z); ds.drop(wv); ds.drop(r); ds.drop(part); ds.drop(dout); ds.drop(coef); ds.drop(dnorm);
   return rc;
 }
 

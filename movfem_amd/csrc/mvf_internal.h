@@ -107,6 +107,7 @@ struct mvf_handle {
   // frequency-independent operators
   double *d_K = nullptr, *d_M = nullptr;
   float *d_K32 = nullptr, *d_M32 = nullptr;
+  int ops_fp32 = 0;  // d_K/d_M were last assembled from FP32 element integration
   double *d_Ke = nullptr, *d_Me = nullptr;  // element matrices (upper packed), kept for mvf_integrate
   int64_t ke_elems = 0;
   // element-0 correction for the carried GPML flags (see MVF_EMU_GPML_PREV_FLAGS)
@@ -186,6 +187,32 @@ inline int dalloc(mvf_handle *h, T **p, size_t n) {
 inline void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
+
+// Temporary device buffers of one call: whatever is allocated through it is
+// freed when the call returns, on every return path (errors included).
+struct DevScope {
+  std::vector<void *> bufs;
+  DevScope() = default;
+  DevScope(const DevScope &) = delete;
+  DevScope &operator=(const DevScope &) = delete;
+  ~DevScope() {
+    for (void *p : bufs)
+      if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(mvf_handle *h, T **p, size_t n) {
+    int rc = dalloc(h, p, n);
+    if (!rc) bufs.push_back((void *)*p);
+    return rc;
+  }
+  // free one buffer before the call ends (lowers the peak)
+  void drop(void *p) {
+    if (!p) return;
+    for (auto &q : bufs)
+      if (q == p) q = nullptr;
+    (void)hipFree(p);
+  }
+};
 
 inline int grid1d(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 

@@ -122,19 +122,20 @@ __global__ void k_number(CopyTab T, Geom g, int dirichlet, const uint64_t *mask,
 }
 
 int setup_numbering(mvf_handle *h) {
+  DevScope ds;
   const Geom &g = h->geo;
   CopyTab T = make_copytab(h->et);
   uint64_t *mask;
   int64_t *cnt, *base;
-  MVF_TRY(dalloc(h, &mask, g.ne));
-  MVF_TRY(dalloc(h, &cnt, g.ne));
-  MVF_TRY(dalloc(h, &base, g.ne + 1));
+  MVF_TRY(ds.alloc(h, &mask, g.ne));
+  MVF_TRY(ds.alloc(h, &cnt, g.ne));
+  MVF_TRY(ds.alloc(h, &base, g.ne + 1));
   MVF_TRY(dalloc(h, &h->d_gne, g.ne * h->et.ne));
   hipLaunchKernelGGL(k_newmask, dim3(grid1d(g.ne, 256)), dim3(256), 0, h->stream, T, g, h->bc.dirichlet, mask, cnt);
   size_t tb = 0;
   MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, base, (int)g.ne, h->stream));
   void *tmp;
-  MVF_TRY(dalloc(h, (char **)&tmp, tb));
+  MVF_TRY(ds.alloc(h, (char **)&tmp, tb));
   MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, base, (int)g.ne, h->stream));
   hipLaunchKernelGGL(k_number, dim3(grid1d(g.ne * h->et.ne, 256)), dim3(256), 0, h->stream, T, g, h->bc.dirichlet,
                      mask, base, h->d_gne);
@@ -144,7 +145,7 @@ int setup_numbering(mvf_handle *h) {
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   MVF_HIP(h, hipGetLastError());
   int64_t nne = last_base + last_cnt;
-  dfree(mask); dfree(cnt); dfree(base); dfree(tmp);
+  ds.drop(mask); ds.drop(cnt); ds.drop(base); ds.drop(tmp);
   if (nne <= 0 || nne > INT32_MAX) return set_err(h, MVF_E_ARG, "DOF count %lld out of range", (long long)nne);
   h->nne = (int32_t)nne;
   return 0;
@@ -252,19 +253,20 @@ static int end_bit_for(uint64_t maxkey) {
 }
 
 int setup_pattern(mvf_handle *h) {
+  DevScope ds;
   const Geom &g = h->geo;
   int me = h->et.ne;
   int64_t nne = h->nne;
   int64_t *cnt, *off, *rcnt, *roff;
-  MVF_TRY(dalloc(h, &cnt, g.ne));
-  MVF_TRY(dalloc(h, &off, g.ne));
-  MVF_TRY(dalloc(h, &rcnt, g.ne));
-  MVF_TRY(dalloc(h, &roff, g.ne));
+  MVF_TRY(ds.alloc(h, &cnt, g.ne));
+  MVF_TRY(ds.alloc(h, &off, g.ne));
+  MVF_TRY(ds.alloc(h, &rcnt, g.ne));
+  MVF_TRY(ds.alloc(h, &roff, g.ne));
   hipLaunchKernelGGL(k_validcount, dim3(grid1d(g.ne, 256)), dim3(256), 0, h->stream, h->d_gne, me, g.ne, cnt, rcnt);
   size_t tb = 0, tb2 = 0;
   MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, (int)g.ne, h->stream));
   char *tmp;
-  MVF_TRY(dalloc(h, &tmp, tb));
+  MVF_TRY(ds.alloc(h, &tmp, tb));
   MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, off, (int)g.ne, h->stream));
   MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(tmp, tb, rcnt, roff, (int)g.ne, h->stream));
   int64_t lo = 0, lc = 0, rlo = 0, rlc = 0;
@@ -273,47 +275,47 @@ int setup_pattern(mvf_handle *h) {
   MVF_HIP(h, hipMemcpyAsync(&rlo, roff + g.ne - 1, 8, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipMemcpyAsync(&rlc, rcnt + g.ne - 1, 8, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
-  dfree(tmp);
+  ds.drop(tmp);
   int64_t np = lo + lc, nr = rlo + rlc;
   h->npairs = np;
   uint64_t *k1, *k2, *rk1, *rk2;
   int64_t *v1, *rv1;
-  MVF_TRY(dalloc(h, &k1, np));
-  MVF_TRY(dalloc(h, &k2, np));
-  MVF_TRY(dalloc(h, &v1, np));
+  MVF_TRY(ds.alloc(h, &k1, np));
+  MVF_TRY(ds.alloc(h, &k2, np));
+  MVF_TRY(ds.alloc(h, &v1, np));
   MVF_TRY(dalloc(h, &h->d_contrib, np));
-  MVF_TRY(dalloc(h, &rk1, nr));
-  MVF_TRY(dalloc(h, &rk2, nr));
-  MVF_TRY(dalloc(h, &rv1, nr));
+  MVF_TRY(ds.alloc(h, &rk1, nr));
+  MVF_TRY(ds.alloc(h, &rk2, nr));
+  MVF_TRY(ds.alloc(h, &rv1, nr));
   MVF_TRY(dalloc(h, &h->d_rcontrib, nr));
   hipLaunchKernelGGL(k_pairs, dim3((unsigned)g.ne), dim3(256), 0, h->stream, h->d_gne, me, nne, off, roff, k1, v1,
                      rk1, rv1);
   MVF_HIP(h, hipGetLastError());
-  dfree(cnt); dfree(off); dfree(rcnt); dfree(roff);
+  ds.drop(cnt); ds.drop(off); ds.drop(rcnt); ds.drop(roff);
   int eb = end_bit_for((uint64_t)nne * (uint64_t)nne);
   int reb = end_bit_for((uint64_t)nne);
   MVF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, v1, h->d_contrib, (int64_t)np, 0, eb, h->stream));
   MVF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, rk1, rk2, rv1, h->d_rcontrib, (int64_t)nr, 0, reb,
                                                  h->stream));
   size_t tbm = tb > tb2 ? tb : tb2;
-  MVF_TRY(dalloc(h, &tmp, tbm));
+  MVF_TRY(ds.alloc(h, &tmp, tbm));
   MVF_HIP(h, hipcub::DeviceRadixSort::SortPairs(tmp, tbm, k1, k2, v1, h->d_contrib, (int64_t)np, 0, eb, h->stream));
   MVF_HIP(h, hipcub::DeviceRadixSort::SortPairs(tmp, tbm, rk1, rk2, rv1, h->d_rcontrib, (int64_t)nr, 0, reb,
                                                  h->stream));
-  dfree(tmp);
-  dfree(k1); dfree(v1); dfree(rk1); dfree(rv1);
+  ds.drop(tmp);
+  ds.drop(k1); ds.drop(v1); ds.drop(rk1); ds.drop(rv1);
   // unique entries
   int64_t *head, *scan;
-  MVF_TRY(dalloc(h, &head, np));
-  MVF_TRY(dalloc(h, &scan, np));
+  MVF_TRY(ds.alloc(h, &head, np));
+  MVF_TRY(ds.alloc(h, &scan, np));
   hipLaunchKernelGGL(k_heads, dim3(grid1d(np, 256)), dim3(256), 0, h->stream, k2, np, head);
   MVF_HIP(h, hipcub::DeviceScan::InclusiveSum(nullptr, tb, head, scan, (int64_t)np, h->stream));
-  MVF_TRY(dalloc(h, &tmp, tb));
+  MVF_TRY(ds.alloc(h, &tmp, tb));
   MVF_HIP(h, hipcub::DeviceScan::InclusiveSum(tmp, tb, head, scan, (int64_t)np, h->stream));
   int64_t nnz = 0;
   MVF_HIP(h, hipMemcpyAsync(&nnz, scan + np - 1, 8, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
-  dfree(tmp);
+  ds.drop(tmp);
   h->nnz = nnz;
   MVF_TRY(dalloc(h, &h->d_seg, nnz + 1));
   MVF_TRY(dalloc(h, &h->d_col, nnz));
@@ -321,7 +323,7 @@ int setup_pattern(mvf_handle *h) {
   MVF_TRY(dalloc(h, &h->d_diagpos, nne));
   MVF_TRY(dalloc(h, &h->d_rseg, nne + 1));
   int64_t *ntriu;
-  MVF_TRY(dalloc(h, &ntriu, 1));
+  MVF_TRY(ds.alloc(h, &ntriu, 1));
   MVF_HIP(h, hipMemsetAsync(ntriu, 0, 8, h->stream));
   hipLaunchKernelGGL(k_entries, dim3(grid1d(np, 256)), dim3(256), 0, h->stream, k2, scan, head, np, nne, h->d_seg,
                      h->d_col, h->d_rowptr, ntriu);
@@ -333,7 +335,7 @@ int setup_pattern(mvf_handle *h) {
   MVF_HIP(h, hipMemcpyAsync(&h->nnz_triu, ntriu, 8, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   MVF_HIP(h, hipGetLastError());
-  dfree(head); dfree(scan); dfree(k2); dfree(rk2); dfree(ntriu);
+  ds.drop(head); ds.drop(scan); ds.drop(k2); ds.drop(rk2); ds.drop(ntriu);
   return 0;
 }
 

@@ -766,17 +766,18 @@ static int true_resid(const Solver &S, const double *b, const double *x, double 
 
 template <int NC>
 static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  DevScope ds;
   mvf_handle *h = S.h;
   int64_t n = S.n, nv = n * NC * 2;
   double *r, *p, *q, *invd, *part, *dnorm;
   ColState *st;
-  MVF_TRY(dalloc(h, &r, nv));
-  MVF_TRY(dalloc(h, &p, nv));
-  MVF_TRY(dalloc(h, &q, nv));
-  MVF_TRY(dalloc(h, &invd, nv));
-  MVF_TRY(dalloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2 + (size_t)spmv_blocks(n) * NC * 2));
-  MVF_TRY(dalloc(h, &dnorm, 4 * NC));
-  MVF_TRY(dalloc(h, &st, NC));
+  MVF_TRY(ds.alloc(h, &r, nv));
+  MVF_TRY(ds.alloc(h, &p, nv));
+  MVF_TRY(ds.alloc(h, &q, nv));
+  MVF_TRY(ds.alloc(h, &invd, nv));
+  MVF_TRY(ds.alloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2 + (size_t)spmv_blocks(n) * NC * 2));
+  MVF_TRY(ds.alloc(h, &dnorm, 4 * NC));
+  MVF_TRY(ds.alloc(h, &st, NC));
   double *spart = part + (size_t)kRedBlocks * 3 * NC * 2;
   if (o->precond == MVF_PC_JACOBI) {
     if (S.c64)
@@ -839,28 +840,29 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
   info->fallbacks = restarts;
   info->converged = rc == 0;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  dfree(r); dfree(p); dfree(q); dfree(invd); dfree(part); dfree(dnorm); dfree(st);
+  ds.drop(r); ds.drop(p); ds.drop(q); ds.drop(invd); ds.drop(part); ds.drop(dnorm); ds.drop(st);
   return rc;
 }
 
 // mixed precision: float32 COCG recurrences, FP64 x and FP64 true residual replacements
 template <int NC>
 static int run_cocg_mixed(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  DevScope ds;
   mvf_handle *h = S.h;
   int64_t n = S.n, nv = n * NC * 2;
   float *r, *p, *q, *x32;
   double *r64, *q64, *invd, *part, *dnorm;
   ColState *st;
-  MVF_TRY(dalloc(h, &r, nv));
-  MVF_TRY(dalloc(h, &p, nv));
-  MVF_TRY(dalloc(h, &q, nv));
-  MVF_TRY(dalloc(h, &x32, nv));
-  MVF_TRY(dalloc(h, &r64, nv));
-  MVF_TRY(dalloc(h, &q64, nv));
-  MVF_TRY(dalloc(h, &invd, nv));
-  MVF_TRY(dalloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2 + (size_t)spmv_blocks(n) * NC * 2));
-  MVF_TRY(dalloc(h, &dnorm, 4 * NC));
-  MVF_TRY(dalloc(h, &st, NC));
+  MVF_TRY(ds.alloc(h, &r, nv));
+  MVF_TRY(ds.alloc(h, &p, nv));
+  MVF_TRY(ds.alloc(h, &q, nv));
+  MVF_TRY(ds.alloc(h, &x32, nv));
+  MVF_TRY(ds.alloc(h, &r64, nv));
+  MVF_TRY(ds.alloc(h, &q64, nv));
+  MVF_TRY(ds.alloc(h, &invd, nv));
+  MVF_TRY(ds.alloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2 + (size_t)spmv_blocks(n) * NC * 2));
+  MVF_TRY(ds.alloc(h, &dnorm, 4 * NC));
+  MVF_TRY(ds.alloc(h, &st, NC));
   double *spart = part + (size_t)kRedBlocks * 3 * NC * 2;
   if (S.c64)
     hipLaunchKernelGGL((k_invdiag<NC, true>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, n, h->d_diagpos, h->d_K,
@@ -951,28 +953,29 @@ static int run_cocg_mixed(Solver &S, const mvf_solver_opts *o, const double *b, 
   info->refinements = refinements;
   info->converged = rc == 0;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  dfree(r); dfree(p); dfree(q); dfree(x32); dfree(r64); dfree(q64); dfree(invd); dfree(part); dfree(dnorm); dfree(st);
+  ds.drop(r); ds.drop(p); ds.drop(q); ds.drop(x32); ds.drop(r64); ds.drop(q64); ds.drop(invd); ds.drop(part); ds.drop(dnorm); ds.drop(st);
   return rc;
 }
 
 template <int NC>
 static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  DevScope ds;
   mvf_handle *h = S.h;
   int64_t n = S.n, nv = n * NC * 2;
   double *r, *rh, *p, *v, *s, *t, *ph, *sh, *invd, *part, *dnorm;
   ColState *st;
-  MVF_TRY(dalloc(h, &r, nv));
-  MVF_TRY(dalloc(h, &rh, nv));
-  MVF_TRY(dalloc(h, &p, nv));
-  MVF_TRY(dalloc(h, &v, nv));
-  MVF_TRY(dalloc(h, &s, nv));
-  MVF_TRY(dalloc(h, &t, nv));
-  MVF_TRY(dalloc(h, &ph, nv));
-  MVF_TRY(dalloc(h, &sh, nv));
-  MVF_TRY(dalloc(h, &invd, nv));
-  MVF_TRY(dalloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2));
-  MVF_TRY(dalloc(h, &dnorm, 4 * NC));
-  MVF_TRY(dalloc(h, &st, NC));
+  MVF_TRY(ds.alloc(h, &r, nv));
+  MVF_TRY(ds.alloc(h, &rh, nv));
+  MVF_TRY(ds.alloc(h, &p, nv));
+  MVF_TRY(ds.alloc(h, &v, nv));
+  MVF_TRY(ds.alloc(h, &s, nv));
+  MVF_TRY(ds.alloc(h, &t, nv));
+  MVF_TRY(ds.alloc(h, &ph, nv));
+  MVF_TRY(ds.alloc(h, &sh, nv));
+  MVF_TRY(ds.alloc(h, &invd, nv));
+  MVF_TRY(ds.alloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2));
+  MVF_TRY(ds.alloc(h, &dnorm, 4 * NC));
+  MVF_TRY(ds.alloc(h, &st, NC));
   if (o->precond == MVF_PC_JACOBI) {
     if (S.c64)
       hipLaunchKernelGGL((k_invdiag<NC, true>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, n, h->d_diagpos, h->d_K,
@@ -1036,8 +1039,8 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
   info->fallbacks = restarts;
   info->converged = rc == 0;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  dfree(r); dfree(rh); dfree(p); dfree(v); dfree(s); dfree(t); dfree(ph); dfree(sh); dfree(invd); dfree(part);
-  dfree(dnorm); dfree(st);
+  ds.drop(r); ds.drop(rh); ds.drop(p); ds.drop(v); ds.drop(s); ds.drop(t); ds.drop(ph); ds.drop(sh); ds.drop(invd); ds.drop(part);
+  ds.drop(dnorm); ds.drop(st);
   return rc;
 }
 

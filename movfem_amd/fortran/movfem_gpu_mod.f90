@@ -52,7 +52,7 @@ module movfem_gpu
     real(c_double) :: spmv_ms(2), spmv_bytes(2)
     integer(c_int32_t) :: spmv_ncol, pad_
     real(c_double) :: factor_seconds, factor_flops, factor_bytes
-    integer(c_int32_t) :: factorizations, static_pivots, precond_applies, pad2_
+    integer(c_int32_t) :: factorizations, static_pivots, precond_applies, operator_fp32
     real(c_double) :: gemm_ms, gemm_flops
     integer(c_int64_t) :: gemm_launches
   end type mvf_solve_info

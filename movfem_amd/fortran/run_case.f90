@@ -33,7 +33,7 @@ program run_case
   real(c_double), target, allocatable :: rho(:), phi(:), nrho(:), nphi(:), xs(:), ys(:), zs(:)
   real(c_double), target :: lsig_t(64), ldz_t(64)
   complex(c_double_complex), target, allocatable :: sigma(:), a(:), rhs(:), esol(:), hsol(:), rx_e(:), rx_h(:)
-  complex(c_double_complex), target, allocatable :: rx_z(:), node_z(:), pep(:), php(:), psrc(:)
+  complex(c_double_complex), target, allocatable :: rx_z(:), node_z(:), pep(:), php(:), psrc(:), rhs0(:)
   integer(c_int32_t), target, allocatable :: nzl(:), ia(:), ja(:), sel(:)
   integer(c_int64_t), target, allocatable :: sids(:)
   integer(c_int32_t) :: nne, nx, ny, nz, mn, nextd, nsf, nsr, ord, ke_top, nxb, nyb, nf
@@ -77,7 +77,7 @@ program run_case
   ierr = mvf_init(grid, model, bc, opts, hgpu)
   if (ierr /= 0) call fail('mvf_init', hgpu)
   ierr = mvf_sizes(hgpu, nne, nnze, ntriu)
-  allocate(ia(nnze), ja(nnze), a(nnze), rhs(2*nne))
+  allocate(ia(nnze), ja(nnze), a(nnze), rhs(2*nne), rhs0(2*nne))
   allocate(esol(2*npt*3), hsol(2*npt*3), node_z(4*npt), nrho(4*npt), nphi(4*npt))
   allocate(rx_e(6*max(nsr, 1)), rx_h(6*max(nsr, 1)), rx_z(4*max(nsr, 1)), rho(4*max(nsr, 1)), phi(4*max(nsr, 1)))
   allocate(elev(max(nsr, 1)), pep(2*npt*3), php(2*npt*3), psrc(2*npt*3))
@@ -126,6 +126,8 @@ program run_case
     if (ierr == 0) ierr = mvf_global_vfem(hgpu, ia, ja, a, rhs, nnze, nne, 2)
     if (ierr == 0) ierr = mvf_remove_zeros(nnze, ia, ja, a, nnz)
     if (ierr == 0) then
+      ! mvf_solve overwrites the RHS with the solution, as zmumps(job=6) does
+      rhs0 = rhs
       so%matrix = MVF_MATRIX_C64REF
       so%maxit = 300
       ierr = mvf_solve(hgpu, rhs, so, info)
@@ -135,10 +137,10 @@ program run_case
         ! solve the FP64 operator (same pattern and values before rounding)
         write(*, '(a,i4,a,2es11.3,a)') 'frequency ', ii, ': GMRES stalls at ', info%true_resid(1), &
              info%true_resid(2), ' on the complex64 matrix; solving the FP64 operator'
-        ierr = mvf_global_vfem(hgpu, ia, ja, a, rhs, nnze, nne, 2)
+        rhs = rhs0
         so%matrix = MVF_MATRIX_FP64
         so%maxit = 100000
-        if (ierr == 0) ierr = mvf_solve(hgpu, rhs, so, info)
+        ierr = mvf_solve(hgpu, rhs, so, info)
       end if
       if (ierr /= 0) write(*, '(a,i4,a,i7,a,2es11.3)') 'frequency ', ii, ': iterations ', info%iterations, &
            ' true residuals ', info%true_resid(1), info%true_resid(2)

@@ -62,7 +62,8 @@ def info_dict(info: L.SolveInfo, ncol: int) -> dict:
             "factor_seconds": info.factor_seconds, "factor_flops": info.factor_flops,
             "factor_bytes": info.factor_bytes, "factorizations": info.factorizations,
             "static_pivots": info.static_pivots, "precond_applies": info.precond_applies,
-            "gemm_ms": info.gemm_ms, "gemm_flops": info.gemm_flops, "gemm_launches": info.gemm_launches}
+            "gemm_ms": info.gemm_ms, "gemm_flops": info.gemm_flops, "gemm_launches": info.gemm_launches,
+            "operator_fp32": bool(info.operator_fp32)}
 
 
 class GpuVfem:

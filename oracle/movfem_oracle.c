@@ -25,6 +25,7 @@
  */
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -1480,4 +1481,66 @@ int64_t orc_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a) {
     k++;
   }
   return k;
+}
+
+/* ---------------------- timed element loop (CPU baseline) ---------------------
+ * The element loop of global_vfem (MoVFEM_3DMT.f90:193-211) for elements
+ * [e0, e1) in reference loop order: int_elem_params, then alocal for every
+ * pair im <= jm and blocal for every edge (integration.f90:60-106), on
+ * nthreads OpenMP threads.  Each element takes the GPML zone flags of the
+ * element before it, as get_pml after int_elem_params leaves them
+ * (MoVFEM_3DMT.f90:203-206; element 0 takes none).  Every thread has its own
+ * workspace and a copy of the state's scalars, so the results do not depend
+ * on the thread count.  out[0] = sum of Re alocal, out[1] = sum of Im alocal,
+ * out[2] = sum of |blocal| over the range (a checksum that keeps the work
+ * observable), out[3] = the thread count used.  Returns 0 or the first
+ * element's error code. */
+int orc_element_loop_timed(orc_state *s, int64_t e0, int64_t e1, int nthreads, double *out) {
+  if (nthreads < 1) nthreads = 1;
+  int me = s->me, rc_all = 0;
+  double sr = 0.0, si = 0.0, sb = 0.0;
+  int used = 1;
+#pragma omp parallel num_threads(nthreads) reduction(+ : sr, si, sb)
+  {
+#pragma omp single
+    used = omp_get_num_threads();
+    orc_state st = *s;
+    orc_ws *w = (orc_ws *)malloc(sizeof(orc_ws));
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t e = e0; e < e1; e++) {
+      int ie = (int)(e / ((int64_t)s->ny * s->nz)) + 1;
+      int je = (int)((e / s->nz) % s->ny) + 1;
+      int ke = (int)(e % s->nz) + 1;
+      st.in_pml[0] = st.in_pml[1] = st.in_pml[2] = 0;
+      if (e > 0 && !s->dirichlet) {
+        int64_t p = e - 1;
+        orc_get_pml(&st, (int)(p / ((int64_t)s->ny * s->nz)) + 1, (int)((p / s->nz) % s->ny) + 1,
+                    (int)(p % s->nz) + 1);
+      }
+      int rc = orc_elem_params(&st, w, ie, je, ke);
+      if (rc) {
+#pragma omp critical
+        if (!rc_all) rc_all = rc;
+        continue;
+      }
+      for (int im = 0; im < me; im++) {
+        for (int jm = im; jm < me; jm++) {
+          zc a = orc_alocal(&st, w, im, jm);
+          sr += creal(a);
+          si += cimag(a);
+        }
+        zc b[2];
+        orc_blocal(&st, w, im, b);
+        sb += cabs(b[0]) + cabs(b[1]);
+      }
+    }
+    free(w);
+  }
+  if (out) {
+    out[0] = sr;
+    out[1] = si;
+    out[2] = sb;
+    out[3] = used;
+  }
+  return rc_all;
 }

@@ -73,6 +73,8 @@ def lib():
         L.orc_bd_field.argtypes = [C.c_void_p, C.c_double, dp]
         L.orc_impedance.restype = None
         L.orc_impedance.argtypes = [dp, dp, C.c_double, dp, dp, dp]
+        L.orc_element_loop_timed.restype = C.c_int
+        L.orc_element_loop_timed.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, dp]
         _LIB = L
     return _LIB
 
@@ -124,6 +126,20 @@ class Oracle:
         L = lib()
         L.orc_set_omega(self.h, self.m.omega(i))
         L.orc_update_sigma(self.h)
+
+    def element_loop_timed(self, e0: int, e1: int, nthreads: int = 1):
+        """int_elem_params + alocal (pairs im <= jm) + blocal for elements [e0, e1) in
+        reference loop order on nthreads threads; returns (seconds, threads used,
+        checksum (sum Re a, sum Im a, sum |b|))."""
+        import time
+
+        out = np.zeros(4)
+        t0 = time.perf_counter()
+        rc = lib().orc_element_loop_timed(self.h, int(e0), int(e1), int(nthreads), _dp(out))
+        dt = time.perf_counter() - t0
+        if rc:
+            raise ValueError(f"oracle element loop failed: {rc}")
+        return dt, int(out[3]), out[:3].copy()
 
     def dirichlet_values(self, ie, je, ke) -> np.ndarray:
         """f_boundary of every edge of element (ie,je,ke) (1-based), complex (me, 2)
