@@ -218,6 +218,15 @@ int mvf_get_rhs(mvf_handle *h, double *b_out, int32_t ncols);
  * [2] element integration (K, M), [3] assembly. */
 int mvf_setup_times(const mvf_handle *h, double *t4);
 
+/* The assembled operators as the device holds them: the full symmetric CSR
+ * pattern (0-based rowptr[nne+1] and col[nnze], rows in DOF order, columns
+ * ascending) and the FP64 values of K (curl-curl) and M (mass) at the current
+ * call state, A(omega) = K + i*fl32(omega)*M before the complex64 rounding of
+ * ga_sort_sparse (global_assembly.f90:152-181).  Any pointer may be NULL.
+ * No counterpart in the reference (whose matrix exists only as the COO
+ * global_vfem returns): for host-side residual checks. */
+int mvf_get_operators(mvf_handle *h, int64_t *rowptr, int32_t *col, double *K, double *M);
+
 /* Re-integrates every element's K and M from the resident lattice and model
  * and re-assembles them in place (the element loop of global_vfem,
  * MoVFEM_3DMT.f90:193-211, without any host transfer).  precision 0: FP64

@@ -30,7 +30,7 @@ EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne",
            "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
            "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface",
            "mvf_read_inputs", "mvf_inputs_view", "mvf_inputs_free", "mvf_grid3d", "mvf_mesh_sizes",
-           "mvf_mesh_arrays", "mvf_mesh_free", "mvf_problem_fields", "mvf_write_problem")
+           "mvf_mesh_arrays", "mvf_mesh_free", "mvf_problem_fields", "mvf_write_problem", "mvf_get_operators")
 
 
 class Grid(C.Structure):
@@ -121,6 +121,7 @@ def lib():
     L.mvf_get_solution.argtypes = [vp, dp, C.c_int32]
     L.mvf_get_rhs.argtypes = [vp, dp, C.c_int32]
     L.mvf_setup_times.argtypes = [vp, dp]
+    L.mvf_get_operators.argtypes = [vp, C.POINTER(C.c_int64), ip, dp, dp]
     L.mvf_assemble.argtypes = [vp, C.c_int32, dp]
     L.mvf_direct_plan_stats.argtypes = [C.c_int32] * 4 + [ip, C.c_int64, i64p, ip, dp]
     L.mvf_readout.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_int32, dp, dp] + [dp] * 11

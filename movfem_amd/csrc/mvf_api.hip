@@ -347,6 +347,17 @@ int mvf_setup_times(const mvf_handle *h, double *t4) {
   return 0;
 }
 
+int mvf_get_operators(mvf_handle *h, int64_t *rowptr, int32_t *col, double *K, double *M) {
+  if (!h) return MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  if (rowptr) MVF_HIP(h, hipMemcpy(rowptr, h->d_rowptr, 8 * ((size_t)h->nne + 1), hipMemcpyDeviceToHost));
+  if (col) MVF_HIP(h, hipMemcpy(col, h->d_col, 4 * (size_t)h->nnz, hipMemcpyDeviceToHost));
+  if (K) MVF_HIP(h, hipMemcpy(K, h->d_K, 8 * (size_t)h->nnz, hipMemcpyDeviceToHost));
+  if (M) MVF_HIP(h, hipMemcpy(M, h->d_M, 8 * (size_t)h->nnz, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int mvf_get_gne(mvf_handle *h, int32_t *gne_out) {
   if (!h || !gne_out) return MVF_E_ARG;
   MVF_HIP(h, hipSetDevice(h->device));

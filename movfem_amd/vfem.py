@@ -127,6 +127,17 @@ class GpuVfem:
         self._check(self._lib.mvf_assemble(self.h, {"fp64": 0, "fp32": 1}[precision], _dp(ms)))
         return float(ms[0])
 
+    def operators(self):
+        """(rowptr, col, K, M): the device's full CSR pattern (0-based) and the
+        FP64 values of K and M as currently assembled."""
+        rp = np.empty(self.nne + 1, np.int64)
+        col = np.empty(self.nnze, np.int32)
+        K = np.empty(self.nnze)
+        Mv = np.empty(self.nnze)
+        self._check(self._lib.mvf_get_operators(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)), _ip(col), _dp(K),
+                                                _dp(Mv)))
+        return rp, col, K, Mv
+
     def gne(self) -> np.ndarray:
         out = np.empty(self.m.ne * self.me, np.int32)
         self._check(self._lib.mvf_get_gne(self.h, _ip(out)))

@@ -1,0 +1,106 @@
+"""Mixed precision on the device: FP32 element integration and FP32 inner
+Krylov iterations inside FP64 iterative refinement, each with its error
+against the FP64 path stated.
+
+FP32 integration (mvf_assemble precision 1) evaluates int_elem_params and
+alocal's sums (integration.f90:60-106) in float32 and assembles K and M from
+those element sums.  Bars, from the float32 unit roundoff u = 2^-24 and the
+length of the sums (up to 27 Gauss points x the 3x3 tensor contraction):
+  * every assembled entry within 2e-5 * sqrt(|X_ii X_jj|) of the FP64 one
+    (X = K or M; both are sums of symmetric semidefinite element matrices in
+    the Dirichlet case, so sqrt(X_ii X_jj) bounds |X_ij| and the terms summed);
+  * the solution of the FP32-integrated operator (mixed COCG: FP32 inner
+    iterations, FP64 true residual <= 1e-8 on that operator) against SciPy
+    SuperLU on the FP64-integrated reference matrix: secondary E within 1e-4
+    relative L2 on the gauge-determined DOFs, receiver rho_a within 0.1 % and
+    phase within 0.05 degrees.
+The FP32 inner Krylov on the FP64 operator (mixed COCG) meets the FP64 path's
+1e-6 bar: the refinement judges the FP64 residual.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from movfem_amd import model as M
+from movfem_amd.vfem import GpuVfem, MvfError
+from oracle.oracle import Oracle
+from tests.gauge import determined_dofs
+from tests.test_gpu_configs import config1_receivers
+
+pytestmark = pytest.mark.gpu
+
+
+def _norm_err(X32, X64, rp, col):
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    diag = X64[rp[:-1] + np.array([np.searchsorted(col[rp[i]:rp[i + 1]], i) for i in range(n)])]
+    scale = np.sqrt(np.abs(diag[rows] * diag[col]))
+    scale = np.where(scale > 0, scale, np.abs(X64).max())
+    return float((np.abs(X32 - X64) / scale).max())
+
+
+@pytest.mark.parametrize("case", ["halfspace_block", "hex20_gpml"])
+def test_fp32_integration_operators(case, need_gpu):
+    if case == "halfspace_block":
+        m = M.halfspace_block()
+    else:
+        m = M.random_small(20, 4, 4, 4, seed=31, dirichlet=False, nextd=1, anisotropic=True, freqs=(1.0,))
+    print(f"fp32 integration {case}", flush=True)
+    g = GpuVfem(m)
+    rp, col, K64, M64 = g.operators()
+    t32 = g.assemble("fp32")
+    _, _, K32, M32 = g.operators()
+    g.set_frequency(0)
+    with pytest.raises(MvfError):
+        g.global_vfem()  # the reference's FP64 matrix is not what the handle holds now
+    t64 = g.assemble("fp64")
+    _, _, K64b, M64b = g.operators()
+    assert np.array_equal(K64, K64b) and np.array_equal(M64, M64b)
+    ek, em = _norm_err(K32, K64, rp, col), _norm_err(M32, M64, rp, col)
+    fk = np.linalg.norm(K32 - K64) / np.linalg.norm(K64)
+    fm = np.linalg.norm(M32 - M64) / np.linalg.norm(M64)
+    print(f"{case}: K max normalized err {ek:.2e} (Frobenius {fk:.2e}), M {em:.2e} ({fm:.2e}); "
+          f"element kernel {t64:.3f} ms fp64, {t32:.3f} ms fp32", flush=True)
+    assert ek <= 2e-5 and em <= 2e-5
+
+
+def test_fp32_integration_solution_error(need_gpu):
+    """Config 1: FP32 integration + mixed COCG against the FP64 reference solve."""
+    m = M.halfspace_block()
+    g = GpuVfem(m)
+    g.set_frequency(0)
+    ia, ja, a, b = g.global_vfem()
+    ia, ja, a = g.remove_zeros(ia, ja, a)
+    n = g.nne
+    U = sp.coo_matrix((a, (ia - 1, ja - 1)), shape=(n, n)).tocsr()
+    A = (U + sp.triu(U, 1).T).tocsc()
+    lu = spla.splu(A)
+    b2 = b.reshape(2, n)
+    xref = np.stack([lu.solve(b2[d]) for d in range(2)])
+    mask = determined_dofs(m, g.gne())
+    xs, ys = config1_receivers(m)
+    ke_top = m.nz - 4
+    o = Oracle(m)
+    o.set_frequency(0)
+    oes, ohs = o.node_solution(xref.ravel())
+    _, _, _, rho_ref, phi_ref = o.receivers(oes, ohs, ke_top, xs, ys)
+    results = {}
+    for label, prec_int in (("fp64 integration, mixed COCG", "fp64"), ("fp32 integration, mixed COCG", "fp32")):
+        g.assemble(prec_int)
+        x, info = g.solve(b, method="cocg", precision="mixed", matrix="fp64", tol=1e-8, maxit=200000)
+        assert info["converged"], info
+        assert info["operator_fp32"] == (prec_int == "fp32")
+        x = x.reshape(2, n)
+        err = max(np.linalg.norm(x[d][mask] - xref[d][mask]) / np.linalg.norm(xref[d][mask]) for d in range(2))
+        out = g.readout(x.ravel(), ke_top=ke_top, rx=xs, ry=ys)
+        drho = max(float((np.abs(out[5][:, c] - rho_ref[:, c]) / rho_ref[:, c]).max()) for c in (1, 2))
+        dphi = max(float(np.abs(out[6][:, c] - phi_ref[:, c]).max()) for c in (1, 2))
+        results[label] = (err, drho, dphi, info["iterations"], info["refinements"])
+        print(f"{label}: E rel L2 {err:.2e}, rho_a {drho:.2e}, phase {dphi:.2e} deg, "
+              f"{info['iterations']} iterations, {info['refinements']} refinements", flush=True)
+    g.assemble("fp64")
+    e64, r64, p64 = results["fp64 integration, mixed COCG"][:3]
+    e32, r32, p32 = results["fp32 integration, mixed COCG"][:3]
+    assert e64 <= 1e-6 and r64 <= 1e-3 and p64 <= 0.05
+    assert e32 <= 1e-4 and r32 <= 1e-3 and p32 <= 0.05
