@@ -202,6 +202,19 @@ void mvf_default_solver_opts(mvf_solver_opts *o);
  * global_vfem filled it; it is overwritten by the solution, as MUMPS does. */
 int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_solve_info *info);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  mvf_solve for method COCG or BiCGStab in FP64 with the
+ * Jacobi or no preconditioner, the same kernels, with every buffer (Krylov
+ * vectors, reduction partials, the staging of rhs_inout) in one static device
+ * array of 64 MiB per GPU instead of allocations: COCG up to about 299,000
+ * DOFs, BiCGStab up to about 174,000; larger systems are MVF_E_ARG, the
+ * direct preconditioner, GMRES and MIXED are MVF_E_UNSUPPORTED.  It calls no
+ * allocating routine.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU race: the caller serializes them.  rhs_inout as in
+ * mvf_solve (overwritten by the solution). */
+int mvf_solve_noalloc(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_solve_info *info);
+
 /* Device-resident path: RHS integrated on the device for each of nfreq
  * frequencies (processed in that order, as the reference's frequency loop
  * would), solved in batches of up to 8 frequencies (16 columns) per batched

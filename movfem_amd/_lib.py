@@ -21,6 +21,9 @@ MVF_PC_NONE = 0
 MVF_PC_JACOBI = 1
 MVF_PC_DIRECT = 2
 
+MVF_OK, MVF_E_ARG, MVF_E_DEVICE, MVF_E_SINGULAR, MVF_E_INDEX = 0, -1, -2, -3, -4
+MVF_E_UNSUPPORTED, MVF_E_NOCONV, MVF_E_BREAKDOWN, MVF_E_MEMORY = -5, -6, -7, -8
+
 ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR", -4: "MVF_E_INDEX",
           -5: "MVF_E_UNSUPPORTED", -6: "MVF_E_NOCONV", -7: "MVF_E_BREAKDOWN", -8: "MVF_E_MEMORY"}
 
@@ -30,7 +33,8 @@ EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne",
            "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
            "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface",
            "mvf_read_inputs", "mvf_inputs_view", "mvf_inputs_free", "mvf_grid3d", "mvf_mesh_sizes",
-           "mvf_mesh_arrays", "mvf_mesh_free", "mvf_problem_fields", "mvf_write_problem", "mvf_get_operators")
+           "mvf_mesh_arrays", "mvf_mesh_free", "mvf_problem_fields", "mvf_write_problem", "mvf_get_operators",
+           "mvf_solve_noalloc")
 
 
 class Grid(C.Structure):
@@ -117,6 +121,7 @@ def lib():
     L.mvf_default_solver_opts.restype = None
     L.mvf_default_solver_opts.argtypes = [C.POINTER(SolverOpts)]
     L.mvf_solve.argtypes = [vp, dp, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
+    L.mvf_solve_noalloc.argtypes = [vp, dp, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
     L.mvf_solve_frequencies.argtypes = [vp, dp, C.c_int32, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
     L.mvf_get_solution.argtypes = [vp, dp, C.c_int32]
     L.mvf_get_rhs.argtypes = [vp, dp, C.c_int32]

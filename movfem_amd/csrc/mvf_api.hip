@@ -147,6 +147,8 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
     h->bd_hsigma = bc->h_sigma;
   }
   MVF_HIP(h, hipSetDevice(h->device));
+  // device address of this GPU's no-allocation scratch array (mvf_solve_noalloc)
+  MVF_TRY(noalloc_setup(h));
   // a blocking stream: synchronous runtime calls on the null stream stay ordered after its work
   MVF_HIP(h, hipStreamCreate(&h->stream));
   h->nodes = grid->nodes_per_elem;

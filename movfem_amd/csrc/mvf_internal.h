@@ -240,5 +240,6 @@ void direct_invalidate(mvf_handle *h);
 int bd_setup(mvf_handle *h, const double *dKe, const double *dMe);
 int bd_values(mvf_handle *h, double omega);
 int bd_lift(mvf_handle *h, double omega, double *d_be);
+int noalloc_setup(mvf_handle *h);
 
 }  // namespace mvf

@@ -705,7 +705,43 @@ struct Solver {
   int lpr;
   int64_t n;  // nne
   SpmvTimer *tm;
+  // device block the Krylov buffers are carved from; nullptr: allocated per call
+  char *ws = nullptr;
+  size_t ws_cap = 0;
 };
+
+// Krylov buffers of one solve: allocated (freed on return), carved from a
+// caller's device block, or only counted (sizing).
+struct Workspace {
+  DevScope ds;
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  bool sizing = false;
+  template <typename T>
+  int get(mvf_handle *h, T **p, size_t n) {
+    size_t bytes = (((n ? n : 1) * sizeof(T)) + 255) & ~(size_t)255;
+    if (sizing) {
+      used += bytes;
+      *p = nullptr;
+      return 0;
+    }
+    if (!base) return ds.alloc(h, p, n);
+    if (used + bytes > cap)
+      return set_err(h, MVF_E_ARG, "Krylov workspace needs more than the %zu bytes provided", cap);
+    *p = (T *)(base + used);
+    used += bytes;
+    return 0;
+  }
+};
+
+// invd = 1 in every column (no preconditioner)
+__global__ void k_unit_diag(int64_t nv, double *invd) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv / 2) {
+    invd[2 * i] = 1.0;
+    invd[2 * i + 1] = 0.0;
+  }
+}
 
 constexpr int kSpmvBlocks = 4096;
 static int spmv_blocks(int64_t n) {
@@ -765,19 +801,35 @@ static int true_resid(const Solver &S, const double *b, const double *x, double 
 }
 
 template <int NC>
-static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
-  DevScope ds;
-  mvf_handle *h = S.h;
-  int64_t n = S.n, nv = n * NC * 2;
+struct CocgBufs {
   double *r, *p, *q, *invd, *part, *dnorm;
   ColState *st;
-  MVF_TRY(ds.alloc(h, &r, nv));
-  MVF_TRY(ds.alloc(h, &p, nv));
-  MVF_TRY(ds.alloc(h, &q, nv));
-  MVF_TRY(ds.alloc(h, &invd, nv));
-  MVF_TRY(ds.alloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2 + (size_t)spmv_blocks(n) * NC * 2));
-  MVF_TRY(ds.alloc(h, &dnorm, 4 * NC));
-  MVF_TRY(ds.alloc(h, &st, NC));
+};
+
+template <int NC>
+static int cocg_bufs(Workspace &W, mvf_handle *h, int64_t n, CocgBufs<NC> &B) {
+  const int64_t nv = n * NC * 2;
+  MVF_TRY(W.get(h, &B.r, nv));
+  MVF_TRY(W.get(h, &B.p, nv));
+  MVF_TRY(W.get(h, &B.q, nv));
+  MVF_TRY(W.get(h, &B.invd, nv));
+  MVF_TRY(W.get(h, &B.part, (size_t)kRedBlocks * 3 * NC * 2 + (size_t)spmv_blocks(n) * NC * 2));
+  MVF_TRY(W.get(h, &B.dnorm, 4 * NC));
+  MVF_TRY(W.get(h, &B.st, NC));
+  return 0;
+}
+
+template <int NC>
+static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  mvf_handle *h = S.h;
+  int64_t n = S.n, nv = n * NC * 2;
+  Workspace W;
+  W.base = S.ws;
+  W.cap = S.ws_cap;
+  CocgBufs<NC> B;
+  MVF_TRY(cocg_bufs<NC>(W, h, n, B));
+  double *r = B.r, *p = B.p, *q = B.q, *invd = B.invd, *part = B.part, *dnorm = B.dnorm;
+  ColState *st = B.st;
   double *spart = part + (size_t)kRedBlocks * 3 * NC * 2;
   if (o->precond == MVF_PC_JACOBI) {
     if (S.c64)
@@ -787,10 +839,7 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
       hipLaunchKernelGGL((k_invdiag<NC, false>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, n, h->d_diagpos,
                          h->d_K, h->d_M, S.w, invd);
   } else {
-    std::vector<double> one(nv, 0.0);
-    for (int64_t i = 0; i < nv; i += 2) one[i] = 1.0;
-    MVF_HIP(h, hipMemcpyAsync(invd, one.data(), nv * 8, hipMemcpyHostToDevice, h->stream));
-    MVF_HIP(h, hipStreamSynchronize(h->stream));
+    hipLaunchKernelGGL(k_unit_diag, dim3(grid1d(nv / 2, 256)), dim3(256), 0, h->stream, nv, invd);
   }
   int nb = vec_blocks(n, NC), sb = spmv_blocks(n);
   int it = 0, restarts = 0;
@@ -840,7 +889,6 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
   info->fallbacks = restarts;
   info->converged = rc == 0;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  ds.drop(r); ds.drop(p); ds.drop(q); ds.drop(invd); ds.drop(part); ds.drop(dnorm); ds.drop(st);
   return rc;
 }
 
@@ -958,24 +1006,33 @@ static int run_cocg_mixed(Solver &S, const mvf_solver_opts *o, const double *b, 
 }
 
 template <int NC>
-static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
-  DevScope ds;
-  mvf_handle *h = S.h;
-  int64_t n = S.n, nv = n * NC * 2;
+struct BicgBufs {
   double *r, *rh, *p, *v, *s, *t, *ph, *sh, *invd, *part, *dnorm;
   ColState *st;
-  MVF_TRY(ds.alloc(h, &r, nv));
-  MVF_TRY(ds.alloc(h, &rh, nv));
-  MVF_TRY(ds.alloc(h, &p, nv));
-  MVF_TRY(ds.alloc(h, &v, nv));
-  MVF_TRY(ds.alloc(h, &s, nv));
-  MVF_TRY(ds.alloc(h, &t, nv));
-  MVF_TRY(ds.alloc(h, &ph, nv));
-  MVF_TRY(ds.alloc(h, &sh, nv));
-  MVF_TRY(ds.alloc(h, &invd, nv));
-  MVF_TRY(ds.alloc(h, &part, (size_t)kRedBlocks * 3 * NC * 2));
-  MVF_TRY(ds.alloc(h, &dnorm, 4 * NC));
-  MVF_TRY(ds.alloc(h, &st, NC));
+};
+
+template <int NC>
+static int bicg_bufs(Workspace &W, mvf_handle *h, int64_t n, BicgBufs<NC> &B) {
+  const int64_t nv = n * NC * 2;
+  for (double **v : {&B.r, &B.rh, &B.p, &B.v, &B.s, &B.t, &B.ph, &B.sh, &B.invd}) MVF_TRY(W.get(h, v, nv));
+  MVF_TRY(W.get(h, &B.part, (size_t)kRedBlocks * 3 * NC * 2));
+  MVF_TRY(W.get(h, &B.dnorm, 4 * NC));
+  MVF_TRY(W.get(h, &B.st, NC));
+  return 0;
+}
+
+template <int NC>
+static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
+  mvf_handle *h = S.h;
+  int64_t n = S.n, nv = n * NC * 2;
+  Workspace W;
+  W.base = S.ws;
+  W.cap = S.ws_cap;
+  BicgBufs<NC> B;
+  MVF_TRY(bicg_bufs<NC>(W, h, n, B));
+  double *r = B.r, *rh = B.rh, *p = B.p, *v = B.v, *s = B.s, *t = B.t, *ph = B.ph, *sh = B.sh, *invd = B.invd;
+  double *part = B.part, *dnorm = B.dnorm;
+  ColState *st = B.st;
   if (o->precond == MVF_PC_JACOBI) {
     if (S.c64)
       hipLaunchKernelGGL((k_invdiag<NC, true>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, n, h->d_diagpos, h->d_K,
@@ -984,9 +1041,7 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
       hipLaunchKernelGGL((k_invdiag<NC, false>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, n, h->d_diagpos,
                          h->d_K, h->d_M, S.w, invd);
   } else {
-    std::vector<double> one(nv, 0.0);
-    for (int64_t i = 0; i < nv; i += 2) one[i] = 1.0;
-    MVF_HIP(h, hipMemcpy(invd, one.data(), nv * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_unit_diag, dim3(grid1d(nv / 2, 256)), dim3(256), 0, h->stream, nv, invd);
   }
   int nb = vec_blocks(n, NC);
   int it = 0, restarts = 0, rc = 0;
@@ -1039,8 +1094,6 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
   info->fallbacks = restarts;
   info->converged = rc == 0;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
-  ds.drop(r); ds.drop(rh); ds.drop(p); ds.drop(v); ds.drop(s); ds.drop(t); ds.drop(ph); ds.drop(sh); ds.drop(invd); ds.drop(part);
-  ds.drop(dnorm); ds.drop(st);
   return rc;
 }
 

@@ -178,6 +178,16 @@ module movfem_gpu
       type(mvf_solve_info), intent(out) :: info
     end function mvf_solve
 
+    ! mvf_solve without device allocation (COCG/BiCGStab, FP64): for callers that
+    ! cannot allocate; buffers in a 64 MiB static device array per GPU
+    integer(c_int) function mvf_solve_noalloc(h, rhs, o, info) bind(C, name='mvf_solve_noalloc')
+      import :: c_int, c_ptr, c_double_complex, mvf_solver_opts, mvf_solve_info
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: rhs(*)
+      type(mvf_solver_opts), intent(in) :: o
+      type(mvf_solve_info), intent(out) :: info
+    end function mvf_solve_noalloc
+
     ! node_solution + rx_data + z_rho_phi on the device; pass c_null_ptr for
     ! outputs not wanted and for x to read frequency f of the last batch
     integer(c_int) function mvf_readout(h, x, f, ke_top, nrx, rx, ry, esol, hsol, rx_e, rx_h, rx_z, rho, phi, &

@@ -200,6 +200,19 @@ class GpuVfem:
             raise MvfError(rc, self._lib.mvf_last_error(self.h).decode() + f" {d}")
         return x, d
 
+    def solve_noalloc(self, rhs, **kw):
+        """mvf_solve_noalloc: as solve() for COCG/BiCGStab (FP64, Jacobi or no
+        preconditioner) with every device buffer in the library's static
+        scratch array; no device allocation."""
+        x = np.ascontiguousarray(rhs, np.complex128).copy()
+        o = solver_opts(**kw)
+        info = L.SolveInfo()
+        rc = self._lib.mvf_solve_noalloc(self.h, _dp(x.view(np.float64)), C.byref(o), C.byref(info))
+        d = info_dict(info, 2)
+        if rc:
+            raise MvfError(rc, self._lib.mvf_last_error(self.h).decode() + f" {d}")
+        return x, d
+
     def solve_frequencies(self, freqs, **kw):
         """Device RHS for each frequency (in order) and one batched solve."""
         f = np.ascontiguousarray(freqs, np.float64)
