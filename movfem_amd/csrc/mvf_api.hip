@@ -112,6 +112,7 @@ void mvf_free(mvf_handle *h) {
                   h->d_rcontrib, h->d_K, h->d_M, h->d_K32, h->d_M32, h->d_Ke, h->d_Me, h->d_dK0, h->d_dM0,
                   h->d_x,  h->d_b, h->d_ntab, h->d_bel, h->d_bidx, h->d_bK, h->d_bM, h->d_dirf, h->d_btab};
   for (void *p : ptrs) dfree(p);
+  for (void *p : h->owned) dfree(p);
   direct_free(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -147,6 +148,12 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
     h->bd_hsigma = bc->h_sigma;
   }
   MVF_HIP(h, hipSetDevice(h->device));
+  {
+    // MVF_ELEM_KERNEL=valu selects the one-workgroup-per-element VALU element
+    // kernel instead of the matrix-core one (same results to rounding)
+    const char *ek = getenv("MVF_ELEM_KERNEL");
+    h->elem_legacy = (ek && strcmp(ek, "valu") == 0) ? 1 : 0;
+  }
   // device address of this GPU's no-allocation scratch array (mvf_solve_noalloc)
   MVF_TRY(noalloc_setup(h));
   // a blocking stream: synchronous runtime calls on the null stream stay ordered after its work
@@ -240,9 +247,12 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
     BasisTab *tab = new BasisTab();
     memset(tab, 0, sizeof(BasisTab));
     fill_basis(h->nodes, tab);
-    MVF_TRY(dalloc(h, &h->d_tab, 1));
-    MVF_HIP(h, hipMemcpy(h->d_tab, tab, sizeof(BasisTab), hipMemcpyHostToDevice));
+    int rc = dalloc(h, &h->d_tab, 1);
+    if (!rc && hipMemcpy(h->d_tab, tab, sizeof(BasisTab), hipMemcpyHostToDevice) != hipSuccess)
+      rc = set_err(h, MVF_E_DEVICE, "basis table upload failed");
+    if (!rc) rc = lane_tables(h, tab);
     delete tab;
+    if (rc) return rc;
   }
   double t1 = now();
   MVF_TRY(setup_numbering(h));

@@ -273,6 +273,285 @@ __global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ ta
       }
 }
 
+template <typename T>
+static int lane_table(mvf_handle *h, const BasisTab *tab, LaneTab<T> &lt) {
+  const int NN = h->et.nn, NE = h->et.ne, NG = h->et.ng;
+  const int NT = (NE + 15) / 16, NU = (NG + 3) / 4;
+  const size_t n4 = (size_t)NU * NT * 64, ndN = (size_t)3 * NN * 32, nN = (size_t)NN * 32;
+  std::vector<T> v(4 * n4 + ndN + nN + 32, (T)0);
+  std::vector<int> dir((size_t)NT * 16, 0);
+  T *d0 = v.data(), *d1 = d0 + n4, *d2 = d1 + n4, *p = d2 + n4, *dN = p + n4, *N = dN + ndN, *w = N + nN;
+  for (int u = 0; u < NU; u++)
+    for (int t = 0; t < NT; t++)
+      for (int l = 0; l < 64; l++) {
+        const int g = 4 * u + (l >> 4), e = 16 * t + (l & 15);
+        if (g >= NG || e >= NE) continue;
+        const size_t i = ((size_t)u * NT + t) * 64 + l;
+        d0[i] = (T)tab->dphi[g][e][0];
+        d1[i] = (T)tab->dphi[g][e][1];
+        d2[i] = (T)tab->dphi[g][e][2];
+        p[i] = (T)tab->phi[g][e];
+      }
+  for (int t = 0; t < NT; t++)
+    for (int r = 0; r < 16; r++)
+      if (16 * t + r < NE) dir[(size_t)t * 16 + r] = tab->dir[16 * t + r];
+  for (int g = 0; g < NG; g++) {
+    for (int n = 0; n < NN; n++) {
+      for (int m = 0; m < 3; m++) dN[((size_t)m * NN + n) * 32 + g] = (T)tab->dN[g][m][n];
+      N[(size_t)n * 32 + g] = (T)tab->N[g][n];
+    }
+    w[g] = (T)tab->w[g];
+  }
+  char *dev;
+  const size_t bytes = v.size() * sizeof(T), dbytes = dir.size() * sizeof(int);
+  MVF_TRY(dalloc(h, &dev, bytes + dbytes));
+  h->owned.push_back(dev);
+  MVF_HIP(h, hipMemcpy(dev, v.data(), bytes, hipMemcpyHostToDevice));
+  MVF_HIP(h, hipMemcpy(dev + bytes, dir.data(), dbytes, hipMemcpyHostToDevice));
+  const T *b = (const T *)dev;
+  lt.d0 = b;
+  lt.d1 = b + n4;
+  lt.d2 = b + 2 * n4;
+  lt.p = b + 3 * n4;
+  lt.dN = b + 4 * n4;
+  lt.N = lt.dN + ndN;
+  lt.w = lt.N + nN;
+  lt.dir = (const int *)(dev + bytes);
+  return 0;
+}
+
+int lane_tables(mvf_handle *h, const BasisTab *tab) {
+  MVF_TRY(lane_table<double>(h, tab, h->lt64));
+  MVF_TRY(lane_table<float>(h, tab, h->lt32));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_elem_km_mfma: the same K_e and M_e on the matrix cores, one wavefront per
+// element (four elements per 256-thread workgroup).
+//
+// With chat_i(g, a) the modified curl and v_i(g, a) the value vector of DOF i
+// at Gauss point g (component a), the element matrices are the products
+//   K = C^T Q,  C[(g,a)][i] = chat_i(g, a),  Q[(g,a)][j] = (w_g m1_g chat_j(g))_a
+//   M = V^T P,  V[(g,a)][i] = v_i(g, a),     P[(g,a)][j] = (w_g h1h2h3 m2_g v_j(g))_a
+// of depth 3G, computed as 16 x 16 tiles with v_mfma_{f64,f32}_16x16x4.  The
+// depth index is ordered (u, a, q) with g = 4u + q: in an MFMA step the lanes
+// l with l / 16 = q supply depth entry q, so lane l holds, for every step of
+// one u and all three a, the entries of Gauss point 4u + l/16 and DOFs
+// 16 t + l%16; it computes them itself from the point's J^-1 (LDS) and the
+// reference-element tables, and no operand goes through LDS.  Only tiles on
+// or above the diagonal are formed; K first, then M.
+// MFMA lane layouts (16 x 16 x 4): A lane l -> A[l%16][l/16], B lane l ->
+// B[l/16][l%16]; D register v of lane l -> D[row][l%16] with row = l/16 + 4v
+// (f64) or 4 (l/16) + v (f32).
+template <typename T>
+__device__ inline int mfma_drow(int l, int v);
+template <>
+__device__ inline int mfma_drow<double>(int l, int v) {
+  return (l >> 4) + 4 * v;
+}
+template <>
+__device__ inline int mfma_drow<float>(int l, int v) {
+  return 4 * (l >> 4) + v;
+}
+
+typedef double mvf_e4d __attribute__((ext_vector_type(4)));
+typedef float mvf_e4f __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<double> {
+  typedef mvf_e4d type;
+  static __device__ inline type mma(double a, double b, type c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Acc4<float> {
+  typedef mvf_e4f type;
+  static __device__ inline type mma(float a, float b, type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T, int NN, int NE, int NG>
+__global__ __launch_bounds__(256, (NE > 36 ? 1 : 2)) void k_elem_km_mfma(
+    const BasisTab *__restrict__ tab, LaneTab<T> lt, const double *__restrict__ xp, const double *__restrict__ yp,
+    const double *__restrict__ zp, const double *__restrict__ inmu, const double *__restrict__ sig, Geom g, Gpml gp,
+    int prev, int first_flags, int64_t e0, int64_t n, T *__restrict__ Kout, T *__restrict__ Mout, int *bad) {
+  constexpr int WPB = 4;                 // elements (waves) per workgroup
+  constexpr int NT = (NE + 15) / 16;     // DOF tiles
+  constexpr int NU = (NG + 3) / 4;       // Gauss-point groups of four
+  constexpr int NPAIR = NT * (NT + 1) / 2;
+  constexpr int NP = NE * (NE + 1) / 2;
+  typedef typename Acc4<T>::type A4;
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+  const int64_t le = (int64_t)blockIdx.x * WPB + wv;  // element index within the launch
+  const bool live = le < n;
+  const int64_t e = e0 + (live ? le : 0);
+  __shared__ double xs64[WPB][NN][3];
+  __shared__ T xsT[WPB][NN][3];
+  __shared__ T nm[WPB][NN][12];
+  // per point: J^-1 (9), GPML curl weights (6), w m1 (6), w h1h2h3 m2 (6)
+  __shared__ T gpd[WPB][NG][27];
+  int i, j, k;
+  elem_index(g, e, i, j, k);
+  if (l < NN) {
+    int64_t id;
+    int ix, jy;
+    load_node(tab, g, i, j, k, l, id, ix, jy);
+    xs64[wv][l][0] = xp[ix];
+    xs64[wv][l][1] = yp[jy];
+    xs64[wv][l][2] = zp[id];
+    xsT[wv][l][0] = (T)xs64[wv][l][0];
+    xsT[wv][l][1] = (T)xs64[wv][l][1];
+    xsT[wv][l][2] = (T)xs64[wv][l][2];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      nm[wv][l][c] = (T)inmu[id * 6 + c];
+      nm[wv][l][6 + c] = (T)sig[2 * (id * 6 + c)];
+    }
+  }
+  __syncthreads();
+  if (l < NG) {
+    // Jacobian from the point-fastest derivative table (coalesced across lanes)
+    T J[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) J[m][c] = 0;
+#pragma unroll 4
+    for (int m = 0; m < NN; m++) {
+      const T x0 = xsT[wv][m][0], x1 = xsT[wv][m][1], x2 = xsT[wv][m][2];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const T dn = lt.dN[(a * NN + m) * 32 + l];
+        J[a][0] += dn * x0;
+        J[a][1] += dn * x1;
+        J[a][2] += dn * x2;
+      }
+    }
+    T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) + J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    const T ad = det < 0 ? -det : det;
+    T Ji[9];
+    Ji[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / ad;
+    Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / ad;
+    Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / ad;
+    Ji[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / ad;
+    Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / ad;
+    Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / ad;
+    Ji[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / ad;
+    Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / ad;
+    Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / ad;
+    if (live && det == (T)0) atomicExch(bad, 1);
+    T wgt = det * lt.w[l];
+    T m1[6], m2[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) m1[c] = m2[c] = 0;
+#pragma unroll 4
+    for (int m = 0; m < NN; m++) {
+      const T Nm = lt.N[m * 32 + l];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        m1[c] += Nm * nm[wv][m][c];
+        m2[c] += Nm * nm[wv][m][6 + c];
+      }
+    }
+    double h[3] = {1.0, 1.0, 1.0};
+    if (gp.on) {
+      int fl = elem_flags(g, gp, prev, first_flags, e);
+      double rr[3] = {0, 0, 0};
+      for (int m = 0; m < NN; m++)
+        for (int c = 0; c < 3; c++) rr[c] += tab->N[l][m] * xs64[wv][m][c];
+      for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, rr[c]);
+    }
+    double h3 = h[0] * h[1] * h[2];
+    double q0 = sqrt(h[0] * h[2] / h[1]), q1 = sqrt(h[0] * h[1] / h[2]), q2 = sqrt(h[1] * h[2] / h[0]);
+    T *o = gpd[wv][l];
+#pragma unroll
+    for (int c = 0; c < 9; c++) o[c] = Ji[c];
+    o[9] = (T)q0; o[10] = (T)q1; o[11] = (T)q1; o[12] = (T)q2; o[13] = (T)q2; o[14] = (T)q0;
+    const T wh = (T)((double)wgt * h3);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      o[15 + c] = wgt * m1[c];
+      o[21 + c] = wh * m2[c];
+    }
+  }
+  __syncthreads();
+  const int64_t base = le * (int64_t)NP;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {  // 0: K, 1: M
+    A4 acc[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
+#pragma unroll 1
+    for (int u = 0; u < NU; u++) {
+      const int gg = 4 * u + q;
+      const bool gv = gg < NG;
+      const T *pd = gpd[wv][gv ? gg : 0];
+      T Ji[9], mw[6];
+#pragma unroll
+      for (int c = 0; c < 9; c++) Ji[c] = pd[c];
+#pragma unroll
+      for (int c = 0; c < 6; c++) mw[c] = pd[(pass == 0 ? 15 : 21) + c];
+      T av[NT][3], bv[NT][3];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int ed = 16 * t + r;
+        const bool ok = gv && ed < NE;
+        const int li = (u * NT + t) * 64 + l;
+        const int d = lt.dir[t * 16 + r];
+        const T v0 = Ji[0 * 3 + d], v1 = Ji[1 * 3 + d], v2 = Ji[2 * 3 + d];
+        T x0, x1, x2;
+        if (pass == 0) {
+          const T dp0 = lt.d0[li], dp1 = lt.d1[li], dp2 = lt.d2[li];
+          const T dn0 = Ji[0] * dp0 + Ji[1] * dp1 + Ji[2] * dp2;
+          const T dn1 = Ji[3] * dp0 + Ji[4] * dp1 + Ji[5] * dp2;
+          const T dn2 = Ji[6] * dp0 + Ji[7] * dp1 + Ji[8] * dp2;
+          x0 = pd[9] * (dn1 * v2) - pd[10] * (dn2 * v1);
+          x1 = pd[11] * (dn2 * v0) - pd[12] * (dn0 * v2);
+          x2 = pd[13] * (dn0 * v1) - pd[14] * (dn1 * v0);
+        } else {
+          const T ph = lt.p[li];
+          x0 = ph * v0;
+          x1 = ph * v1;
+          x2 = ph * v2;
+        }
+        if (!ok) x0 = x1 = x2 = 0;
+        av[t][0] = x0;
+        av[t][1] = x1;
+        av[t][2] = x2;
+        bv[t][0] = mw[0] * x0 + mw[1] * x1 + mw[2] * x2;
+        bv[t][1] = mw[1] * x0 + mw[3] * x1 + mw[4] * x2;
+        bv[t][2] = mw[2] * x0 + mw[4] * x1 + mw[5] * x2;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        int p = 0;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int t2 = t; t2 < NT; t2++, p++) acc[p] = Acc4<T>::mma(av[t][a], bv[t2][a], acc[p]);
+      }
+    }
+    if (live) {
+      T *out = pass == 0 ? Kout : Mout;
+      int p = 0;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int t2 = t; t2 < NT; t2++, p++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int ii = 16 * t + mfma_drow<T>(l, v), jj = 16 * t2 + r;
+            if (ii <= jj && jj < NE) out[base + ii * NE - ii * (ii - 1) / 2 + (jj - ii)] = acc[p][v];
+          }
+    }
+  }
+}
+
 // per-frequency element RHS: be[e][im][d] complex, d = polarization
 template <int NN, int NE, int NG>
 __global__ __launch_bounds__(64) void k_elem_rhs(const BasisTab *__restrict__ tab, const double *__restrict__ xp,
@@ -384,9 +663,33 @@ __global__ __launch_bounds__(64) void k_elem_rhs(const BasisTab *__restrict__ ta
   }
 }
 
+template <typename T>
+static LaneTab<T> lane_tab(const mvf_handle *h);
+template <>
+LaneTab<double> lane_tab<double>(const mvf_handle *h) {
+  return h->lt64;
+}
+template <>
+LaneTab<float> lane_tab<float>(const mvf_handle *h) {
+  return h->lt32;
+}
+
 template <typename T, int NN, int NE, int NG, int GC>
 static int launch_km(mvf_handle *h, int64_t e0, int64_t n, int first_flags, T *dK, T *dM, int *bad) {
   int prev = (h->emulate & MVF_EMU_GPML_PREV_FLAGS) ? 1 : 0;
+  if (!h->elem_legacy) {
+    // four elements per workgroup; at most 2^22 elements per launch
+    const int64_t chunk = (int64_t)1 << 22;
+    for (int64_t s = 0; s < n; s += chunk) {
+      int64_t c = (n - s) < chunk ? (n - s) : chunk;
+      constexpr int64_t NP = NE * (NE + 1) / 2;
+      hipLaunchKernelGGL((k_elem_km_mfma<T, NN, NE, NG>), dim3((unsigned)((c + 3) / 4)), dim3(256), 0, h->stream,
+                         h->d_tab, lane_tab<T>(h), h->d_xp, h->d_yp, h->d_zp, h->d_inmu, h->d_sig, h->geo, h->gp,
+                         prev, first_flags, e0 + s, c, dK + s * NP, dM + s * NP, bad);
+    }
+    MVF_HIP(h, hipGetLastError());
+    return 0;
+  }
   const int64_t chunk = 1 << 20;
   for (int64_t s = 0; s < n; s += chunk) {
     int64_t c = (n - s) < chunk ? (n - s) : chunk;

@@ -70,6 +70,19 @@ struct BdParams {
   double2 cz1, e0;         // inimod 3: surface transfer function, ezl(0, 0)
 };
 
+// Reference-element tables in the lane order of the matrix-core element
+// kernel: for Gauss-point group u, DOF tile t and lane l (point 4u + l/16, DOF
+// 16t + l%16) the DOF's derivative (d0, d1, d2) and value (p) of the edge
+// polynomial, [u][t][l], zero for padding; the DOF's direction [t][l%16]; and
+// the nodal functions with the point fastest, dN [3][node][32], N [node][32],
+// weights w [32].
+template <typename T>
+struct LaneTab {
+  const T *d0, *d1, *d2, *p;
+  const int *dir;
+  const T *dN, *N, *w;
+};
+
 }  // namespace mvf
 
 struct mvf_handle {
@@ -108,6 +121,9 @@ struct mvf_handle {
   double *d_K = nullptr, *d_M = nullptr;
   float *d_K32 = nullptr, *d_M32 = nullptr;
   int ops_fp32 = 0;  // d_K/d_M were last assembled from FP32 element integration
+  int elem_legacy = 0;  // element matrices from the one-workgroup-per-element VALU kernel (comparison)
+  mvf::LaneTab<double> lt64{};
+  mvf::LaneTab<float> lt32{};
   double *d_Ke = nullptr, *d_Me = nullptr;  // element matrices (upper packed), kept for mvf_integrate
   int64_t ke_elems = 0;
   // element-0 correction for the carried GPML flags (see MVF_EMU_GPML_PREV_FLAGS)
@@ -241,5 +257,6 @@ int bd_setup(mvf_handle *h, const double *dKe, const double *dMe);
 int bd_values(mvf_handle *h, double omega);
 int bd_lift(mvf_handle *h, double omega, double *d_be);
 int noalloc_setup(mvf_handle *h);
+int lane_tables(mvf_handle *h, const BasisTab *tab);
 
 }  // namespace mvf

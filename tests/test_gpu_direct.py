@@ -54,16 +54,21 @@ def test_direct_matches_superlu(name, tile, need_gpu):
     lu = spla.splu(A)
     xref = np.stack([lu.solve(b[d]) for d in range(2)])
     print(f"direct {name}: nne={g.nne} solving", flush=True)
-    x, info = g.solve(b.ravel(), method="gmres", precond="direct", tol=1e-8, direct_tile=tile)
+    # 1e-9: the solutions are compared to 1e-6, and at 1 mHz under GPML the
+    # condition number is ~1e3, so a 1e-8 residual leaves that bar no margin
+    x, info = g.solve(b.ravel(), method="gmres", precond="direct", tol=1e-9, direct_tile=tile)
     x = x.reshape(2, -1)
     print(f"direct {name}: its={info['iterations']} applies={info['precond_applies']} "
           f"pert={info['static_pivots']} factor {info['factor_seconds']:.3f}s", flush=True)
     assert info["converged"], info
-    mask = determined_dofs(m, g.gne())
+    # GPML models: outside the stretched zones, whose modes a residual of 1e-8
+    # determines only weakly (as in test_direct_frequency_sweep)
+    mask = determined_dofs(m, g.gne()) & inner_dofs(m, g.gne())
     for d in range(2):
         r = np.linalg.norm(b[d] - A @ x[d]) / np.linalg.norm(b[d])
         assert r <= 1e-8 * 1.0001, (d, r)
         e = np.linalg.norm(x[d][mask] - xref[d][mask]) / np.linalg.norm(xref[d][mask])
+        print(f"direct {name}: pol {d} residual {r:.2e}, solution difference {e:.2e}", flush=True)
         assert e <= 1e-6, (d, e)
 
 
