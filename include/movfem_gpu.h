@@ -158,7 +158,8 @@ typedef struct mvf_solver_opts {
   int32_t maxit;        /* total Krylov iterations */
   int32_t check_every;  /* iterations between convergence checks (host sync) */
   double inner_tol;     /* MIXED: FP32 residual drop before an FP64 refinement step */
-  int32_t use_graph;    /* capture the iteration block in a HIP graph */
+  int32_t use_graph;    /* COCG/BiCGStab: capture check_every iterations once as a HIP graph and replay it
+                           between convergence checks (ignored by mvf_solve_noalloc and the direct path) */
   int32_t gmres_m;      /* GMRES restart length (MVF_PC_DIRECT), 0 = 30 */
   int32_t freqs_per_factor; /* MVF_PC_DIRECT in mvf_solve_frequencies: consecutive frequencies sharing one
                                factorization (at their geometric-mean frequency), 0 or 1 = one each */
@@ -179,7 +180,7 @@ typedef struct mvf_solve_info {
   double spmv_ms[2];       /* summed kernel time */
   double spmv_bytes[2];    /* algorithmic bytes of one launch: nnz*(K+M+col) + nne*(rowptr + ncol*(x+y)) */
   int32_t spmv_ncol;       /* columns per launch (last batch) */
-  int32_t pad_;
+  int32_t spmv_lpr;        /* lanes per matrix row of the FP64 SpMV, chosen by the autotuner */
   /* MVF_PC_DIRECT */
   double factor_seconds;   /* numeric factorizations performed by this call, wall time */
   double factor_flops;     /* algorithmic real flops of one factorization */
@@ -194,6 +195,7 @@ typedef struct mvf_solve_info {
   double gemm_ms;
   double gemm_flops;
   int64_t gemm_launches;
+  int64_t graph_launches;  /* use_graph: replays of the captured block of check_every Krylov iterations */
 } mvf_solve_info;
 
 void mvf_default_solver_opts(mvf_solver_opts *o);
@@ -222,6 +224,16 @@ int mvf_solve_noalloc(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o
  * mvf_get_solution.  1 <= nfreq <= 32 (true_resid holds 64 columns). */
 int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, const mvf_solver_opts *o,
                           mvf_solve_info *info);
+/* Many models (handles, of any size and element type) x nfreq frequencies
+ * each: mvf_solve_frequencies on every handle, nthreads host workers (0: 8)
+ * keeping that many models' work in flight at once, each on its handle's own
+ * stream, so small models' kernels overlap on the device.  freqs_hz is
+ * (nh, nfreq) row-major; infos and status (each nh entries, may be NULL)
+ * receive each model's info and return code; the result is the first
+ * non-zero code.  No counterpart in the reference, which runs one model
+ * through its frequency loop (MoVFEM_3DMT.f90:62-143). */
+int mvf_solve_batch(mvf_handle *const *hs, int32_t nh, const double *freqs_hz, int32_t nfreq,
+                    const mvf_solver_opts *o, mvf_solve_info *infos, int32_t *status, int32_t nthreads);
 /* column-major (nne, 2*nfreq) complex solution of the last solve_frequencies */
 int mvf_get_solution(mvf_handle *h, double *x_out, int32_t ncols);
 /* device RHS (nne, 2*nfreq) of the last solve_frequencies, for checks */
@@ -239,6 +251,10 @@ int mvf_setup_times(const mvf_handle *h, double *t4);
  * No counterpart in the reference (whose matrix exists only as the COO
  * global_vfem returns): for host-side residual checks. */
 int mvf_get_operators(mvf_handle *h, int64_t *rowptr, int32_t *col, double *K, double *M);
+/* Rows [r0, r1) of the same: rowptr[r1 - r0 + 1] relative to the first row's
+ * start, then that many col, K, M entries (any of col, K, M may be NULL:
+ * a call with all three NULL returns the row lengths to size the buffers). */
+int mvf_get_rows(mvf_handle *h, int64_t r0, int64_t r1, int64_t *rowptr, int32_t *col, double *K, double *M);
 
 /* Re-integrates every element's K and M from the resident lattice and model
  * and re-assembles them in place (the element loop of global_vfem,
@@ -410,6 +426,59 @@ int mvf_mesh_sizes(const mvf_mesh *m, int64_t *dims, double *vals);
 int mvf_mesh_arrays(const mvf_mesh *m, double *xp, double *yp, double *zp, double *sigma, double *mu, int32_t *nzl,
                     double *xsr, double *ysr, double *zsr);
 void mvf_mesh_free(mvf_mesh *m);
+
+/* ---- learned initial guess (no counterpart in the reference) --------------
+ * A 3-D U-Net (four levels, 16/32/64/96 channels, 3x3x3 convolutions with
+ * leaky ReLU, 2x2x2 average pooling and nearest upsampling, skip
+ * connections, a 1x1x1 head; 1.16M parameters) trained and evaluated on the
+ * BF16 matrix cores with FP32 accumulation and FP32 master weights (Adam).
+ * Inputs and targets are channels-last FP32 host arrays (batch, nx, ny, nz, c).
+ * It predicts the solution of a frequency-model system from per-cell
+ * features; mvf_solve_guess starts the Krylov solve from that prediction,
+ * the FP64 true residual judging the result, and plain zero-start Krylov is
+ * what remains when the guess does not help (the least-squares scaling never
+ * leaves a larger initial residual than x0 = 0). */
+typedef struct mvf_nn mvf_nn;
+typedef struct mvf_nn_config {
+  int32_t nx, ny, nz;   /* voxel grid, each a multiple of 8                      */
+  int32_t cin, cout;    /* real input / output channels, 1..16                    */
+  int32_t width;        /* channels of the finest level (16)                      */
+  int32_t batch;        /* samples per forward/backward                          */
+  int32_t device;
+  uint64_t seed;        /* weight initialisation                                  */
+  float lr, beta1, beta2, eps; /* Adam                                           */
+  float leaky;          /* negative slope of the activation                       */
+  int32_t pad_;
+} mvf_nn_config;
+void mvf_nn_default_config(mvf_nn_config *c);
+int mvf_nn_create(const mvf_nn_config *cfg, mvf_nn **out);
+void mvf_nn_free(mvf_nn *n);
+const char *mvf_nn_last_error(const mvf_nn *n);
+/* parameters and gradients: per layer (encoder levels 0..3, convolutions 1
+ * and 2; decoder levels 0..2, convolutions 1 and 2; head) the weights
+ * [tap][cout][cin] (tap = 9 dx + 3 dy + dz over offsets -1..1) then the bias
+ * [cout]; the first layer's cin and the head's cout are padded to 16 */
+int64_t mvf_nn_num_params(const mvf_nn *n);
+int mvf_nn_get_params(mvf_nn *n, float *out);
+int mvf_nn_set_params(mvf_nn *n, const float *in);
+int mvf_nn_get_grads(mvf_nn *n, float *out);
+/* loss = sum m (y - t)^2 / sum m over the batch (m = 1 when NULL) and the
+ * gradients of every parameter; nb must equal cfg.batch */
+int mvf_nn_forward_backward(mvf_nn *n, int32_t nb, const float *x, const float *t, const float *m, float *loss);
+int mvf_nn_adam(mvf_nn *n, float lr);
+int mvf_nn_predict(mvf_nn *n, int32_t nb, const float *x, float *y);
+/* HIP-event timing of the matrix-core kernels of the next calls: s3 = summed
+ * kernel ms, algorithmic flops (2 per multiply-add) and launches of the last
+ * forward(-backward) */
+int mvf_nn_set_timing(mvf_nn *n, int32_t on);
+int mvf_nn_stats(const mvf_nn *n, double *s3);
+
+/* mvf_solve starting from x0 (host, (nne, 2) complex like rhs_inout) instead
+ * of zero: each column of x0 is first scaled by the complex factor alpha that
+ * minimises ||b - alpha A x0|| (returned in alpha_out[4] as re, im per
+ * polarization; NULL allowed), so the initial residual is never above ||b||. */
+int mvf_solve_guess(mvf_handle *h, const double *x0, double *rhs_inout, const mvf_solver_opts *o,
+                    mvf_solve_info *info, double *alpha_out);
 
 /* Host-only symbolic analysis of the sparse direct factorization for a
  * numbering (gne, ne x me, 1-based, <= 0 removed) and the full CSR pattern

@@ -34,7 +34,9 @@ EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne",
            "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface",
            "mvf_read_inputs", "mvf_inputs_view", "mvf_inputs_free", "mvf_grid3d", "mvf_mesh_sizes",
            "mvf_mesh_arrays", "mvf_mesh_free", "mvf_problem_fields", "mvf_write_problem", "mvf_get_operators",
-           "mvf_solve_noalloc")
+           "mvf_solve_noalloc", "mvf_solve_guess", "mvf_solve_batch", "mvf_get_rows", "mvf_nn_default_config", "mvf_nn_create", "mvf_nn_free",
+           "mvf_nn_last_error", "mvf_nn_num_params", "mvf_nn_get_params", "mvf_nn_set_params", "mvf_nn_get_grads",
+           "mvf_nn_forward_backward", "mvf_nn_adam", "mvf_nn_predict", "mvf_nn_set_timing", "mvf_nn_stats")
 
 
 class Grid(C.Structure):
@@ -68,11 +70,18 @@ class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("converged", C.c_int32),
                 ("fallbacks", C.c_int32), ("true_resid", C.c_double * 64), ("seconds", C.c_double),
                 ("spmv_launches", C.c_int64 * 2), ("spmv_ms", C.c_double * 2), ("spmv_bytes", C.c_double * 2),
-                ("spmv_ncol", C.c_int32), ("pad_", C.c_int32),
+                ("spmv_ncol", C.c_int32), ("spmv_lpr", C.c_int32),
                 ("factor_seconds", C.c_double), ("factor_flops", C.c_double), ("factor_bytes", C.c_double),
                 ("factorizations", C.c_int32), ("static_pivots", C.c_int32), ("precond_applies", C.c_int32),
                 ("operator_fp32", C.c_int32), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
-                ("gemm_launches", C.c_int64)]
+                ("gemm_launches", C.c_int64), ("graph_launches", C.c_int64)]
+
+
+class NnConfig(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("cin", C.c_int32), ("cout", C.c_int32),
+                ("width", C.c_int32), ("batch", C.c_int32), ("device", C.c_int32), ("seed", C.c_uint64),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("leaky", C.c_float), ("pad_", C.c_int32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -122,11 +131,32 @@ def lib():
     L.mvf_default_solver_opts.argtypes = [C.POINTER(SolverOpts)]
     L.mvf_solve.argtypes = [vp, dp, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
     L.mvf_solve_noalloc.argtypes = [vp, dp, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
+    L.mvf_solve_batch.argtypes = [C.POINTER(vp), C.c_int32, dp, C.c_int32, C.POINTER(SolverOpts),
+                                  C.POINTER(SolveInfo), ip, C.c_int32]
+    L.mvf_solve_guess.argtypes = [vp, dp, dp, C.POINTER(SolverOpts), C.POINTER(SolveInfo), dp]
+    fp = C.POINTER(C.c_float)
+    L.mvf_nn_default_config.restype = None
+    L.mvf_nn_default_config.argtypes = [C.POINTER(NnConfig)]
+    L.mvf_nn_create.argtypes = [C.POINTER(NnConfig), C.POINTER(vp)]
+    L.mvf_nn_free.restype = None
+    L.mvf_nn_free.argtypes = [vp]
+    L.mvf_nn_last_error.restype = C.c_char_p
+    L.mvf_nn_last_error.argtypes = [vp]
+    L.mvf_nn_num_params.restype = C.c_int64
+    L.mvf_nn_num_params.argtypes = [vp]
+    for fn in (L.mvf_nn_get_params, L.mvf_nn_set_params, L.mvf_nn_get_grads):
+        fn.argtypes = [vp, fp]
+    L.mvf_nn_forward_backward.argtypes = [vp, C.c_int32, fp, fp, fp, fp]
+    L.mvf_nn_adam.argtypes = [vp, C.c_float]
+    L.mvf_nn_predict.argtypes = [vp, C.c_int32, fp, fp]
+    L.mvf_nn_set_timing.argtypes = [vp, C.c_int32]
+    L.mvf_nn_stats.argtypes = [vp, dp]
     L.mvf_solve_frequencies.argtypes = [vp, dp, C.c_int32, C.POINTER(SolverOpts), C.POINTER(SolveInfo)]
     L.mvf_get_solution.argtypes = [vp, dp, C.c_int32]
     L.mvf_get_rhs.argtypes = [vp, dp, C.c_int32]
     L.mvf_setup_times.argtypes = [vp, dp]
     L.mvf_get_operators.argtypes = [vp, C.POINTER(C.c_int64), ip, dp, dp]
+    L.mvf_get_rows.argtypes = [vp, C.c_int64, C.c_int64, i64p, ip, dp, dp]
     L.mvf_assemble.argtypes = [vp, C.c_int32, dp]
     L.mvf_direct_plan_stats.argtypes = [C.c_int32] * 4 + [ip, C.c_int64, i64p, ip, dp]
     L.mvf_readout.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_int32, dp, dp] + [dp] * 11

@@ -4,13 +4,17 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <new>
+#include <thread>
 
 #include "mvf_internal.h"
 
 namespace mvf {
 int solve_device(mvf_handle *h, const double *omegas, int ncol, int ncol_pad, const mvf_solver_opts *o,
                  const double *b, double *x, mvf_solve_info *info);
+int solve_device_guess(mvf_handle *h, const double *omegas, const mvf_solver_opts *o, const double *b, double *x,
+                       mvf_solve_info *info, double *alpha_out);
 
 static double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -370,6 +374,19 @@ int mvf_get_operators(mvf_handle *h, int64_t *rowptr, int32_t *col, double *K, d
   return 0;
 }
 
+int mvf_get_rows(mvf_handle *h, int64_t r0, int64_t r1, int64_t *rowptr, int32_t *col, double *K, double *M) {
+  if (!h || r0 < 0 || r1 > h->nne || r1 < r0 || !rowptr) return h ? set_err(h, MVF_E_ARG, "bad row range") : MVF_E_ARG;
+  MVF_HIP(h, hipSetDevice(h->device));
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  MVF_HIP(h, hipMemcpy(rowptr, h->d_rowptr + r0, 8 * (size_t)(r1 - r0 + 1), hipMemcpyDeviceToHost));
+  const int64_t p0 = rowptr[0], n = rowptr[r1 - r0] - p0;
+  for (int64_t i = 0; i <= r1 - r0; i++) rowptr[i] -= p0;
+  if (col) MVF_HIP(h, hipMemcpy(col, h->d_col + p0, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if (K) MVF_HIP(h, hipMemcpy(K, h->d_K + p0, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  if (M) MVF_HIP(h, hipMemcpy(M, h->d_M + p0, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int mvf_get_gne(mvf_handle *h, int32_t *gne_out) {
   if (!h || !gne_out) return MVF_E_ARG;
   MVF_HIP(h, hipSetDevice(h->device));
@@ -544,6 +561,46 @@ int mvf_solve(mvf_handle *h, double *rhs_inout, const mvf_solver_opts *o, mvf_so
   return rc;
 }
 
+int mvf_solve_guess(mvf_handle *h, const double *x0, double *rhs_inout, const mvf_solver_opts *o,
+                    mvf_solve_info *info, double *alpha_out) {
+  DevScope ds;
+  if (!h || !rhs_inout || !x0) return MVF_E_ARG;
+  if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede mvf_solve_guess");
+  mvf_solver_opts od;
+  mvf_default_solver_opts(&od);
+  if (!o) o = &od;
+  mvf_solve_info dummy;
+  if (!info) info = &dummy;
+  memset(info, 0, sizeof(*info));
+  MVF_HIP(h, hipSetDevice(h->device));
+  int32_t nne = h->nne;
+  std::vector<double> hb((size_t)nne * 4), hx((size_t)nne * 4);
+  for (int32_t r = 0; r < nne; r++)
+    for (int d = 0; d < 2; d++)
+      for (int p = 0; p < 2; p++) {
+        hb[4 * (size_t)r + 2 * d + p] = rhs_inout[2 * ((size_t)d * nne + r) + p];
+        hx[4 * (size_t)r + 2 * d + p] = x0[2 * ((size_t)d * nne + r) + p];
+      }
+  double *db, *dx;
+  MVF_TRY(ds.alloc(h, &db, (size_t)nne * 4));
+  MVF_TRY(ds.alloc(h, &dx, (size_t)nne * 4));
+  MVF_HIP(h, hipMemcpyAsync(db, hb.data(), 32 * (size_t)nne, hipMemcpyHostToDevice, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(dx, hx.data(), 32 * (size_t)nne, hipMemcpyHostToDevice, h->stream));
+  double om[2] = {h->omega, h->omega};
+  double t0 = now();
+  int rc = solve_device_guess(h, om, o, db, dx, info, alpha_out);
+  (void)hipStreamSynchronize(h->stream);
+  info->seconds = now() - t0;
+  info->operator_fp32 = h->ops_fp32;
+  if (hipMemcpy(hb.data(), dx, 32 * (size_t)nne, hipMemcpyDeviceToHost) != hipSuccess && !rc) rc = MVF_E_DEVICE;
+  for (int32_t r = 0; r < nne; r++)
+    for (int d = 0; d < 2; d++) {
+      rhs_inout[2 * ((size_t)d * nne + r)] = hb[4 * (size_t)r + 2 * d];
+      rhs_inout[2 * ((size_t)d * nne + r) + 1] = hb[4 * (size_t)r + 2 * d + 1];
+    }
+  return rc;
+}
+
 int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, const mvf_solver_opts *o,
                           mvf_solve_info *info) {
   DevScope ds;
@@ -607,6 +664,7 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
       info->gemm_launches = bi.gemm_launches;
     }
     info->iterations += bi.iterations;
+    info->graph_launches += bi.graph_launches;
     info->refinements += bi.refinements;
     info->fallbacks += bi.fallbacks;
     // SpMV timing is reported for the widest batch (bytes are per launch and
@@ -618,6 +676,7 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
         info->spmv_bytes[p] = bi.spmv_bytes[p];
       }
       info->spmv_ncol = bi.spmv_ncol;
+      info->spmv_lpr = bi.spmv_lpr;
     }
     for (int c = 0; c < 2 * nb && 2 * f + c < 64; c++) info->true_resid[2 * f + c] = bi.true_resid[c];
     MVF_HIP(h, hipMemcpy2DAsync(h->d_x + 2 * (size_t)(2 * f), 16 * (size_t)ncol, xb, 16 * (size_t)pc, 16 * (size_t)(2 * nb),
@@ -633,6 +692,33 @@ int mvf_solve_frequencies(mvf_handle *h, const double *freqs_hz, int32_t nfreq, 
   info->converged = rc == 0;
   info->seconds = now() - t0;
   info->operator_fp32 = h->ops_fp32;
+  return rc;
+}
+
+int mvf_solve_batch(mvf_handle *const *hs, int32_t nh, const double *freqs_hz, int32_t nfreq,
+                    const mvf_solver_opts *o, mvf_solve_info *infos, int32_t *status, int32_t nthreads) {
+  if (!hs || nh < 1 || !freqs_hz || nfreq < 1 || nfreq > 32) return MVF_E_ARG;
+  for (int32_t i = 0; i < nh; i++)
+    if (!hs[i]) return MVF_E_ARG;
+  const int nt = std::max(1, std::min<int>(nthreads > 0 ? nthreads : 8, nh));
+  std::atomic<int> next{0};
+  std::vector<int> rcs(nh, 0);
+  auto worker = [&]() {
+    for (;;) {
+      const int i = next.fetch_add(1);
+      if (i >= nh) break;
+      rcs[i] = mvf_solve_frequencies(hs[i], freqs_hz + (size_t)i * nfreq, nfreq, o, infos ? &infos[i] : nullptr);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
+  int rc = 0;
+  for (int32_t i = 0; i < nh; i++) {
+    if (status) status[i] = rcs[i];
+    if (!rc && rcs[i]) rc = rcs[i];
+  }
   return rc;
 }
 

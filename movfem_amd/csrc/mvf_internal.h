@@ -123,6 +123,9 @@ struct mvf_handle {
   int ops_fp32 = 0;  // d_K/d_M were last assembled from FP32 element integration
   int elem_legacy = 0;  // element matrices from the one-workgroup-per-element VALU kernel (comparison)
   mvf::LaneTab<double> lt64{};
+  int autotune = 1;                  // SpMV lanes per row timed on first use (mvf_solver.hip)
+  int setup_rows = 0;                // pattern and assembly built row by row (large models, mvf_setup.hip)
+  int spmv_lpr[4] = {0, 0, 0, 0};    // tuned lanes per row for 2, 4, 8, 16 columns
   mvf::LaneTab<float> lt32{};
   double *d_Ke = nullptr, *d_Me = nullptr;  // element matrices (upper packed), kept for mvf_integrate
   int64_t ke_elems = 0;

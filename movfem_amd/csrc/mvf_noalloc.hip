@@ -86,7 +86,8 @@ extern "C" int mvf_solve_noalloc(mvf_handle *h, double *rhs_inout, const mvf_sol
   const int64_t n = h->nne;
   // staging: host-layout RHS/solution, then the interleaved b and x
   const size_t vec = (((size_t)n * 2 * sizeof(double2)) + 255) & ~(size_t)255;
-  Solver S{h, 2, {}, o->matrix == MVF_MATRIX_C64REF, 16, n, nullptr};
+  // the lanes per row the handle's autotuner chose for two columns, if it ran
+  Solver S{h, 2, {}, o->matrix == MVF_MATRIX_C64REF, h->spmv_lpr[0] ? h->spmv_lpr[0] : 16, n, nullptr};
   S.w.w[0] = S.w.w[1] = (double)(float)h->omega;
   Workspace W;
   W.sizing = true;
@@ -123,5 +124,6 @@ extern "C" int mvf_solve_noalloc(mvf_handle *h, double *rhs_inout, const mvf_sol
   }
   info->seconds = now() - t0;
   info->operator_fp32 = h->ops_fp32;
+  info->spmv_lpr = S.lpr;
   return rc;
 }

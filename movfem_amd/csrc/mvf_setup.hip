@@ -14,6 +14,13 @@
 // (row, col) keys carrying (element, im, jm) payloads; the sorted payloads are
 // the per-entry gather lists, already in element order, so assembly sums every
 // entry in the same order as the reference's element loop.
+//
+// Large models (the pair sort would need more than a quarter of free HBM, e.g.
+// 4.59e9 pairs, ~147 GB of keys, for 27-node 128 x 128 x 96) build the pattern
+// row by row instead: one wavefront per row gathers the DOFs of the (at most
+// eight) elements holding the row's DOF, sorts them in LDS and keeps the
+// distinct ones; assembly then sums each row's entries element by element in
+// LDS.  Both paths give the same pattern and the same sums in the same order.
 #include <hipcub/hipcub.hpp>
 
 #include "mvf_internal.h"
@@ -178,7 +185,7 @@ __global__ void k_pairs(const int32_t *gne, int me, int64_t nne, const int64_t *
   }
   __syncthreads();
   int v = nv;
-  for (int t = threadIdx.x; t < me * me; t += blockDim.x) {
+  for (int t = threadIdx.x; keys && t < me * me; t += blockDim.x) {
     int a = t / me, b = t % me;
     if (sg[a] < 0 || sg[b] < 0) continue;
     int64_t p = off[e] + (int64_t)pos[a] * v + pos[b];
@@ -246,6 +253,8 @@ __global__ void k_diagpos(const int64_t *rowptr, const int32_t *col, int64_t nne
   diag[r] = pos;
 }
 
+static int setup_pattern_rows(mvf_handle *h);
+
 static int end_bit_for(uint64_t maxkey) {
   int b = 1;
   while (b < 64 && (maxkey >> b)) b++;
@@ -278,6 +287,40 @@ int setup_pattern(mvf_handle *h) {
   ds.drop(tmp);
   int64_t np = lo + lc, nr = rlo + rlc;
   h->npairs = np;
+  {
+    // the pair sort holds two key arrays, the payloads and the gather lists
+    // (32 bytes per pair) at once: by rows when that exceeds a quarter of the
+    // free memory, or when MVF_SETUP says so
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    const char *sv = getenv("MVF_SETUP");
+    h->setup_rows = (sv && strcmp(sv, "rows") == 0) ? 1
+                    : (sv && strcmp(sv, "pairs") == 0) ? 0
+                                                       : ((double)np * 32.0 > 0.25 * (double)fr ? 1 : 0);
+  }
+  if (h->setup_rows) {
+    uint64_t *rk1, *rk2;
+    int64_t *rv1;
+    MVF_TRY(ds.alloc(h, &rk1, nr));
+    MVF_TRY(ds.alloc(h, &rk2, nr));
+    MVF_TRY(ds.alloc(h, &rv1, nr));
+    MVF_TRY(dalloc(h, &h->d_rcontrib, nr));
+    MVF_TRY(dalloc(h, &h->d_rseg, nne + 1));
+    hipLaunchKernelGGL(k_pairs, dim3((unsigned)g.ne), dim3(256), 0, h->stream, h->d_gne, me, nne, off, roff, nullptr,
+                       nullptr, rk1, rv1);
+    MVF_HIP(h, hipGetLastError());
+    ds.drop(cnt); ds.drop(off); ds.drop(rcnt); ds.drop(roff);
+    int reb = end_bit_for((uint64_t)nne);
+    MVF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, rk1, rk2, rv1, h->d_rcontrib, (int64_t)nr, 0, reb,
+                                                   h->stream));
+    MVF_TRY(ds.alloc(h, &tmp, tb));
+    MVF_HIP(h, hipcub::DeviceRadixSort::SortPairs(tmp, tb, rk1, rk2, rv1, h->d_rcontrib, (int64_t)nr, 0, reb,
+                                                   h->stream));
+    hipLaunchKernelGGL(k_rseg, dim3(grid1d(nr, 256)), dim3(256), 0, h->stream, rk2, nr, h->d_rseg, nne);
+    MVF_HIP(h, hipStreamSynchronize(h->stream));
+    ds.drop(tmp); ds.drop(rk1); ds.drop(rk2); ds.drop(rv1);
+    return setup_pattern_rows(h);
+  }
   uint64_t *k1, *k2, *rk1, *rk2;
   int64_t *v1, *rv1;
   MVF_TRY(ds.alloc(h, &k1, np));
@@ -339,6 +382,248 @@ int setup_pattern(mvf_handle *h) {
   return 0;
 }
 
+// ------------------------------------------------------- per-row pattern --
+// A DOF lies in at most four elements (an element edge), so a row couples at
+// most 4 * me candidate columns: NP = 64 (8-node) or 256 (20/27-node).
+constexpr int kRowEl = 4;
+
+// candidates of row r: the DOFs (0-based) of every element that holds r,
+// padded with INT32_MAX to NP and sorted ascending in LDS; all loads of the
+// row are issued before any is used.  Returns the element count or -1.
+template <int NP>
+__device__ inline int row_candidates(const int32_t *gne, int me, const int64_t *rseg, const int64_t *rcontrib,
+                                     int64_t r, int lane, int32_t *v) {
+  const int64_t t0 = rseg[r];
+  const int nel = (int)(rseg[r + 1] - t0);
+  if (nel > kRowEl) return -1;
+  int64_t ea[kRowEl];
+#pragma unroll
+  for (int q = 0; q < kRowEl; q++) ea[q] = q < nel ? rcontrib[t0 + q] : 0;
+#pragma unroll
+  for (int k = 0; k < NP / 64; k++) {
+    const int idx = lane + 64 * k, q = idx / me, b = idx - q * me;
+    int32_t x = INT32_MAX;
+    if (q < nel) {
+      int64_t e = 0;
+#pragma unroll
+      for (int qq = 0; qq < kRowEl; qq++)
+        if (qq == q) e = ea[qq] / me;
+      const int32_t g = gne[e * me + b];
+      if (g > 0) x = g - 1;
+    }
+    v[idx] = x;
+  }
+  __builtin_amdgcn_wave_barrier();
+  // bitonic sort of the NP values, ascending; each step's pairs are disjoint
+  for (int k = 2; k <= NP; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int c = 0; c < NP / 64; c++) {
+        const int i = lane + 64 * c, ij = i ^ j;
+        if (ij > i) {
+          const int32_t a = v[i], b = v[ij];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            v[i] = b;
+            v[ij] = a;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  return nel;
+}
+
+// pass 0: cnt[r] = distinct columns of row r, ntriu += those >= r;
+// pass 1: col[rowptr[r] + rank] for each distinct column in ascending order
+template <int NP, int PASS>
+__global__ __launch_bounds__(256) void k_row_pattern(const int32_t *gne, int me, const int64_t *rseg,
+                                                     const int64_t *rcontrib, int64_t nne, int64_t *cnt,
+                                                     unsigned long long *ntriu, const int64_t *rowptr, int32_t *col,
+                                                     int *bad) {
+  __shared__ int32_t buf[4][NP];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + wv;
+  if (r >= nne) return;
+  int32_t *v = buf[wv];
+  if (row_candidates<NP>(gne, me, rseg, rcontrib, r, lane, v) < 0) {
+    if (lane == 0) atomicExch(bad, 1);
+    return;
+  }
+  const int64_t base = PASS ? rowptr[r] : 0;
+  int count = 0, upper = 0;
+#pragma unroll
+  for (int c0 = 0; c0 < NP; c0 += 64) {
+    const int i = c0 + lane;
+    const int32_t x = v[i];
+    const bool head = x != INT32_MAX && (i == 0 || v[i - 1] != x);
+    const unsigned long long m = __ballot(head);
+    if (PASS) {
+      const int rank = count + (int)__popcll(m & ((1ull << lane) - 1ull));
+      if (head) col[base + rank] = x;
+    } else {
+      upper += (int)__popcll(__ballot(head && x >= r));
+    }
+    count += (int)__popcll(m);
+  }
+  if (!PASS && lane == 0) {
+    cnt[r] = count;
+    atomicAdd(ntriu, (unsigned long long)upper);
+  }
+}
+
+// K and M of row r: every contribution (element q of the row, local column b)
+// is loaded first; then, element by element in element order, each lane adds
+// its contribution to the entry found by binary search in the row's columns
+// (staged in LDS).  One element adds to distinct entries.
+template <typename TE, int NP>
+__global__ __launch_bounds__(256) void k_assemble_rows(const int32_t *gne, int me, const int64_t *rseg,
+                                                       const int64_t *rcontrib, const int64_t *rowptr,
+                                                       const int32_t *col, int64_t nne, const TE *Ke, const TE *Me,
+                                                       double *K, double *M, float *K32, float *M32) {
+  __shared__ int32_t scol[4][NP];
+  __shared__ double acc[4][2][NP];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + wv;
+  if (r >= nne) return;
+  const int64_t p0 = rowptr[r], p1 = rowptr[r + 1];
+  const int n = (int)(p1 - p0);
+  const int64_t t0 = rseg[r];
+  const int nel = (int)(rseg[r + 1] - t0);
+  int32_t *cs = scol[wv];
+  double *ak = acc[wv][0], *am = acc[wv][1];
+  int64_t ea[kRowEl];
+#pragma unroll
+  for (int q = 0; q < kRowEl; q++) ea[q] = q < nel ? rcontrib[t0 + q] : 0;
+#pragma unroll
+  for (int k = 0; k < NP / 64; k++) {
+    const int i = lane + 64 * k;
+    if (i < n) {
+      cs[i] = col[p0 + i];
+      ak[i] = am[i] = 0.0;
+    }
+  }
+  const int64_t npk = (int64_t)me * (me + 1) / 2;
+  int32_t gc[NP / 64];
+  double kv[NP / 64], mv[NP / 64];
+#pragma unroll
+  for (int k = 0; k < NP / 64; k++) {
+    const int idx = lane + 64 * k, q = idx / me, b = idx - q * me;
+    gc[k] = -1;
+    kv[k] = mv[k] = 0.0;
+    if (q < nel) {
+      int64_t c = 0;
+#pragma unroll
+      for (int qq = 0; qq < kRowEl; qq++)
+        if (qq == q) c = ea[qq];
+      const int64_t e = c / me;
+      const int a = (int)(c - e * me);
+      const int32_t g = gne[e * me + b];
+      if (g > 0) {
+        const int i0 = a < b ? a : b, j0 = a < b ? b : a;
+        const int64_t pk = e * npk + (int64_t)i0 * me - (int64_t)i0 * (i0 - 1) / 2 + (j0 - i0);
+        gc[k] = g - 1;
+        kv[k] = (double)Ke[pk];
+        mv[k] = (double)Me[pk];
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int q = 0; q < nel; q++) {
+#pragma unroll
+    for (int k = 0; k < NP / 64; k++) {
+      const int idx = lane + 64 * k;
+      if (idx / me == q && gc[k] >= 0) {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (cs[mid] < gc[k]) lo = mid + 1;
+          else hi = mid;
+        }
+        ak[lo] += kv[k];
+        am[lo] += mv[k];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+#pragma unroll
+  for (int k = 0; k < NP / 64; k++) {
+    const int i = lane + 64 * k;
+    if (i < n) {
+      K[p0 + i] = ak[i];
+      M[p0 + i] = am[i];
+      K32[p0 + i] = (float)ak[i];
+      M32[p0 + i] = (float)am[i];
+    }
+  }
+}
+
+static int np_for(int me) { return me <= 16 ? 64 : 256; }
+
+// pattern by rows (rseg/rcontrib built): rowptr, col, diagpos, nnz, nnz_triu
+static int setup_pattern_rows(mvf_handle *h) {
+  DevScope ds;
+  const int me = h->et.ne;
+  const int64_t nne = h->nne;
+  int64_t *cnt;
+  unsigned long long *ntriu;
+  int *bad;
+  MVF_TRY(ds.alloc(h, &cnt, nne + 1));
+  MVF_TRY(ds.alloc(h, &ntriu, 1));
+  MVF_TRY(ds.alloc(h, &bad, 1));
+  MVF_HIP(h, hipMemsetAsync(ntriu, 0, 8, h->stream));
+  MVF_HIP(h, hipMemsetAsync(bad, 0, 4, h->stream));
+  MVF_HIP(h, hipMemsetAsync(cnt + nne, 0, 8, h->stream));
+  const unsigned grid = (unsigned)((nne + 3) / 4);
+  if (np_for(me) == 64)
+    hipLaunchKernelGGL((k_row_pattern<64, 0>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
+                       h->d_rcontrib, nne, cnt, ntriu, nullptr, nullptr, bad);
+  else
+    hipLaunchKernelGGL((k_row_pattern<256, 0>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
+                       h->d_rcontrib, nne, cnt, ntriu, nullptr, nullptr, bad);
+  MVF_TRY(dalloc(h, &h->d_rowptr, nne + 1));
+  size_t tb = 0;
+  MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, h->d_rowptr, (int64_t)nne + 1, h->stream));
+  char *tmp;
+  MVF_TRY(ds.alloc(h, &tmp, tb));
+  MVF_HIP(h, hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, h->d_rowptr, (int64_t)nne + 1, h->stream));
+  int64_t nnz = 0;
+  int hb = 0;
+  MVF_HIP(h, hipMemcpyAsync(&nnz, h->d_rowptr + nne, 8, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipMemcpyAsync(&h->nnz_triu, ntriu, 8, hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  if (hb) return set_err(h, MVF_E_INDEX, "a DOF lies in more than four elements: not a structured hexahedral grid");
+  h->nnz = nnz;
+  MVF_TRY(dalloc(h, &h->d_col, nnz));
+  MVF_TRY(dalloc(h, &h->d_diagpos, nne));
+  if (np_for(me) == 64)
+    hipLaunchKernelGGL((k_row_pattern<64, 1>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
+                       h->d_rcontrib, nne, nullptr, nullptr, h->d_rowptr, h->d_col, bad);
+  else
+    hipLaunchKernelGGL((k_row_pattern<256, 1>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
+                       h->d_rcontrib, nne, nullptr, nullptr, h->d_rowptr, h->d_col, bad);
+  hipLaunchKernelGGL(k_diagpos, dim3(grid1d(nne, 256)), dim3(256), 0, h->stream, h->d_rowptr, h->d_col, nne,
+                     h->d_diagpos);
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+template <typename TE>
+static void assemble_rows(mvf_handle *h, const TE *Ke, const TE *Me) {
+  const int me = h->et.ne;
+  const unsigned grid = (unsigned)((h->nne + 3) / 4);
+  if (np_for(me) == 64)
+    hipLaunchKernelGGL((k_assemble_rows<TE, 64>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
+                       h->d_rcontrib, h->d_rowptr, h->d_col, (int64_t)h->nne, Ke, Me, h->d_K, h->d_M, h->d_K32,
+                       h->d_M32);
+  else
+    hipLaunchKernelGGL((k_assemble_rows<TE, 256>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
+                       h->d_rcontrib, h->d_rowptr, h->d_col, (int64_t)h->nne, Ke, Me, h->d_K, h->d_M, h->d_K32,
+                       h->d_M32);
+}
+
 // --------------------------------------------------------------- assembly --
 __device__ inline int64_t packed_index(int me, int i, int j) {
   if (i > j) {
@@ -377,8 +662,11 @@ int assemble_operators(mvf_handle *h) {
     MVF_TRY(dalloc(h, &h->d_K32, nnz));
     MVF_TRY(dalloc(h, &h->d_M32, nnz));
   }
-  hipLaunchKernelGGL(k_assemble<double>, dim3(grid1d(nnz, 256)), dim3(256), 0, h->stream, h->d_seg, h->d_contrib,
-                     nnz, h->et.ne, h->d_Ke, h->d_Me, h->d_K, h->d_M, h->d_K32, h->d_M32);
+  if (h->setup_rows)
+    assemble_rows<double>(h, h->d_Ke, h->d_Me);
+  else
+    hipLaunchKernelGGL(k_assemble<double>, dim3(grid1d(nnz, 256)), dim3(256), 0, h->stream, h->d_seg, h->d_contrib,
+                       nnz, h->et.ne, h->d_Ke, h->d_Me, h->d_K, h->d_M, h->d_K32, h->d_M32);
   MVF_HIP(h, hipGetLastError());
   return 0;
 }
@@ -386,8 +674,11 @@ int assemble_operators(mvf_handle *h) {
 // FP32-integrated element matrices into the operators
 int assemble_operators32(mvf_handle *h, const float *Ke, const float *Me) {
   int64_t nnz = h->nnz;
-  hipLaunchKernelGGL(k_assemble<float>, dim3(grid1d(nnz, 256)), dim3(256), 0, h->stream, h->d_seg, h->d_contrib, nnz,
-                     h->et.ne, Ke, Me, h->d_K, h->d_M, h->d_K32, h->d_M32);
+  if (h->setup_rows)
+    assemble_rows<float>(h, Ke, Me);
+  else
+    hipLaunchKernelGGL(k_assemble<float>, dim3(grid1d(nnz, 256)), dim3(256), 0, h->stream, h->d_seg, h->d_contrib,
+                       nnz, h->et.ne, Ke, Me, h->d_K, h->d_M, h->d_K32, h->d_M32);
   MVF_HIP(h, hipGetLastError());
   return 0;
 }

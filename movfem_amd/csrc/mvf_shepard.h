@@ -14,6 +14,17 @@
 //     (getnp2 with nr < 1, toms660.f90:118-121).
 // Cell lists and node numbers are 1-based as in the reference (0 = empty cell,
 // lnext(k) = k ends a list).
+//
+// Provenance: the algorithm and its routine structure (store2, getnp2,
+// givens, rotate, setup2, qshep2, qs2val) are Robert J. Renka's, published as
+// "Algorithm 660: QSHEP2D, Quadratic Shepard method for bivariate
+// interpolation of scattered data", ACM Transactions on Mathematical Software
+// 14 (1988) 149-150, and vendored by the reference as toms660.f90 (its header
+// names that author and reference, toms660.f90:35-45; the file states no
+// licence of its own).  ACM publishes its collected algorithms under the ACM
+// Software License Agreement, which allows use for non-commercial purposes.
+// This file restates the routines in C++ for bit-identical lattices and adds
+// nothing to the method.
 #pragma once
 
 #include <cmath>

@@ -697,6 +697,23 @@ struct SpmvTimer {
   }
 };
 
+// A block of Krylov iterations captured once as a HIP graph and replayed:
+// the kernels of check_every iterations take the same arguments every time
+// (all state lives in device buffers), so one graph launch replaces
+// 5 * check_every kernel launches between two convergence checks.
+struct GraphBlock {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  int64_t launches = 0;
+  GraphBlock() = default;
+  GraphBlock(const GraphBlock &) = delete;
+  GraphBlock &operator=(const GraphBlock &) = delete;
+  ~GraphBlock() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
 struct Solver {
   mvf_handle *h;
   int ncol;
@@ -734,6 +751,11 @@ struct Workspace {
   }
 };
 
+__global__ void k_fill_d(int64_t n, double *p, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // invd = 1 in every column (no preconditioner)
 __global__ void k_unit_diag(int64_t nv, double *invd) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -741,6 +763,28 @@ __global__ void k_unit_diag(int64_t nv, double *invd) {
     invd[2 * i] = 1.0;
     invd[2 * i + 1] = 0.0;
   }
+}
+
+// Replays the captured block of `count` iterations (capturing it on first use);
+// the SpMV timer is off while capturing (its events are not part of the graph)
+template <typename F>
+static int graph_block(Solver &S, GraphBlock &gb, int count, F &&iter) {
+  mvf_handle *h = S.h;
+  if (!gb.x) {
+    bool tm_on = S.tm && S.tm->on;
+    if (S.tm) S.tm->on = false;
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int k = 0; k < count; k++) iter();
+      e = hipStreamEndCapture(h->stream, &gb.g);
+    }
+    if (S.tm) S.tm->on = tm_on;
+    if (e != hipSuccess) return set_err(h, MVF_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    MVF_HIP(h, hipGraphInstantiate(&gb.x, gb.g, nullptr, nullptr, 0));
+  }
+  MVF_HIP(h, hipGraphLaunch(gb.x, h->stream));
+  gb.launches++;
+  return 0;
 }
 
 constexpr int kSpmvBlocks = 4096;
@@ -759,12 +803,22 @@ static void spmv_launch(const Solver &S, const MT *K, const MT *M, const VT *x, 
     tm->kind[tm->used] = sizeof(MT) == 4 ? 1 : 0;
     (void)hipEventRecord(tm->ev[tm->used][0], h->stream);
   }
-  if (S.c64)
-    hipLaunchKernelGGL((k_spmv<VT, MT, NC, 16, true, DOT>), dim3(grid), dim3(kBS), 0, h->stream, S.n, h->d_rowptr,
-                       h->d_col, K, M, S.w, x, y, p, part);
-  else
-    hipLaunchKernelGGL((k_spmv<VT, MT, NC, 16, false, DOT>), dim3(grid), dim3(kBS), 0, h->stream, S.n, h->d_rowptr,
-                       h->d_col, K, M, S.w, x, y, p, part);
+  // lanes per row: the autotuned choice for the FP64 operator, 16 otherwise
+  const int lpr = sizeof(MT) == 8 ? S.lpr : 16;
+#define MVF_SPMV(L_)                                                                                             \
+  do {                                                                                                           \
+    if (S.c64)                                                                                                   \
+      hipLaunchKernelGGL((k_spmv<VT, MT, NC, L_, true, DOT>), dim3(grid), dim3(kBS), 0, h->stream, S.n,          \
+                         h->d_rowptr, h->d_col, K, M, S.w, x, y, p, part);                                       \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_spmv<VT, MT, NC, L_, false, DOT>), dim3(grid), dim3(kBS), 0, h->stream, S.n,         \
+                         h->d_rowptr, h->d_col, K, M, S.w, x, y, p, part);                                       \
+  } while (0)
+  if (sizeof(MT) == 8 && lpr == 4) MVF_SPMV(4);
+  else if (sizeof(MT) == 8 && lpr == 8) MVF_SPMV(8);
+  else if (sizeof(MT) == 8 && lpr == 32) MVF_SPMV(32);
+  else MVF_SPMV(16);
+#undef MVF_SPMV
   if (tm && tm->on) {
     (void)hipEventRecord(tm->ev[tm->used][1], h->stream);
     tm->used++;
@@ -828,6 +882,8 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
   W.cap = S.ws_cap;
   CocgBufs<NC> B;
   MVF_TRY(cocg_bufs<NC>(W, h, n, B));
+  const bool use_graph = o->use_graph && !S.ws && o->check_every > 0;
+  GraphBlock gb;
   double *r = B.r, *p = B.p, *q = B.q, *invd = B.invd, *part = B.part, *dnorm = B.dnorm;
   ColState *st = B.st;
   double *spart = part + (size_t)kRedBlocks * 3 * NC * 2;
@@ -853,15 +909,25 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
     hipLaunchKernelGGL((k_cocg_init<double, NC>), dim3(nb), dim3(kBS), 0, h->stream, n, b, q, invd, r, p, part);
     hipLaunchKernelGGL(k_cocg_fin_init<NC>, dim3(1), dim3(1024), 0, h->stream, part, nb, st);
     int since = 0;
-    while (it < o->maxit) {
+    auto iter = [&]() {
       spmv_launch<double, double, NC, true>(S, h->d_K, h->d_M, p, q, p, spart);
       hipLaunchKernelGGL(k_cocg_alpha<NC>, dim3(1), dim3(1024), 0, h->stream, spart, sb, st);
       hipLaunchKernelGGL((k_cocg_upd1<double, double, NC>), dim3(nb), dim3(kBS), 0, h->stream, n, st, x, r, p, q,
                          invd, part);
       hipLaunchKernelGGL(k_cocg_beta<NC>, dim3(1), dim3(1024), 0, h->stream, part, nb, st, o->tol, 1);
       hipLaunchKernelGGL((k_cocg_upd2<double, NC>), dim3(nb), dim3(kBS), 0, h->stream, n, st, r, p, invd);
-      it++;
-      if (++since >= o->check_every) {
+    };
+    while (it < o->maxit) {
+      if (use_graph && since == 0 && it + o->check_every <= o->maxit) {
+        MVF_TRY(graph_block(S, gb, o->check_every, iter));
+        it += o->check_every;
+        since = o->check_every;
+      } else {
+        iter();
+        it++;
+        since++;
+      }
+      if (since >= o->check_every) {
         since = 0;
         MVF_TRY(host_state<NC>(h, hs, st));
         bool any = false;
@@ -888,6 +954,7 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
   info->iterations = it;
   info->fallbacks = restarts;
   info->converged = rc == 0;
+  info->graph_launches = gb.launches;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
   return rc;
 }
@@ -1030,6 +1097,8 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
   W.cap = S.ws_cap;
   BicgBufs<NC> B;
   MVF_TRY(bicg_bufs<NC>(W, h, n, B));
+  const bool use_graph = o->use_graph && !S.ws && o->check_every > 0;
+  GraphBlock gb;
   double *r = B.r, *rh = B.rh, *p = B.p, *v = B.v, *s = B.s, *t = B.t, *ph = B.ph, *sh = B.sh, *invd = B.invd;
   double *part = B.part, *dnorm = B.dnorm;
   ColState *st = B.st;
@@ -1057,7 +1126,7 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
     hipLaunchKernelGGL(k_bicg_ts<NC>, dim3(nb), dim3(kBS), 0, h->stream, n, rh, r, part);
     hipLaunchKernelGGL(k_bicg_beta<NC>, dim3(1), dim3(1024), 0, h->stream, part, nb, st, o->tol, 0);
     int since = 0;
-    while (it < o->maxit) {
+    auto iter = [&]() {
       hipLaunchKernelGGL(k_bicg_p<NC>, dim3(nb), dim3(kBS), 0, h->stream, n, st, r, p, v, invd, ph);
       spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, ph, v, nullptr, nullptr);
       hipLaunchKernelGGL(k_hdot<NC>, dim3(nb), dim3(kBS), 0, h->stream, n, rh, v, part);
@@ -1068,8 +1137,18 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
       hipLaunchKernelGGL(k_bicg_omega<NC>, dim3(1), dim3(1024), 0, h->stream, part, nb, st);
       hipLaunchKernelGGL(k_bicg_x<NC>, dim3(nb), dim3(kBS), 0, h->stream, n, st, x, r, s, t, ph, sh, rh, part);
       hipLaunchKernelGGL(k_bicg_beta<NC>, dim3(1), dim3(1024), 0, h->stream, part, nb, st, o->tol, 1);
-      it++;
-      if (++since >= o->check_every) {
+    };
+    while (it < o->maxit) {
+      if (use_graph && since == 0 && it + o->check_every <= o->maxit) {
+        MVF_TRY(graph_block(S, gb, o->check_every, iter));
+        it += o->check_every;
+        since = o->check_every;
+      } else {
+        iter();
+        it++;
+        since++;
+      }
+      if (since >= o->check_every) {
         since = 0;
         MVF_TRY(host_state<NC>(h, hs, st));
         bool any = false;
@@ -1093,6 +1172,7 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
   info->iterations = it;
   info->fallbacks = restarts;
   info->converged = rc == 0;
+  info->graph_launches = gb.launches;
   for (int c = 0; c < S.ncol && c < 64; c++) info->true_resid[c] = res[c];
   return rc;
 }
@@ -1100,15 +1180,140 @@ static int run_bicgstab(Solver &S, const mvf_solver_opts *o, const double *b, do
 template <int NC>
 static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info);
 
+// Per column: acc[4c..4c+3] += (Re, Im) of y^H b, |y|^2, |b|^2
+template <int NC>
+__global__ void k_guess_dots(int64_t n, const double2 *__restrict__ y, const double2 *__restrict__ b, double *acc) {
+  __shared__ double red[4 * NC][256];
+  double s[4 * NC];
+#pragma unroll
+  for (int k = 0; k < 4 * NC; k++) s[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const double2 yy = y[i * NC + c], bb = b[i * NC + c];
+      s[4 * c] += yy.x * bb.x + yy.y * bb.y;
+      s[4 * c + 1] += yy.x * bb.y - yy.y * bb.x;
+      s[4 * c + 2] += yy.x * yy.x + yy.y * yy.y;
+      s[4 * c + 3] += bb.x * bb.x + bb.y * bb.y;
+    }
+#pragma unroll
+  for (int k = 0; k < 4 * NC; k++) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+#pragma unroll
+      for (int k = 0; k < 4 * NC; k++) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4 * NC) atomicAdd(&acc[threadIdx.x], red[threadIdx.x][0]);
+}
+
+template <int NC>
+__global__ void k_scale_cols(int64_t n, double2 *x, double ar0, double ai0, double ar1, double ai1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * NC) return;
+  const int c = (int)(i % NC);
+  // columns beyond the first pair keep alpha of their polarization (column % 2)
+  const double ar = (c & 1) ? ar1 : ar0, ai = (c & 1) ? ai1 : ai0;
+  const double2 v = x[i];
+  x[i] = make_double2(ar * v.x - ai * v.y, ar * v.y + ai * v.x);
+}
+
+// x <- alpha x per polarization with alpha = (Ax)^H b / |Ax|^2 on the operator
+// the solve is judged on; writes alpha (re, im per polarization) to alpha_out
+template <int NC>
+static int guess_scale(Solver &S, const double *b, double *x, double *alpha_out) {
+  mvf_handle *h = S.h;
+  const int64_t n = S.n;
+  DevScope ds;
+  double *q, *acc;
+  MVF_TRY(ds.alloc(h, &q, (size_t)n * NC * 2));
+  MVF_TRY(ds.alloc(h, &acc, 4 * NC));
+  spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, x, q, nullptr, nullptr);
+  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, h->stream, (int64_t)(4 * NC), acc, 0.0);
+  hipLaunchKernelGGL(k_guess_dots<NC>, dim3(512), dim3(256), 0, h->stream, n, (const double2 *)q,
+                     (const double2 *)b, acc);
+  double ha[4 * NC];
+  MVF_HIP(h, hipMemcpyAsync(ha, acc, sizeof(ha), hipMemcpyDeviceToHost, h->stream));
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
+  double al[2][2] = {{0, 0}, {0, 0}};
+  for (int c = 0; c < 2 && c < NC; c++) {
+    const double yy = ha[4 * c + 2];
+    if (yy > 0) {
+      al[c][0] = ha[4 * c] / yy;
+      al[c][1] = ha[4 * c + 1] / yy;
+    }
+    if (alpha_out) {
+      alpha_out[2 * c] = al[c][0];
+      alpha_out[2 * c + 1] = al[c][1];
+    }
+  }
+  hipLaunchKernelGGL(k_scale_cols<NC>, dim3(grid1d(n * NC, 256)), dim3(256), 0, h->stream, n, (double2 *)x,
+                     al[0][0], al[0][1], al[1][0], al[1][1]);
+  MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+constexpr int nc_slot(int nc) { return nc == 2 ? 0 : (nc == 4 ? 1 : (nc == 8 ? 2 : 3)); }
+
+// SpMV autotuner: the FP64 operator's lanes per row (4, 8, 16 or 32) timed
+// once per handle and column count with HIP events (one warm-up and three
+// timed launches each), the fastest kept in the handle.  Rows of 8-node
+// systems hold <= 33 entries, 20-node ~84, 27-node ~104: the best width
+// follows the row length and the column count.
+template <int NC>
+static int tune_lpr(Solver &S) {
+  mvf_handle *h = S.h;
+  int &slot = h->spmv_lpr[nc_slot(NC)];
+  if (slot) {
+    S.lpr = slot;
+    return 0;
+  }
+  DevScope ds;
+  double *xa, *ya;
+  const int64_t nv = S.n * NC * 2;
+  MVF_TRY(ds.alloc(h, &xa, nv));
+  MVF_TRY(ds.alloc(h, &ya, nv));
+  hipLaunchKernelGGL(k_fill_d, dim3(grid1d(nv, 256)), dim3(256), 0, h->stream, nv, xa, 1.0);
+  hipEvent_t e0, e1;
+  MVF_HIP(h, hipEventCreate(&e0));
+  MVF_HIP(h, hipEventCreate(&e1));
+  SpmvTimer *tm = S.tm;
+  S.tm = nullptr;
+  int best = 16;
+  float best_ms = 1e30f;
+  for (int L : {4, 8, 16, 32}) {
+    S.lpr = L;
+    spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, xa, ya, nullptr, nullptr);
+    (void)hipEventRecord(e0, h->stream);
+    for (int k = 0; k < 3; k++) spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, xa, ya, nullptr, nullptr);
+    (void)hipEventRecord(e1, h->stream);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = L;
+    }
+  }
+  S.tm = tm;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MVF_HIP(h, hipGetLastError());
+  slot = best;
+  S.lpr = best;
+  return 0;
+}
+
 template <int NC>
 static int solve_nc(mvf_handle *h, const double *omegas, int ncol, const mvf_solver_opts *o, const double *b,
-                    double *x, mvf_solve_info *info) {
+                    double *x, mvf_solve_info *info, double *alpha_out = nullptr) {
   SpmvTimer *tm = new SpmvTimer();
   tm->init();
   Solver S{h, ncol, {}, o->matrix == MVF_MATRIX_C64REF, 16, (int64_t)h->nne, tm};
   for (int c = 0; c < NC; c++) S.w.w[c] = c < ncol ? (double)(float)omegas[c] : 0.0;
-  int rc;
-  if (o->precond == MVF_PC_DIRECT) rc = run_gmres_direct<NC>(S, o, b, x, info);
+  int rc = h->autotune ? tune_lpr<NC>(S) : 0;
+  if (!rc && alpha_out) rc = guess_scale<NC>(S, b, x, alpha_out);
+  if (rc) {
+  } else if (o->precond == MVF_PC_DIRECT) rc = run_gmres_direct<NC>(S, o, b, x, info);
   else if (o->precision == MVF_PREC_MIXED) rc = run_cocg_mixed<NC>(S, o, b, x, info);
   else if (o->method == MVF_KRYLOV_BICGSTAB) rc = run_bicgstab<NC>(S, o, b, x, info);
   else rc = run_cocg64<NC>(S, o, b, x, info);
@@ -1123,6 +1328,7 @@ static int solve_nc(mvf_handle *h, const double *omegas, int ncol, const mvf_sol
   info->spmv_bytes[0] = nnz * (8 + 8 + 4) + n * (8 + NC * (16 + 16));
   info->spmv_bytes[1] = nnz * (4 + 4 + 4) + n * (8 + NC * (8 + 8));
   info->spmv_ncol = NC;
+  info->spmv_lpr = S.lpr;
   delete tm;
   return rc;
 }
@@ -1137,6 +1343,14 @@ int solve_device(mvf_handle *h, const double *omegas, int ncol, int ncol_pad, co
     case 16: return solve_nc<16>(h, omegas, ncol, o, b, x, info);
   }
   return set_err(h, MVF_E_ARG, "unsupported column count %d", ncol_pad);
+}
+
+// the same, starting from the caller's x scaled per polarization by the
+// least-squares factor (two columns)
+int solve_device_guess(mvf_handle *h, const double *omegas, const mvf_solver_opts *o, const double *b, double *x,
+                       mvf_solve_info *info, double *alpha_out) {
+  double a4[4];
+  return solve_nc<2>(h, omegas, 2, o, b, x, info, alpha_out ? alpha_out : a4);
 }
 
 }  // namespace mvf

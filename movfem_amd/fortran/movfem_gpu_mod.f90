@@ -50,11 +50,11 @@ module movfem_gpu
     real(c_double) :: seconds
     integer(c_int64_t) :: spmv_launches(2)
     real(c_double) :: spmv_ms(2), spmv_bytes(2)
-    integer(c_int32_t) :: spmv_ncol, pad_
+    integer(c_int32_t) :: spmv_ncol, spmv_lpr
     real(c_double) :: factor_seconds, factor_flops, factor_bytes
     integer(c_int32_t) :: factorizations, static_pivots, precond_applies, operator_fp32
     real(c_double) :: gemm_ms, gemm_flops
-    integer(c_int64_t) :: gemm_launches
+    integer(c_int64_t) :: gemm_launches, graph_launches
   end type mvf_solve_info
 
   ! grid_3d's arguments in the reference's layouts (geometry.f90:48-62)

@@ -299,3 +299,30 @@ def random_small(nodes=8, nx=3, ny=2, nz=2, seed=0, dirichlet=True, anisotropic=
     m.nextd = nextd
     m.nzl_top = max(1, min(nz_air, nextd))
     return m
+
+
+def hills_tensor(nx=128, ny=128, nz_earth=88, nz_air=8, h=500.0, nodes=27, freqs=tuple(np.logspace(-3, 3, 16)),
+                 n_hills=8, block=4, seed=5, dirichlet=False, nextd=4, name="hills_tensor") -> MTModel:
+    """Full-tensor earth under topography: the surface is the sum of n_hills
+    Gaussian hills (amplitude ~ U(0, 200] m, width ~ U[1, 5] km, centres ~ U
+    over the inner domain, seeded), every node column shifted by the surface
+    height times its relative height above the grid bottom (elements stay
+    right-handed); sigma as in anisotropic_blocks(full_tensor=True); GPML on +x,
+    +y and the top by default."""
+    m = anisotropic_blocks(nx=nx, ny=ny, nz_earth=nz_earth, nz_air=nz_air, h=h, nodes=nodes, freqs=freqs,
+                           block=block, seed=seed, full_tensor=True, dirichlet=dirichlet, nextd=nextd, name=name)
+    rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    amp = rng.uniform(0.0, 200.0, n_hills)
+    amp = np.where(amp == 0.0, 200.0, amp)
+    width = rng.uniform(1000.0, 5000.0, n_hills)
+    lo, hi = nextd * h, (nx - nextd) * h
+    cx = rng.uniform(lo, hi, n_hills)
+    cy = rng.uniform(nextd * h, (ny - nextd) * h, n_hills)
+    i, j, k = node_grid(m)
+    x, y = m.xp[i], m.yp[j]
+    surf = np.zeros(m.npt)
+    for a, w, x0, y0 in zip(amp, width, cx, cy):
+        surf += a * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2.0 * w * w))
+    zmax = m.zp.max()
+    m.zp = m.zp + surf * (m.zp / zmax)
+    return m

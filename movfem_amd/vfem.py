@@ -37,7 +37,8 @@ def _ip(a):
 
 
 def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacobi", tol=1e-8, maxit=100000,
-                check_every=25, inner_tol=1e-2, gmres_m=0, freqs_per_factor=1, direct_tile=0) -> L.SolverOpts:
+                check_every=25, inner_tol=1e-2, gmres_m=0, freqs_per_factor=1, direct_tile=0,
+                use_graph=False) -> L.SolverOpts:
     o = L.SolverOpts()
     L.lib().mvf_default_solver_opts(C.byref(o))
     o.method = {"cocg": L.MVF_KRYLOV_COCG, "bicgstab": L.MVF_KRYLOV_BICGSTAB, "gmres": L.MVF_KRYLOV_GMRES}[method]
@@ -47,6 +48,7 @@ def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacob
     o.gmres_m = gmres_m
     o.freqs_per_factor = freqs_per_factor
     o.direct_tile = direct_tile
+    o.use_graph = int(bool(use_graph))
     o.tol = tol
     o.maxit = maxit
     o.check_every = check_every
@@ -63,7 +65,8 @@ def info_dict(info: L.SolveInfo, ncol: int) -> dict:
             "factor_bytes": info.factor_bytes, "factorizations": info.factorizations,
             "static_pivots": info.static_pivots, "precond_applies": info.precond_applies,
             "gemm_ms": info.gemm_ms, "gemm_flops": info.gemm_flops, "gemm_launches": info.gemm_launches,
-            "operator_fp32": bool(info.operator_fp32)}
+            "operator_fp32": bool(info.operator_fp32), "spmv_lpr": info.spmv_lpr,
+            "graph_launches": info.graph_launches}
 
 
 class GpuVfem:
@@ -286,6 +289,27 @@ class GpuVfem:
         out = [np.empty(2 * npt * 3, np.complex128) for _ in range(3)]
         self._check(self._lib.mvf_problem_fields(self.h, *(_dp(a.view(np.float64)) for a in out)))
         return tuple(a.reshape(2, npt, 3) for a in out)
+
+
+def solve_batch(handles, freqs, nthreads=0, **kw):
+    """mvf_solve_batch: every handle (model) solves its row of freqs (nh, nfreq)
+    with solve_frequencies' semantics, several models in flight at once.
+    Returns (infos, status codes)."""
+    hs = (C.c_void_p * len(handles))(*[g.h.value for g in handles])
+    f = np.ascontiguousarray(freqs, np.float64)
+    if f.ndim == 1:
+        f = np.tile(f, (len(handles), 1))
+    nh, nf = f.shape
+    o = solver_opts(**kw)
+    infos = (L.SolveInfo * nh)()
+    st = np.zeros(nh, np.int32)
+    rc = L.lib().mvf_solve_batch(hs, nh, _dp(f), nf, C.byref(o), infos, _ip(st), int(nthreads))
+    out = [info_dict(infos[i], 2 * nf) for i in range(nh)]
+    if rc:
+        bad = [i for i in range(nh) if st[i]]
+        raise MvfError(rc, f"models {bad} failed: " + "; ".join(handles[i]._lib.mvf_last_error(handles[i].h).decode()
+                                                               for i in bad))
+    return out, st
 
 
 # ---------------------------------------------------------------- text output
