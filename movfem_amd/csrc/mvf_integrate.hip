@@ -338,9 +338,19 @@ int lane_tables(mvf_handle *h, const BasisTab *tab) {
 // depth index is ordered (u, a, q) with g = 4u + q: in an MFMA step the lanes
 // l with l / 16 = q supply depth entry q, so lane l holds, for every step of
 // one u and all three a, the entries of Gauss point 4u + l/16 and DOFs
-// 16 t + l%16; it computes them itself from the point's J^-1 (LDS) and the
-// reference-element tables, and no operand goes through LDS.  Only tiles on
-// or above the diagonal are formed; K first, then M.
+// 16 t + l%16.  Only tiles on or above the diagonal are formed; K first, then M.
+//
+// Operands per lane.  DOF i is N_i = phi_i grad(xi_d) with d = d_i its edge
+// direction, so with u = J^-1 grad_ref(phi_i) and v = J^-1 e_d (column d of
+// J^-1) the GPML-weighted curl is chat_a = q_a1 u_b v_c - q_a2 u_c v_b
+// ((a, b, c) cyclic): a linear map of grad_ref(phi_i),
+//   chat = Md(g) grad_ref(phi_i),  Md[a][m] = q_a1 v_c Ji[b][m] - q_a2 v_b Ji[c][m],
+// whose 3 x 3 matrix depends only on the point and the direction.  The setup
+// forms Md for the three directions at every point into LDS; a lane then
+// reads the matrix of its DOF's direction (a per-lane LDS address, no
+// selects), and its operands cost one 3 x 3 product each for C and for Q.
+// The reference-element tables are zero outside the element's points and
+// DOFs, so padded rows and columns vanish without masking.
 // MFMA lane layouts (16 x 16 x 4): A lane l -> A[l%16][l/16], B lane l ->
 // B[l/16][l%16]; D register v of lane l -> D[row][l%16] with row = l/16 + 4v
 // (f64) or 4 (l/16) + v (f32).
@@ -374,8 +384,21 @@ struct Acc4<float> {
   }
 };
 
+// per-point record in LDS: Md for d = 0, 1, 2 (27), J^-1 columns v_d (9),
+// w m1 (6), w h1h2h3 m2 (6)
+constexpr int kPt = 48;
+constexpr int kPtV = 27, kPtM1 = 36, kPtM2 = 42;
+
+// workgroups are dealt round-robin to the 8 XCDs; renumber them so that each
+// XCD takes a contiguous range of elements, whose shared lattice nodes then
+// meet in that XCD's L2
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t x = b & 7, i = b >> 3, q = nb >> 3, rm = nb & 7;
+  return x * q + (x < rm ? x : rm) + i;
+}
+
 template <typename T, int NN, int NE, int NG>
-__global__ __launch_bounds__(256, (NE > 36 ? 1 : 2)) void k_elem_km_mfma(
+__global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
     const BasisTab *__restrict__ tab, LaneTab<T> lt, const double *__restrict__ xp, const double *__restrict__ yp,
     const double *__restrict__ zp, const double *__restrict__ inmu, const double *__restrict__ sig, Geom g, Gpml gp,
     int prev, int first_flags, int64_t e0, int64_t n, T *__restrict__ Kout, T *__restrict__ Mout, int *bad) {
@@ -386,14 +409,12 @@ __global__ __launch_bounds__(256, (NE > 36 ? 1 : 2)) void k_elem_km_mfma(
   constexpr int NP = NE * (NE + 1) / 2;
   typedef typename Acc4<T>::type A4;
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
-  const int64_t le = (int64_t)blockIdx.x * WPB + wv;  // element index within the launch
+  const int64_t le = xcd_block(blockIdx.x, gridDim.x) * WPB + wv;  // element index within the launch
   const bool live = le < n;
   const int64_t e = e0 + (live ? le : 0);
   __shared__ double xs64[WPB][NN][3];
-  __shared__ T xsT[WPB][NN][3];
   __shared__ T nm[WPB][NN][12];
-  // per point: J^-1 (9), GPML curl weights (6), w m1 (6), w h1h2h3 m2 (6)
-  __shared__ T gpd[WPB][NG][27];
+  __shared__ T gpd[WPB][NG][kPt];
   int i, j, k;
   elem_index(g, e, i, j, k);
   if (l < NN) {
@@ -403,26 +424,37 @@ __global__ __launch_bounds__(256, (NE > 36 ? 1 : 2)) void k_elem_km_mfma(
     xs64[wv][l][0] = xp[ix];
     xs64[wv][l][1] = yp[jy];
     xs64[wv][l][2] = zp[id];
-    xsT[wv][l][0] = (T)xs64[wv][l][0];
-    xsT[wv][l][1] = (T)xs64[wv][l][1];
-    xsT[wv][l][2] = (T)xs64[wv][l][2];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       nm[wv][l][c] = (T)inmu[id * 6 + c];
       nm[wv][l][6 + c] = (T)sig[2 * (id * 6 + c)];
     }
   }
-  __syncthreads();
+  // every array above is private to its wave: a wave barrier orders the
+  // stores before the reads of other lanes
+  __builtin_amdgcn_wave_barrier();
   if (l < NG) {
+    // GPML stretch of the point first, while little else is live
+    double h[3] = {1.0, 1.0, 1.0};
+    if (gp.on) {
+      int fl = elem_flags(g, gp, prev, first_flags, e);
+      double rr[3] = {0, 0, 0};
+      for (int m = 0; m < NN; m++)
+        for (int c = 0; c < 3; c++) rr[c] += tab->N[l][m] * xs64[wv][m][c];
+      for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, rr[c]);
+    }
+    const double h3 = h[0] * h[1] * h[2];
+    // GPML curl weights: component a takes q_a1 on u_b v_c and q_a2 on u_c v_b
+    const T q0 = (T)sqrt(h[0] * h[2] / h[1]), q1 = (T)sqrt(h[0] * h[1] / h[2]), q2 = (T)sqrt(h[1] * h[2] / h[0]);
     // Jacobian from the point-fastest derivative table (coalesced across lanes)
     T J[3][3];
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
       for (int c = 0; c < 3; c++) J[m][c] = 0;
-#pragma unroll 4
+#pragma unroll 2
     for (int m = 0; m < NN; m++) {
-      const T x0 = xsT[wv][m][0], x1 = xsT[wv][m][1], x2 = xsT[wv][m][2];
+      const T x0 = (T)xs64[wv][m][0], x1 = (T)xs64[wv][m][1], x2 = (T)xs64[wv][m][2];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         const T dn = lt.dN[(a * NN + m) * 32 + l];
@@ -449,7 +481,7 @@ __global__ __launch_bounds__(256, (NE > 36 ? 1 : 2)) void k_elem_km_mfma(
     T m1[6], m2[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) m1[c] = m2[c] = 0;
-#pragma unroll 4
+#pragma unroll 2
     for (int m = 0; m < NN; m++) {
       const T Nm = lt.N[m * 32 + l];
 #pragma unroll
@@ -458,97 +490,116 @@ __global__ __launch_bounds__(256, (NE > 36 ? 1 : 2)) void k_elem_km_mfma(
         m2[c] += Nm * nm[wv][m][6 + c];
       }
     }
-    double h[3] = {1.0, 1.0, 1.0};
-    if (gp.on) {
-      int fl = elem_flags(g, gp, prev, first_flags, e);
-      double rr[3] = {0, 0, 0};
-      for (int m = 0; m < NN; m++)
-        for (int c = 0; c < 3; c++) rr[c] += tab->N[l][m] * xs64[wv][m][c];
-      for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, rr[c]);
-    }
-    double h3 = h[0] * h[1] * h[2];
-    double q0 = sqrt(h[0] * h[2] / h[1]), q1 = sqrt(h[0] * h[1] / h[2]), q2 = sqrt(h[1] * h[2] / h[0]);
+    const T qa1[3] = {q0, q1, q2}, qa2[3] = {q1, q2, q0};
     T *o = gpd[wv][l];
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) {
+      const T v[3] = {Ji[0 * 3 + d], Ji[1 * 3 + d], Ji[2 * 3 + d]};
 #pragma unroll
-    for (int c = 0; c < 9; c++) o[c] = Ji[c];
-    o[9] = (T)q0; o[10] = (T)q1; o[11] = (T)q1; o[12] = (T)q2; o[13] = (T)q2; o[14] = (T)q0;
+      for (int a = 0; a < 3; a++) {
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+#pragma unroll
+        for (int m = 0; m < 3; m++) o[d * 9 + a * 3 + m] = qa1[a] * v[c] * Ji[b * 3 + m] - qa2[a] * v[b] * Ji[c * 3 + m];
+        o[kPtV + d * 3 + a] = v[a];
+      }
+    }
     const T wh = (T)((double)wgt * h3);
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      o[15 + c] = wgt * m1[c];
-      o[21 + c] = wh * m2[c];
+      o[kPtM1 + c] = wgt * m1[c];
+      o[kPtM2 + c] = wh * m2[c];
     }
   }
-  __syncthreads();
+  __builtin_amdgcn_wave_barrier();
+  // this lane's DOF direction in each tile (0 for padding DOFs, whose table
+  // entries are zero)
+  int dr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) dr[t] = lt.dir[t * 16 + r];
   const int64_t base = le * (int64_t)NP;
-#pragma unroll 1
-  for (int pass = 0; pass < 2; pass++) {  // 0: K, 1: M
+  auto store = [&](T *out, const A4 *acc) {
+    if (!live) return;
+    int p = 0;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int t2 = t; t2 < NT; t2++, p++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int ii = 16 * t + mfma_drow<T>(l, v), jj = 16 * t2 + r;
+          if (ii <= jj && jj < NE) out[base + ii * NE - ii * (ii - 1) / 2 + (jj - ii)] = acc[p][v];
+        }
+  };
+  {  // K = C^T Q
     A4 acc[NPAIR];
 #pragma unroll
     for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
 #pragma unroll 1
     for (int u = 0; u < NU; u++) {
       const int gg = 4 * u + q;
-      const bool gv = gg < NG;
-      const T *pd = gpd[wv][gv ? gg : 0];
-      T Ji[9], mw[6];
+      const T *pd = gpd[wv][gg < NG ? gg : 0];
+      T mw[6];
 #pragma unroll
-      for (int c = 0; c < 9; c++) Ji[c] = pd[c];
+      for (int c = 0; c < 6; c++) mw[c] = pd[kPtM1 + c];
+      // tiles from the last: the products of tile t need the Q columns of
+      // tiles t..NT-1 only, so each tile's C columns die after its MFMAs
+      T bv[NT][3];
 #pragma unroll
-      for (int c = 0; c < 6; c++) mw[c] = pd[(pass == 0 ? 15 : 21) + c];
-      T av[NT][3], bv[NT][3];
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const int ed = 16 * t + r;
-        const bool ok = gv && ed < NE;
+      for (int t = NT - 1; t >= 0; t--) {
         const int li = (u * NT + t) * 64 + l;
-        const int d = lt.dir[t * 16 + r];
-        const T v0 = Ji[0 * 3 + d], v1 = Ji[1 * 3 + d], v2 = Ji[2 * 3 + d];
-        T x0, x1, x2;
-        if (pass == 0) {
-          const T dp0 = lt.d0[li], dp1 = lt.d1[li], dp2 = lt.d2[li];
-          const T dn0 = Ji[0] * dp0 + Ji[1] * dp1 + Ji[2] * dp2;
-          const T dn1 = Ji[3] * dp0 + Ji[4] * dp1 + Ji[5] * dp2;
-          const T dn2 = Ji[6] * dp0 + Ji[7] * dp1 + Ji[8] * dp2;
-          x0 = pd[9] * (dn1 * v2) - pd[10] * (dn2 * v1);
-          x1 = pd[11] * (dn2 * v0) - pd[12] * (dn0 * v2);
-          x2 = pd[13] * (dn0 * v1) - pd[14] * (dn1 * v0);
-        } else {
-          const T ph = lt.p[li];
-          x0 = ph * v0;
-          x1 = ph * v1;
-          x2 = ph * v2;
-        }
-        if (!ok) x0 = x1 = x2 = 0;
-        av[t][0] = x0;
-        av[t][1] = x1;
-        av[t][2] = x2;
-        bv[t][0] = mw[0] * x0 + mw[1] * x1 + mw[2] * x2;
-        bv[t][1] = mw[1] * x0 + mw[3] * x1 + mw[4] * x2;
-        bv[t][2] = mw[2] * x0 + mw[4] * x1 + mw[5] * x2;
-      }
+        const T dp0 = lt.d0[li], dp1 = lt.d1[li], dp2 = lt.d2[li];
+        const T *md = pd + dr[t] * 9;
+        T av[3];
+        av[0] = md[0] * dp0 + md[1] * dp1 + md[2] * dp2;
+        av[1] = md[3] * dp0 + md[4] * dp1 + md[5] * dp2;
+        av[2] = md[6] * dp0 + md[7] * dp1 + md[8] * dp2;
+        bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
+        bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
+        bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
 #pragma unroll
-      for (int a = 0; a < 3; a++) {
-        int p = 0;
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int t = 0; t < NT; t++)
-#pragma unroll
-          for (int t2 = t; t2 < NT; t2++, p++) acc[p] = Acc4<T>::mma(av[t][a], bv[t2][a], acc[p]);
-      }
-    }
-    if (live) {
-      T *out = pass == 0 ? Kout : Mout;
-      int p = 0;
-#pragma unroll
-      for (int t = 0; t < NT; t++)
-#pragma unroll
-        for (int t2 = t; t2 < NT; t2++, p++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int ii = 16 * t + mfma_drow<T>(l, v), jj = 16 * t2 + r;
-            if (ii <= jj && jj < NE) out[base + ii * NE - ii * (ii - 1) / 2 + (jj - ii)] = acc[p][v];
+          for (int t2 = t; t2 < NT; t2++) {
+            const int p = t * NT - t * (t - 1) / 2 + (t2 - t);
+            acc[p] = Acc4<T>::mma(av[a], bv[t2][a], acc[p]);
           }
+      }
     }
+    store(Kout, acc);
+  }
+  {  // M = V^T P
+    A4 acc[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
+#pragma unroll 1
+    for (int u = 0; u < NU; u++) {
+      const int gg = 4 * u + q;
+      const T *pd = gpd[wv][gg < NG ? gg : 0];
+      T mw[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
+      T bv[NT][3];
+#pragma unroll
+      for (int t = NT - 1; t >= 0; t--) {
+        const T ph = lt.p[(u * NT + t) * 64 + l];
+        const T *vd = pd + kPtV + dr[t] * 3;
+        T av[3];
+        av[0] = ph * vd[0];
+        av[1] = ph * vd[1];
+        av[2] = ph * vd[2];
+        bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
+        bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
+        bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int t2 = t; t2 < NT; t2++) {
+            const int p = t * NT - t * (t - 1) / 2 + (t2 - t);
+            acc[p] = Acc4<T>::mma(av[a], bv[t2][a], acc[p]);
+          }
+      }
+    }
+    store(Mout, acc);
   }
 }
 
