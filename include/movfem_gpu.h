@@ -140,10 +140,13 @@ int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_
 
 /* ---- solve (replaces zmumps(job=6), MoVFEM_3DMT.f90:113-115) --------------- */
 enum { MVF_KRYLOV_COCG = 0, MVF_KRYLOV_BICGSTAB = 1, MVF_KRYLOV_GMRES = 2 };
-enum { MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1 };  /* MIXED: FP32 inner Krylov, FP64 refinement */
+/* MIXED: FP32 inner Krylov, FP64 refinement; with MVF_PC_DIRECT: the factor's
+ * entries rounded to complex64 after every update (an emulated complex64
+ * factorization, FP64 arithmetic between the roundings), FP64 GMRES outside */
+enum { MVF_PREC_FP64 = 0, MVF_PREC_MIXED = 1 };
 enum { MVF_MATRIX_C64REF = 0, MVF_MATRIX_FP64 = 1 };/* which operator the residual is judged on */
 /* MVF_PC_DIRECT: multifrontal LU (FP64, nested dissection of the element box)
- * of the operator at the batch's first frequency, with a small imaginary
+ * of the operator at the geometric mean of the batch's frequencies, with a small imaginary
  * shift on the diagonal of massless (all-air) DOFs; the outer method is then
  * right-preconditioned GMRES on the unshifted operator whatever `method` says.
  * It stands where zmumps(job=6) stands (MoVFEM_3DMT.f90:113-115). */

@@ -98,6 +98,7 @@ struct DirectPlan {
   int factored = 0;
   double omega = -1.0;
   int c64 = 0;
+  int f32 = 0;  // factor entries rounded to complex64 (emulated complex64 factorization)
   int npert = 0;
   double t_symbolic = 0.0, t_factor = 0.0;
   int64_t factorizations = 0;
@@ -454,6 +455,14 @@ __device__ __forceinline__ double lane_bcast(double v, int q) {
 __device__ __forceinline__ double2 cinv(double2 a) {
   const double d = a.x * a.x + a.y * a.y;
   return make_double2(a.x / d, -a.y / d);
+}
+
+// every entry rounded to complex64 (the emulated complex64 factorization)
+__global__ void k_round_c64(double2 *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = p[i];
+    p[i] = make_double2((double)(float)v.x, (double)(float)v.y);
+  }
 }
 
 __global__ void k_fill2(double2 *p, int64_t n) {
@@ -1613,7 +1622,13 @@ static void launch_gemm(bool big, unsigned gx, unsigned gy, hipStream_t st, cons
                        pool);
 }
 
-static int direct_factor(mvf_handle *h, double w, int c64) {
+// f32: emulated complex64 factorization.  Every front entry (top band in the
+// factor store, bottom band in the level pool) and the block inverses are
+// rounded to complex64 after the assembly, after each outer-block update, after
+// the Y solve and after the Schur update; the arithmetic between roundings is
+// FP64 and the lookahead is off, so the rounding points are ordered on one
+// stream.  A factorization in complex64 arithmetic rounds at least as often.
+static int direct_factor(mvf_handle *h, double w, int c64, int f32 = 0) {
   DirectPlan *P = (DirectPlan *)h->direct;
   hipStream_t st = h->stream;
   double t0 = now_s();
@@ -1656,6 +1671,16 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
                            P->d_rel, s, P->d_fac, pool, prev);
     }
     MVF_HIP(h, hipGetLastError());
+    auto round_level = [&]() {
+      if (!f32) return;
+      hipLaunchKernelGGL(k_round_c64, dim3(cap_grid((nfac + 255) / 256)), dim3(256), 0, st, P->d_fac + P->lev_fac0[d],
+                         nfac);
+      hipLaunchKernelGGL(k_round_c64, dim3(cap_grid((P->lev_pool[d] + 255) / 256)), dim3(256), 0, st, pool,
+                         P->lev_pool[d]);
+      hipLaunchKernelGGL(k_round_c64, dim3(cap_grid((P->inv_entries + 255) / 256)), dim3(256), 0, st, P->d_inv,
+                         P->inv_entries);
+    };
+    round_level();
     const bool big = P->tile ? P->tile == kBig : mm >= kBigFront;
     const int64_t tile = big ? kBig : kTile;
     const int64_t tile_n = big ? kBigN : kTile;
@@ -1707,7 +1732,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
       // pivoting is restricted to the panel's rows, so those panels touch
       // neither the trailing part's rows nor its columns.  The next outer
       // update waits for it.
-      const bool ahead = mn1 > kOuter;
+      const bool ahead = !f32 && mn1 > kOuter;
       bool pending = false;
       for (int64_t K0 = 0; K0 < mn1; K0 += kOuter) {
         const int64_t Kend = std::min(K0 + kOuter, mn1);
@@ -1746,6 +1771,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
         } else {
           gemm(M4{}, 0, K0, st);
         }
+        round_level();
       }
       if (pending) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
       if (mn2 > 0) {
@@ -1776,7 +1802,9 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
           }
         }
         if (ypend) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
+        round_level();
         gemm(M2{}, 0, 0, st);
+        round_level();
       }
       hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
     }
@@ -1812,6 +1840,7 @@ static int direct_factor(mvf_handle *h, double w, int c64) {
   P->factored = 1;
   P->omega = w;
   P->c64 = c64;
+  P->f32 = f32;
   P->factorizations++;
   P->t_factor = now_s() - t0;
   return 0;
@@ -1943,9 +1972,10 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
   const double w0 = std::sqrt(wlo * whi);
   if (o->direct_tile != 0 && o->direct_tile != kTile && o->direct_tile != kBig)
     return set_err(h, MVF_E_ARG, "direct_tile must be 0, %d or %d", kTile, kBig);
-  if (!P->factored || P->omega != w0 || P->c64 != (int)S.c64 || P->tile != o->direct_tile) {
+  const int f32 = o->precision == MVF_PREC_MIXED;
+  if (!P->factored || P->omega != w0 || P->c64 != (int)S.c64 || P->f32 != f32 || P->tile != o->direct_tile) {
     P->tile = o->direct_tile;
-    MVF_TRY(direct_factor(h, w0, (int)S.c64));
+    MVF_TRY(direct_factor(h, w0, (int)S.c64, f32));
   }
   info->factor_seconds += P->t_factor;
   info->factorizations = (int32_t)P->factorizations;

@@ -326,3 +326,22 @@ def hills_tensor(nx=128, ny=128, nz_earth=88, nz_air=8, h=500.0, nodes=27, freqs
     zmax = m.zp.max()
     m.zp = m.zp + surf * (m.zp / zmax)
     return m
+
+
+def many_models(count=32, lo=(32, 32, 24), hi=(80, 80, 60), nz_air=4, h=500.0, freqs=tuple(np.logspace(-3, 2, 4)),
+                seed=3, dirichlet=True) -> list:
+    """The many-model workload: `count` models whose dimensions are drawn per
+    model as lo + round(s * (hi - lo)) with s ~ U[0, 1] (PCG64 seed), element
+    types alternating 8 / 20 nodes, full-tensor sigma per 4^3 cell block
+    (principal values log-uniform in [1e-3, 1] S/m, ZYZ Euler angles), nz_air
+    air layers on top of the earth, Dirichlet boundaries (or GPML)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for q in range(count):
+        s = rng.uniform(0.0, 1.0)
+        nx, ny, nz = (int(a + round(s * (b - a))) for a, b in zip(lo, hi))
+        out.append(anisotropic_blocks(nx=nx, ny=ny, nz_earth=nz - nz_air, nz_air=nz_air, h=h,
+                                      nodes=8 if q % 2 == 0 else 20, freqs=freqs, block=4,
+                                      seed=int(rng.integers(1 << 31)), full_tensor=True, dirichlet=dirichlet,
+                                      nextd=2 if dirichlet else 4, name=f"many_{q}"))
+    return out
