@@ -160,9 +160,12 @@ __global__ __launch_bounds__(256) void k_conv(Dims d, const bf16 *__restrict__ X
 }
 
 // dW[tap][co][ci] += sum over a chunk of voxels of dY[v][co] X[v+off(tap)][ci].
-// Grid (chunks, taps); each wave takes the (co tile, ci tile) pairs p with
-// p % 4 == wave.  A operand (rows co, depth voxels) and B operand (depth
-// voxels, columns ci) gathered four voxels per lane.
+// Grid (chunks, taps).  With at least four (co tile, ci tile) pairs each wave
+// takes the pairs p with p % 4 == wave over the whole chunk; with fewer, every
+// wave takes all pairs over every fourth group of 16 voxels, so that no wave
+// idles on the finest level (16 x 16 channels, one pair).  A operand (rows co,
+// depth voxels) and B operand (depth voxels, columns ci) gathered four voxels
+// per lane.
 template <int KS>
 __global__ __launch_bounds__(256) void k_wgrad(Dims d, const bf16 *__restrict__ X, int cx, int cin,
                                                const bf16 *__restrict__ dY, int cdy, int cout, int64_t chunk,
@@ -173,13 +176,16 @@ __global__ __launch_bounds__(256) void k_wgrad(Dims d, const bf16 *__restrict__ 
   const int dx = tap / (KS * KS) - R, dy = (tap / KS) % KS - R, dz = tap % KS - R;
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
   const int mtn = cout / 16, ntn = cin / 16, np = mtn * ntn;
+  const int pw = np >= 4 ? 4 : 1;            // pair stride across waves
+  const int pofs = wv % pw, vsl = wv / pw;   // this wave's first pair and voxel slice
+  const int64_t vstep = 16 * (4 / pw);
   const int64_t V = d.vox();
   const int64_t vb = (int64_t)blockIdx.x * chunk, ve = vb + chunk < V ? vb + chunk : V;
   f4v acc[MAXP];
 #pragma unroll
   for (int p = 0; p < MAXP; p++) acc[p] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
-  for (int64_t k0 = vb; k0 < ve; k0 += 16) {
+  for (int64_t k0 = vb + 16 * vsl; k0 < ve; k0 += vstep) {
     // the four voxels of this lane's depth slice, and their shifted partners
     int64_t vv[4], vs[4];
     bool okv[4], oks[4];
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(256) void k_wgrad(Dims d, const bf16 *__restrict__ 
     }
 #pragma unroll
     for (int pp = 0; pp < MAXP; pp++) {
-      const int p = wv + 4 * pp;
+      const int p = pofs + pw * pp;
       if (p >= np) continue;
       const int mt = p / ntn, nt = p % ntn;
       s4v a, b;
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(256) void k_wgrad(Dims d, const bf16 *__restrict__ 
   }
 #pragma unroll
   for (int pp = 0; pp < MAXP; pp++) {
-    const int p = wv + 4 * pp;
+    const int p = pofs + pw * pp;
     if (p >= np) continue;
     const int mt = p / ntn, nt = p % ntn;
 #pragma unroll

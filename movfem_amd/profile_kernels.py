@@ -1,0 +1,92 @@
+"""Drives one kernel family for counter collection (rocprofv3 --pmc / --kernel-trace).
+
+    python -m movfem_amd.profile_kernels elem  [--nodes 20] [--precision fp64|fp32] [--reps 5]
+    python -m movfem_amd.profile_kernels spmv  [--nodes 20] [--iters 50]
+    python -m movfem_amd.profile_kernels unet  [--steps 3]
+    python -m movfem_amd.profile_kernels copy  [--mib 1024]
+
+elem: mvf_assemble (element integration and assembly of K and M) repeated on
+  the anisotropic GPML workload of the given element type (20-node: 64 x 64 x 48
+  cells; 8-node: the same grid; 27-node: 32 x 32 x 24 cells).
+spmv: Jacobi COCG on the same model at 1 Hz for a fixed iteration count (the
+  operator apply of every Krylov path, two columns); the solve is stopped by its
+  iteration cap, not converged.
+unet: forward + backward + Adam steps of the learned-guess network on a batch
+  of two 64 x 64 x 48 samples (random inputs).
+copy: three device-to-device copies of a known size, the reference the HBM
+  traffic counters are scaled by.
+Each prints one JSON line with what it ran.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+import numpy as np
+
+
+def model(nodes: int):
+    from . import model as M
+
+    if nodes == 27:
+        return M.anisotropic_blocks(nx=32, ny=32, nz_earth=20, nz_air=4, h=500.0, nodes=27, freqs=(1.0,), block=4,
+                                    seed=2, nextd=4)
+    return M.anisotropic_blocks(nx=64, ny=64, nz_earth=40, nz_air=8, h=500.0, nodes=nodes, freqs=(1.0,), block=4,
+                                seed=2, nextd=4)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["elem", "spmv", "unet", "copy"])
+    ap.add_argument("--nodes", type=int, default=20)
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--mib", type=int, default=1024)
+    a = ap.parse_args()
+    out = {"what": a.what}
+    if a.what == "copy":
+        import torch
+
+        n = a.mib << 20
+        x = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        out["bytes_per_copy"] = n
+    elif a.what == "unet":
+        from .learned import Net
+
+        net = Net(batch=2)
+        rng = np.random.default_rng(0)
+        x = rng.standard_normal((2, 64, 64, 48, 3)).astype(np.float32)
+        t = rng.standard_normal((2, 64, 64, 48, 12)).astype(np.float32)
+        net.set_timing(True)
+        for _ in range(a.steps):
+            out["loss"] = net.forward_backward(x, t)
+            net.adam(1e-3)
+        out.update(net.stats())
+        out["tflops"] = out["conv_flops"] / out["conv_ms"] / 1e9
+    else:
+        from .vfem import GpuVfem, MvfError
+
+        m = model(a.nodes)
+        g = GpuVfem(m)
+        out.update(nodes=a.nodes, elements=m.ne, nne=g.nne, nnz=g.nnze)
+        if a.what == "elem":
+            ms = [g.assemble(a.precision) for _ in range(a.reps)]
+            out.update(precision=a.precision, reps=a.reps, kernel_ms=ms, elem_per_s=m.ne / (np.median(ms) * 1e-3))
+        else:
+            try:
+                info = g.solve_frequencies([1.0], method="cocg", tol=1e-14, maxit=a.iters, check_every=a.iters)
+            except MvfError as e:
+                info = {"stopped": str(e)[:120]}
+            out.update(iters=a.iters, info={k: info[k] for k in ("spmv_launches", "spmv_ms", "spmv_bytes", "spmv_lpr")
+                                              if k in info})
+    print(json.dumps(out, default=float), flush=True)
+
+
+if __name__ == "__main__":
+    main()
