@@ -307,8 +307,10 @@ def solve_batch(handles, freqs, nthreads=0, **kw):
     out = [info_dict(infos[i], 2 * nf) for i in range(nh)]
     if rc:
         bad = [i for i in range(nh) if st[i]]
-        raise MvfError(rc, f"models {bad} failed: " + "; ".join(handles[i]._lib.mvf_last_error(handles[i].h).decode()
-                                                               for i in bad))
+        raise MvfError(rc, f"models {bad} failed: " + "; ".join(
+            f"{i}: {handles[i]._lib.mvf_last_error(handles[i].h).decode()} (iterations {out[i]['iterations']}, "
+            f"factorizations {out[i]['factorizations']}, static pivots {out[i]['static_pivots']}, true residual "
+            f"{max(out[i]['true_resid'][:2 * nf]):.2e})" for i in bad))
     return out, st
 
 
