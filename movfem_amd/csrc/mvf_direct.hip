@@ -1786,10 +1786,21 @@ static int direct_factor(mvf_handle *h, double w, int c64, int f32 = 0) {
                              P->d_nodes, lev, s, P->d_fac);
           gemm(M1{}, s, 0, st);
           if (ahead) {
-            bool any = false;
-            for (int t : P->levels[d]) any = any || gemm_args(5, P->nodes[t], s, 2).ok;
-            if (any) {
-              if (ypend) MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
+            // at a group boundary the side stream's pending update (rows above
+            // the previous group's next group) must end before this stream
+            // touches those rows again: part 1 here writes the next group,
+            // which the pending part 2 covered, and the top group has no part 2
+            // of its own to carry the wait
+            bool any1 = false, any2 = false;
+            for (int t : P->levels[d]) {
+              any1 = any1 || gemm_args(5, P->nodes[t], s, 1).ok;
+              any2 = any2 || gemm_args(5, P->nodes[t], s, 2).ok;
+            }
+            if ((any1 || any2) && ypend) {
+              MVF_HIP(h, hipStreamWaitEvent(st, P->ev_join, 0));
+              ypend = false;
+            }
+            if (any2) {
               MVF_HIP(h, hipEventRecord(P->ev_fork, st));
               MVF_HIP(h, hipStreamWaitEvent(P->st2, P->ev_fork, 0));
               gemm(M5{}, s, 2, P->st2);
