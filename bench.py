@@ -313,7 +313,7 @@ def main():
            "iterations": [i["iterations"] for i in infos], "true_resid": resid, "true_resid_max": max(resid),
            "step_seconds": step_s, "roofline": roofline, "spmv_roofline": spmv,
            "elem_mats_per_s": m.ne / (e64 * 1e-3), "elem_kernel_ms": e64,
-           "elem_roofline": {"bound": "fp64 valu", "achieved": m.ne * F / (e64 * 1e-3) / 1e12,
+           "elem_roofline": {"bound": "mfma", "achieved": m.ne * F / (e64 * 1e-3) / 1e12,
                              "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                              "frac": m.ne * F / (e64 * 1e-3) / 1e12 / FP64_PEAK_TFS, "flops_per_elem": F}}
     log(f"timed {steps} steps in {dt:.1f}s: {value:.4f} solves/s; elem {out['elem_mats_per_s']:.3g}/s")

@@ -389,14 +389,6 @@ struct Acc4<float> {
 constexpr int kPt = 48;
 constexpr int kPtV = 27, kPtM1 = 36, kPtM2 = 42;
 
-// workgroups are dealt round-robin to the 8 XCDs; renumber them so that each
-// XCD takes a contiguous range of elements, whose shared lattice nodes then
-// meet in that XCD's L2
-__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
-  const int64_t x = b & 7, i = b >> 3, q = nb >> 3, rm = nb & 7;
-  return x * q + (x < rm ? x : rm) + i;
-}
-
 template <typename T, int NN, int NE, int NG>
 __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
     const BasisTab *__restrict__ tab, LaneTab<T> lt, const double *__restrict__ xp, const double *__restrict__ yp,
@@ -534,7 +526,7 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
     A4 acc[NPAIR];
 #pragma unroll
     for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
-#pragma unroll 1
+#pragma unroll 2
     for (int u = 0; u < NU; u++) {
       const int gg = 4 * u + q;
       const T *pd = gpd[wv][gg < NG ? gg : 0];
@@ -571,7 +563,7 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
     A4 acc[NPAIR];
 #pragma unroll
     for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
-#pragma unroll 1
+#pragma unroll 2
     for (int u = 0; u < NU; u++) {
       const int gg = 4 * u + q;
       const T *pd = gpd[wv][gg < NG ? gg : 0];
@@ -600,6 +592,232 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
       }
     }
     store(Mout, acc);
+  }
+}
+
+// k_elem_km_mfma_multi: k_elem_km_mfma with EPW elements per wave, for elements
+// whose nodes and points fill a fraction of a wave (8-node: 8 nodes, 8 points,
+// four elements per wave).  The node gather and the point setup then keep
+// EPW * 8 lanes busy instead of 8; the MFMA passes run element by element.
+// (One kernel for both costs the 20-node instantiation 11 registers and an
+// occupancy step, measured: 72 -> 61M element matrices/s.)
+template <typename T, int NN, int NE, int NG, int EPW>
+__global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
+    const BasisTab *__restrict__ tab, LaneTab<T> lt, const double *__restrict__ xp, const double *__restrict__ yp,
+    const double *__restrict__ zp, const double *__restrict__ inmu, const double *__restrict__ sig, Geom g, Gpml gp,
+    int prev, int first_flags, int64_t e0, int64_t n, T *__restrict__ Kout, T *__restrict__ Mout, int *bad) {
+  constexpr int WPB = 4;                 // waves per workgroup
+  static_assert(EPW * NN <= 64 && EPW * NG <= 64, "the elements of a wave share its 64 lanes");
+  constexpr int NT = (NE + 15) / 16;     // DOF tiles
+  constexpr int NU = (NG + 3) / 4;       // Gauss-point groups of four
+  constexpr int NPAIR = NT * (NT + 1) / 2;
+  constexpr int NP = NE * (NE + 1) / 2;
+  typedef typename Acc4<T>::type A4;
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+  // first element of this wave within the launch
+  const int64_t le0 = (xcd_block(blockIdx.x, gridDim.x) * WPB + wv) * EPW;
+  __shared__ double xs64[WPB][EPW][NN][3];
+  __shared__ T nm[WPB][EPW][NN][12];
+  __shared__ T gpd[WPB][EPW][NG][kPt];
+  // lane -> (element of the wave, node); the element's lattice position
+  const int ek0 = l / NN;
+  const int64_t le_n = le0 + ek0;
+  const int64_t e_n = e0 + (le_n < n ? le_n : 0);
+  int i, j, k;
+  elem_index(g, e_n, i, j, k);
+  if (l < EPW * NN) {
+    const int ek = ek0, nd = l - ek * NN;
+    int64_t id;
+    int ix, jy;
+    load_node(tab, g, i, j, k, nd, id, ix, jy);
+    xs64[wv][ek][nd][0] = xp[ix];
+    xs64[wv][ek][nd][1] = yp[jy];
+    xs64[wv][ek][nd][2] = zp[id];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      nm[wv][ek][nd][c] = (T)inmu[id * 6 + c];
+      nm[wv][ek][nd][6 + c] = (T)sig[2 * (id * 6 + c)];
+    }
+  }
+  // every array above is private to its wave: a wave barrier orders the
+  // stores before the reads of other lanes
+  __builtin_amdgcn_wave_barrier();
+  if (l < EPW * NG) {
+    const int ek = l / NG, pt = l - ek * NG;
+    const int64_t le = le0 + ek;
+    const bool live = le < n;
+    const int64_t e = e0 + (live ? le : 0);
+    // GPML stretch of the point first, while little else is live
+    double h[3] = {1.0, 1.0, 1.0};
+    if (gp.on) {
+      int fl = elem_flags(g, gp, prev, first_flags, e);
+      double rr[3] = {0, 0, 0};
+      for (int m = 0; m < NN; m++)
+        for (int c = 0; c < 3; c++) rr[c] += tab->N[pt][m] * xs64[wv][ek][m][c];
+      for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, rr[c]);
+    }
+    const double h3 = h[0] * h[1] * h[2];
+    // GPML curl weights: component a takes q_a1 on u_b v_c and q_a2 on u_c v_b
+    const T q0 = (T)sqrt(h[0] * h[2] / h[1]), q1 = (T)sqrt(h[0] * h[1] / h[2]), q2 = (T)sqrt(h[1] * h[2] / h[0]);
+    // Jacobian from the point-fastest derivative table (coalesced across lanes)
+    T J[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) J[m][c] = 0;
+#pragma unroll 2
+    for (int m = 0; m < NN; m++) {
+      const T x0 = (T)xs64[wv][ek][m][0], x1 = (T)xs64[wv][ek][m][1], x2 = (T)xs64[wv][ek][m][2];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const T dn = lt.dN[(a * NN + m) * 32 + pt];
+        J[a][0] += dn * x0;
+        J[a][1] += dn * x1;
+        J[a][2] += dn * x2;
+      }
+    }
+    T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) + J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    const T ad = det < 0 ? -det : det;
+    T Ji[9];
+    Ji[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / ad;
+    Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / ad;
+    Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / ad;
+    Ji[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / ad;
+    Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / ad;
+    Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / ad;
+    Ji[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / ad;
+    Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / ad;
+    Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / ad;
+    if (live && det == (T)0) atomicExch(bad, 1);
+    T wgt = det * lt.w[pt];
+    T m1[6], m2[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) m1[c] = m2[c] = 0;
+#pragma unroll 2
+    for (int m = 0; m < NN; m++) {
+      const T Nm = lt.N[m * 32 + pt];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        m1[c] += Nm * nm[wv][ek][m][c];
+        m2[c] += Nm * nm[wv][ek][m][6 + c];
+      }
+    }
+    const T qa1[3] = {q0, q1, q2}, qa2[3] = {q1, q2, q0};
+    T *o = gpd[wv][ek][pt];
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) {
+      const T v[3] = {Ji[0 * 3 + d], Ji[1 * 3 + d], Ji[2 * 3 + d]};
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+#pragma unroll
+        for (int m = 0; m < 3; m++) o[d * 9 + a * 3 + m] = qa1[a] * v[c] * Ji[b * 3 + m] - qa2[a] * v[b] * Ji[c * 3 + m];
+        o[kPtV + d * 3 + a] = v[a];
+      }
+    }
+    const T wh = (T)((double)wgt * h3);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      o[kPtM1 + c] = wgt * m1[c];
+      o[kPtM2 + c] = wh * m2[c];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // this lane's DOF direction in each tile (0 for padding DOFs, whose table
+  // entries are zero)
+  int dr[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) dr[t] = lt.dir[t * 16 + r];
+  for (int ek = 0; ek < EPW; ek++) {
+  const int64_t le = le0 + ek;
+  const bool live = le < n;
+  const int64_t base = le * (int64_t)NP;
+  const T(*gk)[kPt] = gpd[wv][ek];
+  auto store = [&](T *out, const A4 *acc) {
+    if (!live) return;
+    int p = 0;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int t2 = t; t2 < NT; t2++, p++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int ii = 16 * t + mfma_drow<T>(l, v), jj = 16 * t2 + r;
+          if (ii <= jj && jj < NE) out[base + ii * NE - ii * (ii - 1) / 2 + (jj - ii)] = acc[p][v];
+        }
+  };
+  {  // K = C^T Q
+    A4 acc[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
+#pragma unroll 2
+    for (int u = 0; u < NU; u++) {
+      const int gg = 4 * u + q;
+      const T *pd = gk[gg < NG ? gg : 0];
+      T mw[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) mw[c] = pd[kPtM1 + c];
+      // tiles from the last: the products of tile t need the Q columns of
+      // tiles t..NT-1 only, so each tile's C columns die after its MFMAs
+      T bv[NT][3];
+#pragma unroll
+      for (int t = NT - 1; t >= 0; t--) {
+        const int li = (u * NT + t) * 64 + l;
+        const T dp0 = lt.d0[li], dp1 = lt.d1[li], dp2 = lt.d2[li];
+        const T *md = pd + dr[t] * 9;
+        T av[3];
+        av[0] = md[0] * dp0 + md[1] * dp1 + md[2] * dp2;
+        av[1] = md[3] * dp0 + md[4] * dp1 + md[5] * dp2;
+        av[2] = md[6] * dp0 + md[7] * dp1 + md[8] * dp2;
+        bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
+        bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
+        bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int t2 = t; t2 < NT; t2++) {
+            const int p = t * NT - t * (t - 1) / 2 + (t2 - t);
+            acc[p] = Acc4<T>::mma(av[a], bv[t2][a], acc[p]);
+          }
+      }
+    }
+    store(Kout, acc);
+  }
+  {  // M = V^T P
+    A4 acc[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; p++) acc[p] = A4{0, 0, 0, 0};
+#pragma unroll 2
+    for (int u = 0; u < NU; u++) {
+      const int gg = 4 * u + q;
+      const T *pd = gk[gg < NG ? gg : 0];
+      T mw[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
+      T bv[NT][3];
+#pragma unroll
+      for (int t = NT - 1; t >= 0; t--) {
+        const T ph = lt.p[(u * NT + t) * 64 + l];
+        const T *vd = pd + kPtV + dr[t] * 3;
+        T av[3];
+        av[0] = ph * vd[0];
+        av[1] = ph * vd[1];
+        av[2] = ph * vd[2];
+        bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
+        bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
+        bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int t2 = t; t2 < NT; t2++) {
+            const int p = t * NT - t * (t - 1) / 2 + (t2 - t);
+            acc[p] = Acc4<T>::mma(av[a], bv[t2][a], acc[p]);
+          }
+      }
+    }
+    store(Mout, acc);
+  }
   }
 }
 
@@ -729,14 +947,19 @@ template <typename T, int NN, int NE, int NG, int GC>
 static int launch_km(mvf_handle *h, int64_t e0, int64_t n, int first_flags, T *dK, T *dM, int *bad) {
   int prev = (h->emulate & MVF_EMU_GPML_PREV_FLAGS) ? 1 : 0;
   if (!h->elem_legacy) {
-    // four elements per workgroup; at most 2^22 elements per launch
+    // four waves per workgroup, one element per wave (four for 8-node); at most 2^22 elements per launch
     const int64_t chunk = (int64_t)1 << 22;
     for (int64_t s = 0; s < n; s += chunk) {
       int64_t c = (n - s) < chunk ? (n - s) : chunk;
       constexpr int64_t NP = NE * (NE + 1) / 2;
-      hipLaunchKernelGGL((k_elem_km_mfma<T, NN, NE, NG>), dim3((unsigned)((c + 3) / 4)), dim3(256), 0, h->stream,
-                         h->d_tab, lane_tab<T>(h), h->d_xp, h->d_yp, h->d_zp, h->d_inmu, h->d_sig, h->geo, h->gp,
-                         prev, first_flags, e0 + s, c, dK + s * NP, dM + s * NP, bad);
+      if constexpr (NN == 8)
+        hipLaunchKernelGGL((k_elem_km_mfma_multi<T, NN, NE, NG, 4>), dim3((unsigned)((c + 15) / 16)), dim3(256), 0,
+                           h->stream, h->d_tab, lane_tab<T>(h), h->d_xp, h->d_yp, h->d_zp, h->d_inmu, h->d_sig, h->geo,
+                           h->gp, prev, first_flags, e0 + s, c, dK + s * NP, dM + s * NP, bad);
+      else
+        hipLaunchKernelGGL((k_elem_km_mfma<T, NN, NE, NG>), dim3((unsigned)((c + 3) / 4)), dim3(256), 0, h->stream,
+                           h->d_tab, lane_tab<T>(h), h->d_xp, h->d_yp, h->d_zp, h->d_inmu, h->d_sig, h->geo, h->gp,
+                           prev, first_flags, e0 + s, c, dK + s * NP, dM + s * NP, bad);
     }
     MVF_HIP(h, hipGetLastError());
     return 0;

@@ -83,6 +83,14 @@ struct LaneTab {
   const T *dN, *N, *w;
 };
 
+// Workgroups are dealt round-robin to the 8 XCDs.  xcd_block renumbers block
+// b of nb so that each XCD takes a contiguous range of logical blocks (a
+// bijection of [0, nb)): neighbouring work then meets in that XCD's L2.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t x = b & 7, i = b >> 3, q = nb >> 3, rm = nb & 7;
+  return x * q + (x < rm ? x : rm) + i;
+}
+
 }  // namespace mvf
 
 struct mvf_handle {
