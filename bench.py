@@ -20,10 +20,12 @@ The value is frequency-model solves per second.
 
 Bounded extras after the timed loop, each with its own limit: the FP32
 element kernel, the plain GPU Krylov path (Jacobi COCG, FP64, which does not
-reach 1e-8 on this GPML model), a reduced many-model batch (8 models x 4
-frequencies, batched against one model at a time) and the CPU baseline (the oracle's C
-restatement of the element loop on a sample of elements, one core and all
-cores; the CPU direct solve is infeasible at this size and reported as such).
+reach 1e-8 on this GPML model), the CPU baseline (the oracle's C restatement
+of the element loop on a sample of elements, one core and all cores; the CPU
+direct solve is infeasible at this size and reported as such), the small
+8-node case where the whole CPU path (element loop + SuperLU) is feasible, and
+a reduced many-model batch (8 models x 4 frequencies, batched against one
+model at a time).
 Progress goes to stderr after every step; stdout carries one JSON line.
 """
 from __future__ import annotations
@@ -47,7 +49,8 @@ FP32_PEAK_TFS = 157.3
 ELEM_FLOPS = {8: 21768, 20: 376731, 27: 744795}
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
 # by what the counter reads for a plain copy of known size (profiles/)
-TRAFFIC_MEASURED = {("halfspace8", "k_spmv"): 20.17e6, ("aniso20", "k_fa_gemm"): 3.43e9}
+TRAFFIC_MEASURED = {("halfspace8", "k_spmv"): 20.17e6, ("aniso20", "k_fa_gemm"): 3.43e9,
+                    ("aniso20", "k_spmv"): 7.45e9, ("aniso20", "k_elem_km"): 2.40e9}
 
 
 def log(msg):
@@ -315,7 +318,8 @@ def main():
            "elem_mats_per_s": m.ne / (e64 * 1e-3), "elem_kernel_ms": e64,
            "elem_roofline": {"bound": "mfma", "achieved": m.ne * F / (e64 * 1e-3) / 1e12,
                              "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                             "frac": m.ne * F / (e64 * 1e-3) / 1e12 / FP64_PEAK_TFS, "flops_per_elem": F}}
+                             "frac": m.ne * F / (e64 * 1e-3) / 1e12 / FP64_PEAK_TFS, "flops_per_elem": F,
+                             "traffic": TRAFFIC_MEASURED.get((args.workload, "k_elem_km"))}}
     log(f"timed {steps} steps in {dt:.1f}s: {value:.4f} solves/s; elem {out['elem_mats_per_s']:.3g}/s")
     # the extras below never take the line down: a failing leg records its error
     try:
@@ -334,6 +338,41 @@ def main():
         log(f"extras failed: {str(e)[:300]}")
     out["job_wall_s"] = time.perf_counter() - t_job
     print(json.dumps(out), flush=True)
+
+
+def small_leg(steps=20):
+    """The small plain-method case (8-node 20x20x(16+4) cells, 0.01 S/m half-space
+    with a 1 S/m block, 1 Hz, Dirichlet), where the CPU path is feasible: the
+    device step (re-integration, assembly, RHS, factorization, GMRES to 1e-8)
+    against the oracle's element loop plus SciPy SuperLU (the MUMPS stand-in)
+    on the host, whole model, one frequency."""
+    from movfem_amd import _lib as L
+    from movfem_amd import model as M
+    from movfem_amd.vfem import GpuVfem
+
+    t0 = time.perf_counter()
+    m = M.halfspace_block()
+    g = GpuVfem(m, emulate=L.MVF_EMU_GPML_PREV_FLAGS)
+    f = float(m.freqs[0])
+    try:
+        for _ in range(2):
+            g.assemble("fp64")
+            g.solve_frequencies([f], method="gmres", precond="direct", tol=1e-8)
+        ts = time.perf_counter()
+        res = []
+        for _ in range(steps):
+            g.assemble("fp64")
+            inf = g.solve_frequencies([f], method="gmres", precond="direct", tol=1e-8)
+            res.append(max(inf["true_resid"][:2]))
+        gpu = steps / (time.perf_counter() - ts)
+        nne = g.nne
+    finally:
+        g.close()
+    cb = cpu_baseline(m, nne, budget_s=12.0)
+    return {"workload": "8-node 20x20x(16+4) cells, half-space with a block, 1 Hz, Dirichlet", "nne": nne,
+            "gpu_solves_per_s": gpu, "gpu_worst_resid": max(res), "cpu_solves_per_s": cb["value"],
+            "cpu_cores": cb["cores"], "cpu_sample": cb["sample"], "speedup_vs_cpu": gpu / cb["value"],
+            "wall_s": time.perf_counter() - t0}
 
 
 def extras(args, g, m, freqs, value, out):
@@ -367,6 +406,12 @@ def extras(args, g, m, freqs, value, out):
         out["speedup_vs_cpu"] = value / cb["value"]
         out["elem_speedup_vs_cpu_all_cores"] = out["elem_mats_per_s"] / cb["elem_mats_per_s_all_cores"]
         log("CPU baseline leg done")
+    if not args.no_cpu:
+        try:
+            out["small_case"] = small_leg()
+        except Exception as e:  # noqa: BLE001
+            out["small_case"] = {"error": str(e)[:300]}
+        log("small-case leg done")
     if not args.no_batch:
         # the main workload's factor (86 GB) and pools go first: the batch
         # factors its models concurrently
