@@ -162,7 +162,9 @@ typedef struct mvf_solver_opts {
   int32_t check_every;  /* iterations between convergence checks (host sync) */
   double inner_tol;     /* MIXED: FP32 residual drop before an FP64 refinement step */
   int32_t use_graph;    /* COCG/BiCGStab: capture check_every iterations once as a HIP graph and replay it
-                           between convergence checks (ignored by mvf_solve_noalloc and the direct path) */
+                           between convergence checks; MVF_PC_DIRECT: replay each application of the factor
+                           (its triangular sweeps) from a graph captured once per column count.  Ignored by
+                           mvf_solve_noalloc, and by mvf_solve_batch with more than one thread */
   int32_t gmres_m;      /* GMRES restart length (MVF_PC_DIRECT), 0 = 30 */
   int32_t freqs_per_factor; /* MVF_PC_DIRECT in mvf_solve_frequencies: consecutive frequencies sharing one
                                factorization (at their geometric-mean frequency), 0 or 1 = one each */

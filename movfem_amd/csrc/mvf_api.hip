@@ -701,6 +701,17 @@ int mvf_solve_batch(mvf_handle *const *hs, int32_t nh, const double *freqs_hz, i
   for (int32_t i = 0; i < nh; i++)
     if (!hs[i]) return MVF_E_ARG;
   const int nt = std::max(1, std::min<int>(nthreads > 0 ? nthreads : 8, nh));
+  // several host threads: no stream capture, since another thread's copy on
+  // the null stream synchronizes with every blocking stream and would break a
+  // capture in progress on one of them
+  mvf_solver_opts od;
+  if (o) {
+    od = *o;
+  } else {
+    mvf_default_solver_opts(&od);
+  }
+  if (nt > 1) od.use_graph = 0;
+  o = &od;
   std::atomic<int> next{0};
   std::vector<int> rcs(nh, 0);
   auto worker = [&]() {

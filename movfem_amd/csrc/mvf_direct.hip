@@ -66,6 +66,8 @@ struct DNode {
   int32_t depth;
 };
 
+constexpr int kMaxColPairs = 8;  // right-hand-side columns per factor application: 2 .. 16
+
 struct DirectPlan {
   int T = 0;
   int maxdepth = 0;
@@ -90,6 +92,12 @@ struct DirectPlan {
   int *d_npert = nullptr;
   double2 *d_sv = nullptr, *d_zb = nullptr;
   int sv_cols = 0;
+  // the sweeps of one factor application captured as a HIP graph per column
+  // count (index NC / 2), reading d_gin and writing d_gout: the sweeps are
+  // thousands of short launches whose gaps a replay removes
+  hipGraphExec_t sv_exec[kMaxColPairs + 1] = {};
+  double2 *d_gin = nullptr, *d_gout = nullptr;
+  int g_cols = 0;
   // side stream (default priority) for the trailing part of the outer-block
   // updates, with the events that order it against the handle's stream
   hipStream_t st2 = nullptr;
@@ -1520,6 +1528,10 @@ void direct_free(mvf_handle *h) {
   dfree(P->d_piv); dfree(P->d_perm); dfree(P->d_pos); dfree(P->d_fac); dfree(P->d_inv); dfree(P->d_npert);
   dfree(P->d_sv);
   dfree(P->d_zb);
+  dfree(P->d_gin);
+  dfree(P->d_gout);
+  for (auto &e : P->sv_exec)
+    if (e) (void)hipGraphExecDestroy(e);
   dfree(P->d_arena);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
@@ -1857,19 +1869,36 @@ static int direct_factor(mvf_handle *h, double w, int c64, int f32 = 0) {
   return 0;
 }
 
-// out = A_pc^-1 in, NC columns ([nne][NC] complex, device)
+static void drop_sv_graphs(DirectPlan *P) {
+  for (auto &e : P->sv_exec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+}
+
+// the solve phase's work buffers for NC columns (a graph captured for fewer
+// columns would point at the freed ones)
 template <int NC>
-static int direct_apply(mvf_handle *h, const double2 *in, double2 *out) {
+static int sv_ensure(mvf_handle *h, DirectPlan *P) {
+  if (P->sv_cols >= NC) return 0;
+  drop_sv_graphs(P);
+  dfree(P->d_sv);
+  dfree(P->d_zb);
+  P->d_sv = P->d_zb = nullptr;
+  P->sv_cols = 0;
+  MVF_TRY(dalloc(h, &P->d_sv, P->sv_rows * NC));
+  MVF_TRY(dalloc(h, &P->d_zb, P->nne * NC));
+  P->sv_cols = NC;
+  return 0;
+}
+
+// out = A_pc^-1 in, NC columns ([nne][NC] complex, device): the launches
+template <int NC>
+static int direct_sweeps(mvf_handle *h, const double2 *in, double2 *out) {
   DirectPlan *P = (DirectPlan *)h->direct;
   hipStream_t st = h->stream;
-  if (P->sv_cols < NC) {
-    dfree(P->d_sv);
-    dfree(P->d_zb);
-    P->d_sv = P->d_zb = nullptr;
-    MVF_TRY(dalloc(h, &P->d_sv, P->sv_rows * NC));
-    MVF_TRY(dalloc(h, &P->d_zb, P->nne * NC));
-    P->sv_cols = NC;
-  }
+  MVF_TRY(sv_ensure<NC>(h, P));
   double2 *W = P->d_sv, *zb = P->d_zb;
   for (int d = P->maxdepth; d >= 0; d--) {
     const int nl = (int)P->levels[d].size();
@@ -1914,6 +1943,51 @@ static int direct_apply(mvf_handle *h, const double2 *in, double2 *out) {
   hipLaunchKernelGGL((k_sv_out<NC>), dim3(cap_grid((P->nne * NC + 255) / 256)), dim3(256), 0, st, P->nne, P->d_pos, zb,
                      out);
   MVF_HIP(h, hipGetLastError());
+  return 0;
+}
+
+// out = A_pc^-1 in; with graph, the sweeps replay from a HIP graph captured on
+// the first application for this column count (the factor's buffers keep
+// their addresses across refactorizations, so the graph stays valid), with
+// the input and output copied through fixed buffers
+template <int NC>
+static int direct_apply(mvf_handle *h, const double2 *in, double2 *out, bool graph) {
+  static_assert(NC % 2 == 0 && NC / 2 <= kMaxColPairs, "column count");
+  if (!graph) return direct_sweeps<NC>(h, in, out);
+  DirectPlan *P = (DirectPlan *)h->direct;
+  hipStream_t st = h->stream;
+  MVF_TRY(sv_ensure<NC>(h, P));
+  if (P->g_cols < NC) {
+    drop_sv_graphs(P);
+    dfree(P->d_gin);
+    dfree(P->d_gout);
+    P->d_gin = P->d_gout = nullptr;
+    P->g_cols = 0;
+    MVF_TRY(dalloc(h, &P->d_gin, P->nne * NC));
+    MVF_TRY(dalloc(h, &P->d_gout, P->nne * NC));
+    P->g_cols = NC;
+  }
+  hipGraphExec_t &ex = P->sv_exec[NC / 2];
+  if (!ex) {
+    MVF_HIP(h, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = direct_sweeps<NC>(h, P->d_gin, P->d_gout);
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &gr);
+    if (rc || ec != hipSuccess || !gr) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc ? rc : set_err(h, MVF_E_DEVICE, "capturing the solve sweeps failed: %s", hipGetErrorString(ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess) {
+      ex = nullptr;
+      return set_err(h, MVF_E_DEVICE, "instantiating the solve sweeps failed: %s", hipGetErrorString(ei));
+    }
+  }
+  const size_t bytes = sizeof(double2) * (size_t)P->nne * NC;
+  MVF_HIP(h, hipMemcpyAsync(P->d_gin, in, bytes, hipMemcpyDeviceToDevice, st));
+  MVF_HIP(h, hipGraphLaunch(ex, st));
+  MVF_HIP(h, hipMemcpyAsync(out, P->d_gout, bytes, hipMemcpyDeviceToDevice, st));
   return 0;
 }
 
@@ -2061,7 +2135,7 @@ static int run_gmres_direct(Solver &S, const mvf_solver_opts *o, const double *b
     for (int j = 0; j < mres && it < o->maxit; j++) {
       double ta = now_s();
       double2 *zj = Z + (size_t)j * nv;
-      MVF_TRY(direct_apply<NC>(h, V + (size_t)j * nv, zj));
+      MVF_TRY(direct_apply<NC>(h, V + (size_t)j * nv, zj, o->use_graph != 0));
       spmv_launch<double, double, NC, false>(S, h->d_K, h->d_M, (const double *)zj, (double *)wv, nullptr, nullptr);
       std::vector<cplx> hv;
       for (int pass = 0; pass < 2; pass++) {
