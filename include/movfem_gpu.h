@@ -161,8 +161,10 @@ typedef struct mvf_solver_opts {
   int32_t maxit;        /* total Krylov iterations */
   int32_t check_every;  /* iterations between convergence checks (host sync) */
   double inner_tol;     /* MIXED: FP32 residual drop before an FP64 refinement step */
-  int32_t use_graph;    /* COCG/BiCGStab: capture check_every iterations once as a HIP graph and replay it
-                           between convergence checks; MVF_PC_DIRECT: replay each application of the factor
+  int32_t use_graph;    /* COCG/BiCGStab: 1 = capture check_every iterations once as a HIP graph and replay
+                           it between convergence checks; COCG: 2 = run each check_every iterations in one
+                           cooperative launch of a persistent kernel (grid barriers between the steps);
+                           MVF_PC_DIRECT: nonzero = replay each application of the factor
                            (its triangular sweeps) from a graph captured once per column count.  Ignored by
                            mvf_solve_noalloc, and by mvf_solve_batch with more than one thread */
   int32_t gmres_m;      /* GMRES restart length (MVF_PC_DIRECT), 0 = 30 */

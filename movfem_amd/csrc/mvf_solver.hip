@@ -16,6 +16,7 @@
 // the FP64 true residual b - A x (FP64 operator, FP64 x) whenever it has
 // dropped by inner_tol since the last replacement ("reliable updates"); all
 // dot products accumulate in FP64.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -55,10 +56,10 @@ __device__ inline double ld(const T *p, int64_t i) {
 // y = A x (columns 0..NC-1); LPR lanes share a row.  With DOT, block partials
 // of p^T y (bilinear) are written to part[block][c].
 template <typename VT, typename MT, int NC, int LPR, bool C64, bool DOT>
-__global__ __launch_bounds__(kBS) void k_spmv(int64_t nne, const int64_t *__restrict__ rowptr,
-                                              const int32_t *__restrict__ col, const MT *__restrict__ K,
-                                              const MT *__restrict__ M, ColW w, const VT *__restrict__ x,
-                                              VT *__restrict__ y, const VT *__restrict__ p, double *__restrict__ part) {
+__device__ __forceinline__ void spmv_body(int64_t nne, const int64_t *__restrict__ rowptr,
+                                          const int32_t *__restrict__ col, const MT *__restrict__ K,
+                                          const MT *__restrict__ M, const ColW &w, const VT *__restrict__ x,
+                                          VT *__restrict__ y, const VT *__restrict__ p, double *__restrict__ part) {
   const int lane = threadIdx.x % LPR;
   const int64_t rstride = (int64_t)gridDim.x * (kBS / LPR);
   double dr[NC], di[NC];
@@ -141,6 +142,14 @@ __global__ __launch_bounds__(kBS) void k_spmv(int64_t nne, const int64_t *__rest
       part[((int64_t)blockIdx.x * NC + c) * 2 + q] = s;
     }
   }
+}
+
+template <typename VT, typename MT, int NC, int LPR, bool C64, bool DOT>
+__global__ __launch_bounds__(kBS) void k_spmv(int64_t nne, const int64_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ col, const MT *__restrict__ K,
+                                              const MT *__restrict__ M, ColW w, const VT *__restrict__ x,
+                                              VT *__restrict__ y, const VT *__restrict__ p, double *__restrict__ part) {
+  spmv_body<VT, MT, NC, LPR, C64, DOT>(nne, rowptr, col, K, M, w, x, y, p, part);
 }
 
 // ------------------------------------------------------- block reduction ----
@@ -258,24 +267,24 @@ __global__ void k_cocg_fin_init(const double *part, int nb, ColState *st) {
 }
 
 // alpha = rho / (p^T q); a zero denominator is a breakdown of that column
+__device__ inline void cocg_alpha_step(ColState &s, double pr, double pi) {
+  if (!s.active) {
+    s.alpha_r = s.alpha_i = 0.0;
+  } else if (pr == 0.0 && pi == 0.0) {
+    s.alpha_r = s.alpha_i = 0.0;
+    s.breakdown = 1;
+    s.active = 0;
+  } else {
+    cdiv(s.rho_r, s.rho_i, pr, pi, s.alpha_r, s.alpha_i);
+  }
+}
+
 template <int NC>
 __global__ void k_cocg_alpha(const double *part, int nb, ColState *st) {
   __shared__ double out[NC * 2];
   reduce_partials<NC, 1>(part, nb, out);
   __syncthreads();
-  if (threadIdx.x < NC) {
-    ColState &s = st[threadIdx.x];
-    double pr = out[2 * threadIdx.x], pi = out[2 * threadIdx.x + 1];
-    if (!s.active) {
-      s.alpha_r = s.alpha_i = 0.0;
-    } else if (pr == 0.0 && pi == 0.0) {
-      s.alpha_r = s.alpha_i = 0.0;
-      s.breakdown = 1;
-      s.active = 0;
-    } else {
-      cdiv(s.rho_r, s.rho_i, pr, pi, s.alpha_r, s.alpha_i);
-    }
-  }
+  if (threadIdx.x < NC) cocg_alpha_step(st[threadIdx.x], out[2 * threadIdx.x], out[2 * threadIdx.x + 1]);
 }
 
 // x += alpha p (x in XT precision), r -= alpha q; partials rho' = r^T D^-1 r, |r|^2
@@ -318,30 +327,34 @@ __global__ __launch_bounds__(kBS) void k_cocg_upd1(int64_t nne, const ColState *
 }
 
 // beta = rho'/rho, rho = rho'; convergence of the recursive residual freezes a column
+__device__ inline void cocg_beta_step(ColState &s, double nr, double ni, double rn2, double tol_abs_scale,
+                                      int freeze) {
+  s.rnorm = sqrt(rn2);
+  if (!s.active) {
+    s.beta_r = s.beta_i = 0.0;
+    return;
+  }
+  if (s.rho_r == 0.0 && s.rho_i == 0.0) {
+    s.breakdown = 1;
+    s.active = 0;
+    s.beta_r = s.beta_i = 0.0;
+    return;
+  }
+  cdiv(nr, ni, s.rho_r, s.rho_i, s.beta_r, s.beta_i);
+  s.rho_r = nr;
+  s.rho_i = ni;
+  if (freeze && s.rnorm <= tol_abs_scale * s.bnorm) s.active = 0;
+}
+
 template <int NC>
 __global__ void k_cocg_beta(const double *part, int nb, ColState *st, double tol_abs_scale, int freeze) {
   __shared__ double out[2 * NC * 2];
   reduce_partials<NC, 2>(part, nb, out);
   __syncthreads();
   if (threadIdx.x < NC) {
-    int c = threadIdx.x;
-    ColState &s = st[c];
-    double nr = out[(0 * NC + c) * 2], ni = out[(0 * NC + c) * 2 + 1];
-    s.rnorm = sqrt(out[(1 * NC + c) * 2]);
-    if (!s.active) {
-      s.beta_r = s.beta_i = 0.0;
-      return;
-    }
-    if (s.rho_r == 0.0 && s.rho_i == 0.0) {
-      s.breakdown = 1;
-      s.active = 0;
-      s.beta_r = s.beta_i = 0.0;
-      return;
-    }
-    cdiv(nr, ni, s.rho_r, s.rho_i, s.beta_r, s.beta_i);
-    s.rho_r = nr;
-    s.rho_i = ni;
-    if (freeze && s.rnorm <= tol_abs_scale * s.bnorm) s.active = 0;
+    const int c = threadIdx.x;
+    cocg_beta_step(st[c], out[(0 * NC + c) * 2], out[(0 * NC + c) * 2 + 1], out[(1 * NC + c) * 2], tol_abs_scale,
+                   freeze);
   }
 }
 
@@ -365,6 +378,81 @@ __global__ __launch_bounds__(kBS) void k_cocg_upd2(int64_t nne, const ColState *
     p[2 * i] = (VT)(zr + br * pr - bi * pi);
     p[2 * i + 1] = (VT)(zi + br * pi + bi * pr);
   }
+}
+
+// ------------------------------------------------------- persistent COCG ----
+// `iters` COCG iterations in one cooperative launch: the SpMV with its fused
+// p^T A p partials, alpha, the x/r update with its partials, beta and the p
+// update, separated by three grid barriers per iteration.  Every block reduces
+// the partials itself (the same order as the one-block reduction kernels) and
+// keeps its own copy of the column state, so no further barrier is needed; block 0
+// writes the state back at the end.  Every block runs every iteration: inactive
+// columns only carry zero alpha and beta.
+template <int NC, int LPR, bool C64>
+__global__ __launch_bounds__(kBS) void k_cocg_persist(int iters, int64_t nne, const int64_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col, const double *__restrict__ K,
+                                                      const double *__restrict__ M, ColW w, double *__restrict__ x,
+                                                      double *__restrict__ r, double *__restrict__ p,
+                                                      double *__restrict__ q, const double *__restrict__ invd,
+                                                      double *__restrict__ spart, double *__restrict__ part,
+                                                      ColState *__restrict__ st, double tol) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  __shared__ ColState cs[NC];
+  __shared__ double out1[NC * 2], out2[2 * NC * 2];
+  if (threadIdx.x < NC) cs[threadIdx.x] = st[threadIdx.x];
+  __syncthreads();
+  const int nb = gridDim.x;
+  for (int k = 0; k < iters; k++) {
+    spmv_body<double, double, NC, LPR, C64, true>(nne, rowptr, col, K, M, w, p, q, p, spart);
+    grid.sync();
+    reduce_partials<NC, 1>(spart, nb, out1);
+    __syncthreads();
+    if (threadIdx.x < NC) cocg_alpha_step(cs[threadIdx.x], out1[2 * threadIdx.x], out1[2 * threadIdx.x + 1]);
+    __syncthreads();
+    {
+      double v[2][NC][2];
+      for (int a = 0; a < 2; a++)
+        for (int c = 0; c < NC; c++) v[a][c][0] = v[a][c][1] = 0.0;
+      for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < nne * NC; i += (int64_t)gridDim.x * kBS) {
+        const int c = (int)(i % NC);
+        const double alr = cs[c].alpha_r, ali = cs[c].alpha_i;
+        const double pr = p[2 * i], pi = p[2 * i + 1], qr = q[2 * i], qi = q[2 * i + 1];
+        x[2 * i] = x[2 * i] + alr * pr - ali * pi;
+        x[2 * i + 1] = x[2 * i + 1] + alr * pi + ali * pr;
+        const double rr = r[2 * i] - (alr * qr - ali * qi), ri = r[2 * i + 1] - (alr * qi + ali * qr);
+        r[2 * i] = rr;
+        r[2 * i + 1] = ri;
+        const double dr = invd[2 * i], di = invd[2 * i + 1];
+        const double zr = dr * rr - di * ri, zi = dr * ri + di * rr;
+#pragma unroll
+        for (int cc = 0; cc < NC; cc++)
+          if (cc == c) {
+            v[0][cc][0] += rr * zr - ri * zi;
+            v[0][cc][1] += rr * zi + ri * zr;
+            v[1][cc][0] += rr * rr + ri * ri;
+          }
+      }
+      block_partials<NC, 2>(v, part);
+    }
+    grid.sync();
+    reduce_partials<NC, 2>(part, nb, out2);
+    __syncthreads();
+    if (threadIdx.x < NC) {
+      const int c = threadIdx.x;
+      cocg_beta_step(cs[c], out2[(0 * NC + c) * 2], out2[(0 * NC + c) * 2 + 1], out2[(1 * NC + c) * 2], tol, 1);
+    }
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < nne * NC; i += (int64_t)gridDim.x * kBS) {
+      const int c = (int)(i % NC);
+      const double rr = r[2 * i], ri = r[2 * i + 1], dr = invd[2 * i], di = invd[2 * i + 1];
+      const double zr = dr * rr - di * ri, zi = dr * ri + di * rr;
+      const double pr = p[2 * i], pi = p[2 * i + 1], br = cs[c].beta_r, bi = cs[c].beta_i;
+      p[2 * i] = zr + br * pr - bi * pi;
+      p[2 * i + 1] = zi + br * pi + bi * pr;
+    }
+    grid.sync();
+  }
+  if (blockIdx.x == 0 && threadIdx.x < NC) st[threadIdx.x] = cs[threadIdx.x];
 }
 
 // --------------------------------------------------------------- helpers ----
@@ -873,6 +961,68 @@ static int cocg_bufs(Workspace &W, mvf_handle *h, int64_t n, CocgBufs<NC> &B) {
   return 0;
 }
 
+// the persistent COCG kernel for this solver's lane width and matrix: grid of
+// co-resident blocks (cooperative launch requires it), at most as many as the
+// partial-sum buffers hold
+template <int NC, int LPR, bool C64>
+static int cocg_persist_grid(mvf_handle *h, int64_t n, int &grid) {
+  int per = 0, ncu = 0;
+  MVF_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_cocg_persist<NC, LPR, C64>, kBS, 0));
+  MVF_HIP(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+  int64_t g = (int64_t)per * ncu;
+  // MVF_PERSIST_BLOCKS caps the grid: fewer blocks make the barriers and the
+  // per-block reductions cheaper and leave fewer waves for the SpMV
+  if (const char *e = getenv("MVF_PERSIST_BLOCKS")) {
+    const long v = strtol(e, nullptr, 10);
+    if (v > 0) g = std::min<int64_t>(g, v);
+  }
+  g = std::min<int64_t>(g, kRedBlocks);
+  g = std::min<int64_t>(g, spmv_blocks(n));
+  if (g < 1) return set_err(h, MVF_E_DEVICE, "persistent COCG: no co-resident blocks");
+  grid = (int)g;
+  return 0;
+}
+
+template <int NC, int LPR, bool C64>
+static int cocg_persist_run(Solver &S, int grid, int iters, double *x, double *r, double *p, double *q,
+                            const double *invd, double *spart, double *part, ColState *st, double tol) {
+  mvf_handle *h = S.h;
+  int64_t n = S.n;
+  const int64_t *rowptr = h->d_rowptr;
+  const int32_t *col = h->d_col;
+  const double *K = h->d_K, *M = h->d_M;
+  ColW w = S.w;
+  void *args[] = {&iters, &n, &rowptr, &col, &K, &M, &w, &x, &r, &p, &q, &invd, &spart, &part, &st, &tol};
+  MVF_HIP(h, hipLaunchCooperativeKernel((const void *)k_cocg_persist<NC, LPR, C64>, dim3(grid), dim3(kBS), args, 0,
+                                        h->stream));
+  return 0;
+}
+
+// dispatch over the autotuned lane width and the matrix rounding
+template <int NC>
+static int cocg_persist(Solver &S, bool grid_only, int &grid, int iters, double *x, double *r, double *p, double *q,
+                        const double *invd, double *spart, double *part, ColState *st, double tol) {
+#define MVF_PERSIST(LPR_, C64_)                                                                                    \
+  return grid_only ? cocg_persist_grid<NC, LPR_, C64_>(S.h, S.n, grid)                                            \
+                   : cocg_persist_run<NC, LPR_, C64_>(S, grid, iters, x, r, p, q, invd, spart, part, st, tol)
+  if (S.c64) {
+    switch (S.lpr) {
+      case 4: MVF_PERSIST(4, true);
+      case 8: MVF_PERSIST(8, true);
+      case 32: MVF_PERSIST(32, true);
+      default: MVF_PERSIST(16, true);
+    }
+  } else {
+    switch (S.lpr) {
+      case 4: MVF_PERSIST(4, false);
+      case 8: MVF_PERSIST(8, false);
+      case 32: MVF_PERSIST(32, false);
+      default: MVF_PERSIST(16, false);
+    }
+  }
+#undef MVF_PERSIST
+}
+
 template <int NC>
 static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, double *x, mvf_solve_info *info) {
   mvf_handle *h = S.h;
@@ -882,11 +1032,16 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
   W.cap = S.ws_cap;
   CocgBufs<NC> B;
   MVF_TRY(cocg_bufs<NC>(W, h, n, B));
-  const bool use_graph = o->use_graph && !S.ws && o->check_every > 0;
+  // use_graph 1: replay check_every iterations from a HIP graph; 2: run them in
+  // one cooperative launch of the persistent kernel
+  const bool persist = o->use_graph == 2 && !S.ws && o->check_every > 0;
+  const bool use_graph = o->use_graph == 1 && !S.ws && o->check_every > 0;
   GraphBlock gb;
   double *r = B.r, *p = B.p, *q = B.q, *invd = B.invd, *part = B.part, *dnorm = B.dnorm;
   ColState *st = B.st;
   double *spart = part + (size_t)kRedBlocks * 3 * NC * 2;
+  int pgrid = 0;
+  if (persist) MVF_TRY(cocg_persist<NC>(S, true, pgrid, 0, x, r, p, q, invd, spart, part, st, o->tol));
   if (o->precond == MVF_PC_JACOBI) {
     if (S.c64)
       hipLaunchKernelGGL((k_invdiag<NC, true>), dim3(grid1d(n, 256)), dim3(256), 0, h->stream, n, h->d_diagpos, h->d_K,
@@ -918,7 +1073,13 @@ static int run_cocg64(Solver &S, const mvf_solver_opts *o, const double *b, doub
       hipLaunchKernelGGL((k_cocg_upd2<double, NC>), dim3(nb), dim3(kBS), 0, h->stream, n, st, r, p, invd);
     };
     while (it < o->maxit) {
-      if (use_graph && since == 0 && it + o->check_every <= o->maxit) {
+      if (persist && since == 0) {
+        const int k = std::min(o->check_every, o->maxit - it);
+        MVF_TRY(cocg_persist<NC>(S, false, pgrid, k, x, r, p, q, invd, spart, part, st, o->tol));
+        gb.launches++;
+        it += k;
+        since = o->check_every;
+      } else if (use_graph && since == 0 && it + o->check_every <= o->maxit) {
         MVF_TRY(graph_block(S, gb, o->check_every, iter));
         it += o->check_every;
         since = o->check_every;

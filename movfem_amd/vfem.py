@@ -48,7 +48,7 @@ def solver_opts(method="cocg", precision="fp64", matrix="c64ref", precond="jacob
     o.gmres_m = gmres_m
     o.freqs_per_factor = freqs_per_factor
     o.direct_tile = direct_tile
-    o.use_graph = int(bool(use_graph))
+    o.use_graph = 2 if use_graph == "persistent" else int(bool(use_graph))
     o.tol = tol
     o.maxit = maxit
     o.check_every = check_every

@@ -90,3 +90,24 @@ def test_direct_sweeps_graph_matches_eager(need_gpu):
         print(f"f={f:g} Hz: {ie['iterations']} GMRES iterations eager, {ig['iterations']} with graphs", flush=True)
         assert ig["iterations"] == ie["iterations"] and np.array_equal(xg, xe)
     g.close()
+
+
+def test_persistent_cocg(need_gpu):
+    """COCG with check_every iterations per cooperative launch of the persistent
+    kernel (three grid barriers per iteration) against the per-kernel
+    iteration: both reach the FP64 true residual bar and agree to 1e-8; the
+    block partial sums differ in grouping, so the iterates are not identical."""
+    for m in (M.random_small(8, 5, 5, 4, seed=41, nz_air=1, freqs=(1.0,)),
+              M.anisotropic_blocks(nx=16, ny=16, nz_earth=10, nz_air=2, nodes=8, freqs=(10.0,), block=4, seed=44,
+                                   full_tensor=True, dirichlet=True, nextd=2)):
+        g = GpuVfem(m)
+        g.set_frequency(0)
+        _, _, _, b = g.global_vfem()
+        xe, ie = g.solve(b, method="cocg", tol=1e-10, check_every=25, maxit=20000)
+        xp, ip = g.solve(b, method="cocg", tol=1e-10, check_every=25, maxit=20000, use_graph="persistent")
+        d = np.linalg.norm(xp - xe) / np.linalg.norm(xe)
+        print(f"nne={g.nne}: {ie['iterations']} iterations per kernel, {ip['iterations']} persistent "
+              f"({ip['graph_launches']} launches); difference {d:.1e}", flush=True)
+        assert ip["converged"] and max(ip["true_resid"][:2]) <= 1e-10 and ip["graph_launches"] > 0
+        assert d <= 1e-8
+        g.close()
