@@ -442,33 +442,42 @@ __global__ __launch_bounds__(256) void k_row_pattern(const int32_t *gne, int me,
                                                      unsigned long long *ntriu, const int64_t *rowptr, int32_t *col,
                                                      int *bad) {
   __shared__ int32_t buf[4][NP];
+  __shared__ unsigned long long up4[4];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * 4 + wv;
-  if (r >= nne) return;
   int32_t *v = buf[wv];
-  if (row_candidates<NP>(gne, me, rseg, rcontrib, r, lane, v) < 0) {
-    if (lane == 0) atomicExch(bad, 1);
-    return;
-  }
-  const int64_t base = PASS ? rowptr[r] : 0;
-  int count = 0, upper = 0;
-#pragma unroll
-  for (int c0 = 0; c0 < NP; c0 += 64) {
-    const int i = c0 + lane;
-    const int32_t x = v[i];
-    const bool head = x != INT32_MAX && (i == 0 || v[i - 1] != x);
-    const unsigned long long m = __ballot(head);
-    if (PASS) {
-      const int rank = count + (int)__popcll(m & ((1ull << lane) - 1ull));
-      if (head) col[base + rank] = x;
-    } else {
-      upper += (int)__popcll(__ballot(head && x >= r));
+  // each wave walks rows with a grid stride and sums its upper-triangle count;
+  // one atomic per workgroup (an atomic per row on one counter serialized the
+  // count pass of a 38M-row model)
+  unsigned long long upper_all = 0;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wv; r < nne; r += (int64_t)gridDim.x * 4) {
+    if (row_candidates<NP>(gne, me, rseg, rcontrib, r, lane, v) < 0) {
+      if (lane == 0) atomicExch(bad, 1);
+      continue;
     }
-    count += (int)__popcll(m);
+    const int64_t base = PASS ? rowptr[r] : 0;
+    int count = 0, upper = 0;
+#pragma unroll
+    for (int c0 = 0; c0 < NP; c0 += 64) {
+      const int i = c0 + lane;
+      const int32_t x = v[i];
+      const bool head = x != INT32_MAX && (i == 0 || v[i - 1] != x);
+      const unsigned long long m = __ballot(head);
+      if (PASS) {
+        const int rank = count + (int)__popcll(m & ((1ull << lane) - 1ull));
+        if (head) col[base + rank] = x;
+      } else {
+        upper += (int)__popcll(__ballot(head && x >= r));
+      }
+      count += (int)__popcll(m);
+    }
+    if (!PASS && lane == 0) cnt[r] = count;
+    upper_all += (unsigned long long)upper;
+    __builtin_amdgcn_wave_barrier();  // the next row's candidates overwrite v
   }
-  if (!PASS && lane == 0) {
-    cnt[r] = count;
-    atomicAdd(ntriu, (unsigned long long)upper);
+  if (!PASS) {
+    if (lane == 0) up4[wv] = upper_all;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(ntriu, up4[0] + up4[1] + up4[2] + up4[3]);
   }
 }
 
@@ -574,7 +583,8 @@ static int setup_pattern_rows(mvf_handle *h) {
   MVF_HIP(h, hipMemsetAsync(ntriu, 0, 8, h->stream));
   MVF_HIP(h, hipMemsetAsync(bad, 0, 4, h->stream));
   MVF_HIP(h, hipMemsetAsync(cnt + nne, 0, 8, h->stream));
-  const unsigned grid = (unsigned)((nne + 3) / 4);
+  // four rows per workgroup and pass, at most 32768 workgroups (grid stride)
+  const unsigned grid = (unsigned)std::min<int64_t>((nne + 3) / 4, 32768);
   if (np_for(me) == 64)
     hipLaunchKernelGGL((k_row_pattern<64, 0>), dim3(grid), dim3(256), 0, h->stream, h->d_gne, me, h->d_rseg,
                        h->d_rcontrib, nne, cnt, ntriu, nullptr, nullptr, bad);

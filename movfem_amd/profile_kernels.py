@@ -7,7 +7,8 @@
 
 elem: mvf_assemble (element integration and assembly of K and M) repeated on
   the anisotropic GPML workload of the given element type (20-node: 64 x 64 x 48
-  cells; 8-node: the same grid; 27-node: 32 x 32 x 24 cells).
+  cells; 8-node: the same grid; 27-node: 32 x 32 x 24 cells; with --full the
+  27-node 128 x 128 x 96 topography model, 38M DOFs).
 spmv: Jacobi COCG on the same model at 1 Hz for a fixed iteration count (the
   operator apply of every Krylov path, two columns); the solve is stopped by its
   iteration cap, not converged.
@@ -25,9 +26,12 @@ import json
 import numpy as np
 
 
-def model(nodes: int):
+def model(nodes: int, full: bool = False):
     from . import model as M
 
+    if full:
+        # the 27-node workload at full size: 128 x 128 x 96 cells under Gaussian hills, full-tensor sigma
+        return M.hills_tensor()
     if nodes == 27:
         return M.anisotropic_blocks(nx=32, ny=32, nz_earth=20, nz_air=4, h=500.0, nodes=27, freqs=(1.0,), block=4,
                                     seed=2, nextd=4)
@@ -44,6 +48,7 @@ def main():
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--full", action="store_true", help="elem: the 27-node 128 x 128 x 96 topography model")
     a = ap.parse_args()
     out = {"what": a.what}
     if a.what == "copy":
@@ -72,7 +77,7 @@ def main():
     else:
         from .vfem import GpuVfem, MvfError
 
-        m = model(a.nodes)
+        m = model(a.nodes, a.full)
         g = GpuVfem(m)
         out.update(nodes=a.nodes, elements=m.ne, nne=g.nne, nnz=g.nnze)
         if a.what == "elem":
