@@ -188,6 +188,31 @@ module movfem_gpu
       type(mvf_solve_info), intent(out) :: info
     end function mvf_solve_noalloc
 
+    ! mvf_solve started from x0 (the least-squares-scaled initial guess);
+    ! alpha(4): the scaling per polarization (re, im)
+    integer(c_int) function mvf_solve_guess(h, x0, rhs, o, info, alpha) bind(C, name='mvf_solve_guess')
+      import :: c_int, c_ptr, c_double, c_double_complex, mvf_solver_opts, mvf_solve_info
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: x0(*)
+      complex(c_double_complex), intent(inout) :: rhs(*)
+      type(mvf_solver_opts), intent(in) :: o
+      type(mvf_solve_info), intent(out) :: info
+      real(c_double), intent(out) :: alpha(4)
+    end function mvf_solve_guess
+
+    ! several models x frequencies: hs(nh) handles, freqs(nfreq, nh) in Hz (one
+    ! column per model), infos(nh), status(nh); nthreads host threads
+    integer(c_int) function mvf_solve_batch(hs, nh, freqs, nfreq, o, infos, status, nthreads) &
+         bind(C, name='mvf_solve_batch')
+      import :: c_int, c_ptr, c_int32_t, c_double, mvf_solver_opts, mvf_solve_info
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int32_t), value :: nh, nfreq, nthreads
+      real(c_double), intent(in) :: freqs(*)
+      type(mvf_solver_opts), intent(in) :: o
+      type(mvf_solve_info), intent(out) :: infos(*)
+      integer(c_int32_t), intent(out) :: status(*)
+    end function mvf_solve_batch
+
     ! node_solution + rx_data + z_rho_phi on the device; pass c_null_ptr for
     ! outputs not wanted and for x to read frequency f of the last batch
     integer(c_int) function mvf_readout(h, x, f, ke_top, nrx, rx, ry, esol, hsol, rx_e, rx_h, rx_z, rho, phi, &
