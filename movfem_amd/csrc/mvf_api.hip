@@ -172,6 +172,8 @@ int mvf_init(const mvf_grid *grid, const mvf_model *model, const mvf_bc *bc, con
   g.nyz = (int64_t)g.nny * g.nnz;
   g.npt = (int64_t)g.nnx * g.nyz;
   g.ne = (int64_t)g.nx * g.ny * g.nz;
+  // element indices are decomposed into lattice positions in 32-bit arithmetic
+  if (g.ne > INT32_MAX) return set_err(h, MVF_E_ARG, "grid too large: %lld elements, at most 2^31 - 1", (long long)g.ne);
   if ((int64_t)h->et.ne * h->et.ne * g.ne / h->et.ne > INT64_MAX / 64) return set_err(h, MVF_E_ARG, "grid too large");
   // GPML zone state (init_gpml, boundary_conds.f90:51-70)
   Gpml &gp = h->gp;

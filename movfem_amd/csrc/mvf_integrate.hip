@@ -27,24 +27,38 @@
 
 namespace mvf {
 
+// lattice position (1-based) of element e = ((i-1) ny + j-1) nz + k-1; mvf_init
+// bounds the element count by 2^31 - 1, so two 32-bit divisions suffice
 __device__ inline void elem_index(const Geom &g, int64_t e, int &i, int &j, int &k) {
-  i = (int)(e / ((int64_t)g.ny * g.nz)) + 1;
-  j = (int)((e / g.nz) % g.ny) + 1;
-  k = (int)(e % g.nz) + 1;
+  const uint32_t ue = (uint32_t)e, nz = (uint32_t)g.nz, ny = (uint32_t)g.ny;
+  const uint32_t col = ue / nz, a = col / ny;
+  k = (int)(ue - col * nz) + 1;
+  j = (int)(col - a * ny) + 1;
+  i = (int)a + 1;
 }
 
-// GPML flags used for element e (MVF_EMU_GPML_PREV_FLAGS reproduces that the
-// reference's int_elem_params reads the flags get_pml left for the previous
-// element, MoVFEM_3DMT.f90:203-206)
-__device__ inline int elem_flags(const Geom &g, const Gpml &gp, int prev, int first_flags, int64_t e) {
+// GPML flags used for element e at lattice position (i, j, k)
+// (MVF_EMU_GPML_PREV_FLAGS reproduces that the reference's int_elem_params
+// reads the flags get_pml left for the previous element,
+// MoVFEM_3DMT.f90:203-206; that element is one step back along k, wrapping
+// to j and then i)
+__device__ inline int elem_flags(const Geom &g, const Gpml &gp, int prev, int first_flags, int64_t e, int i,
+                                 int j, int k) {
   if (!gp.on) return 0;
-  int i, j, k;
-  if (!prev) {
-    elem_index(g, e, i, j, k);
-    return zone_flags(gp, i, j, k);
+  if (prev) {
+    if (e == 0) return first_flags;
+    if (k > 1) {
+      k--;
+    } else {
+      k = g.nz;
+      if (j > 1) {
+        j--;
+      } else {
+        j = g.ny;
+        i--;
+      }
+    }
   }
-  if (e == 0) return first_flags;
-  elem_index(g, e - 1, i, j, k);
   return zone_flags(gp, i, j, k);
 }
 
@@ -181,7 +195,7 @@ __global__ __launch_bounds__(256) void k_elem_km(const BasisTab *__restrict__ ta
     }
     double h[3] = {1.0, 1.0, 1.0};
     if (gp.on) {
-      int fl = elem_flags(g, gp, prev, first_flags, e);
+      int fl = elem_flags(g, gp, prev, first_flags, e, i, j, k);
       double r[3] = {0, 0, 0};
       for (int n = 0; n < NN; n++)
         for (int c = 0; c < 3; c++) r[c] += tab->N[t][n] * xs64[n][c];
@@ -349,6 +363,9 @@ int lane_tables(mvf_handle *h, const BasisTab *tab) {
 // forms Md for the three directions at every point into LDS; a lane then
 // reads the matrix of its DOF's direction (a per-lane LDS address, no
 // selects), and its operands cost one 3 x 3 product each for C and for Q.
+// The mass operands are V = phi v_d and P = phi u_d with u_d = (w h1h2h3 m2)
+// v_d formed per point in the setup (the 20-node FP32 kernel applies the 3 x 3
+// w h1h2h3 m2 per DOF instead, see PtRec).
 // The reference-element tables are zero outside the element's points and
 // DOFs, so padded rows and columns vanish without masking.
 // MFMA lane layouts (16 x 16 x 4): A lane l -> A[l%16][l/16], B lane l ->
@@ -385,12 +402,44 @@ struct Acc4<float> {
 };
 
 // per-point record in LDS: Md for d = 0, 1, 2 (27), J^-1 columns v_d (9),
-// w m1 (6), w h1h2h3 m2 (6)
-constexpr int kPt = 48;
+// w m1 (6), then u_d = (w h1h2h3 m2) v_d for d = 0, 1, 2 (9), so that the
+// mass operand of a DOF is phi u_d (three products instead of a 3 x 3 one).
+// The 20-node FP32 kernel keeps w h1h2h3 m2 itself (6) instead: the longer
+// record would cost it one of its six workgroups per CU (measured: u_d makes
+// the FP64 kernels 6-12 % faster and the 20-node FP32 one 11 % slower).
+template <typename T, int NN>
+struct PtRec {
+  static constexpr bool ud = !(sizeof(T) == 4 && NN == 20);
+  static constexpr int n = ud ? 51 : 48;
+};
 constexpr int kPtV = 27, kPtM1 = 36, kPtM2 = 42;
 
+// Stores the i <= j entries of the 16 x 16 tiles acc (tile pairs t <= t2 in
+// order) into one element's packed upper triangle, row-major:
+// (ii, jj) -> ii NE - ii (ii - 1) / 2 + jj - ii.  ob is the element's
+// (wave-uniform) base.  Lane l holds rows ii = c + s, c = 16 t + drow(0, v)
+// known at compile time and s = drow(l, 0), so the offset splits into a
+// constant, one per-lane term and -c s.
+template <typename T, int NE, int NT, typename A4>
+__device__ inline void store_upper(T *__restrict__ ob, const A4 *acc, int l) {
+  const int r = l & 15, s = mfma_drow<T>(l, 0);
+  const int lo = s * (NE - 1) - s * (s - 1) / 2 + r, dq = r - s;
+  int p = 0;
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int t2 = t; t2 < NT; t2++, p++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int c = 16 * t + mfma_drow<T>(0, v);
+        const bool upper = t2 > t || dq >= c - 16 * t2;           // ii <= jj
+        const bool inside = 16 * t2 + 16 <= NE || r < NE - 16 * t2;  // jj < NE
+        if (upper && inside) ob[(uint32_t)(c * (NE - 1) - c * (c - 1) / 2 + 16 * t2 + lo - c * s)] = acc[p][v];
+      }
+}
+
 template <typename T, int NN, int NE, int NG>
-__global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
+__global__ __launch_bounds__(256, sizeof(T) == 4 && NN == 20 ? 6 : 2) void k_elem_km_mfma(
     const BasisTab *__restrict__ tab, LaneTab<T> lt, const double *__restrict__ xp, const double *__restrict__ yp,
     const double *__restrict__ zp, const double *__restrict__ inmu, const double *__restrict__ sig, Geom g, Gpml gp,
     int prev, int first_flags, int64_t e0, int64_t n, T *__restrict__ Kout, T *__restrict__ Mout, int *bad) {
@@ -400,13 +449,15 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
   constexpr int NPAIR = NT * (NT + 1) / 2;
   constexpr int NP = NE * (NE + 1) / 2;
   typedef typename Acc4<T>::type A4;
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+  // the wave index is uniform: taking it from lane 0 keeps the element's
+  // index arithmetic scalar
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63, q = l >> 4, r = l & 15;
   const int64_t le = xcd_block(blockIdx.x, gridDim.x) * WPB + wv;  // element index within the launch
   const bool live = le < n;
   const int64_t e = e0 + (live ? le : 0);
   __shared__ double xs64[WPB][NN][3];
   __shared__ T nm[WPB][NN][12];
-  __shared__ T gpd[WPB][NG][kPt];
+  __shared__ T gpd[WPB][NG][PtRec<T, NN>::n];
   int i, j, k;
   elem_index(g, e, i, j, k);
   if (l < NN) {
@@ -425,14 +476,31 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
   // every array above is private to its wave: a wave barrier orders the
   // stores before the reads of other lanes
   __builtin_amdgcn_wave_barrier();
-  if (l < NG) {
+  {
+    // Point setup on the whole wave: lane l takes point pt = l & 31, in half
+    // hf = l >> 5.  The halves sum over complementary node sets in one
+    // instruction stream (the Jacobian and the point coordinates over nodes
+    // [hf NH, hf NH + NH), mu^-1 and Re sigma over the other half) and then
+    // add the partial sums of lanes l and l ^ 32; a + b = b + a, so both
+    // halves hold the same totals and write the same values.  Lanes with
+    // pt >= NG read the tables' zero padding and write nothing.
+    constexpr int NH = (NN + 1) / 2;
+    static_assert(NG <= 32, "one point per lane of each half wave");
+    const int hf = l >> 5, pt = l & 31;
+    const bool on = pt < NG;
+    const int pc = on ? pt : 0;
+    const int nj0 = hf * NH, nm0 = NH - hf * NH;
     // GPML stretch of the point first, while little else is live
     double h[3] = {1.0, 1.0, 1.0};
     if (gp.on) {
-      int fl = elem_flags(g, gp, prev, first_flags, e);
+      int fl = elem_flags(g, gp, prev, first_flags, e, i, j, k);
       double rr[3] = {0, 0, 0};
-      for (int m = 0; m < NN; m++)
-        for (int c = 0; c < 3; c++) rr[c] += tab->N[l][m] * xs64[wv][m][c];
+      for (int s = 0; s < NH; s++) {
+        const int m = nj0 + s;
+        if (NN % 2 == 0 || m < NN)
+          for (int c = 0; c < 3; c++) rr[c] += tab->N[pc][m] * xs64[wv][m][c];
+      }
+      for (int c = 0; c < 3; c++) rr[c] += __shfl_xor(rr[c], 32);
       for (int c = 0; c < 3; c++) h[c] = gpml_stretch(gp, fl, c, rr[c]);
     }
     const double h3 = h[0] * h[1] * h[2];
@@ -445,16 +513,23 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
 #pragma unroll
       for (int c = 0; c < 3; c++) J[m][c] = 0;
 #pragma unroll 2
-    for (int m = 0; m < NN; m++) {
-      const T x0 = (T)xs64[wv][m][0], x1 = (T)xs64[wv][m][1], x2 = (T)xs64[wv][m][2];
+    for (int s = 0; s < NH; s++) {
+      const int m = nj0 + s;
+      if (NN % 2 == 0 || m < NN) {
+        const T x0 = (T)xs64[wv][m][0], x1 = (T)xs64[wv][m][1], x2 = (T)xs64[wv][m][2];
 #pragma unroll
-      for (int a = 0; a < 3; a++) {
-        const T dn = lt.dN[(a * NN + m) * 32 + l];
-        J[a][0] += dn * x0;
-        J[a][1] += dn * x1;
-        J[a][2] += dn * x2;
+        for (int a = 0; a < 3; a++) {
+          const T dn = lt.dN[(a * NN + m) * 32 + pt];
+          J[a][0] += dn * x0;
+          J[a][1] += dn * x1;
+          J[a][2] += dn * x2;
+        }
       }
     }
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) J[m][c] += __shfl_xor(J[m][c], 32);
     T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) + J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
     const T ad = det < 0 ? -det : det;
@@ -468,38 +543,61 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
     Ji[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / ad;
     Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / ad;
     Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / ad;
-    if (live && det == (T)0) atomicExch(bad, 1);
-    T wgt = det * lt.w[l];
+    if (live && on && det == (T)0) atomicExch(bad, 1);
+    T wgt = det * lt.w[pt];
     T m1[6], m2[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) m1[c] = m2[c] = 0;
 #pragma unroll 2
-    for (int m = 0; m < NN; m++) {
-      const T Nm = lt.N[m * 32 + l];
+    for (int s = 0; s < NH; s++) {
+      const int m = nm0 + s;
+      if (NN % 2 == 0 || m < NN) {
+        const T Nm = lt.N[m * 32 + pt];
 #pragma unroll
-      for (int c = 0; c < 6; c++) {
-        m1[c] += Nm * nm[wv][m][c];
-        m2[c] += Nm * nm[wv][m][6 + c];
+        for (int c = 0; c < 6; c++) {
+          m1[c] += Nm * nm[wv][m][c];
+          m2[c] += Nm * nm[wv][m][6 + c];
+        }
       }
     }
-    const T qa1[3] = {q0, q1, q2}, qa2[3] = {q1, q2, q0};
-    T *o = gpd[wv][l];
-#pragma unroll 1
-    for (int d = 0; d < 3; d++) {
-      const T v[3] = {Ji[0 * 3 + d], Ji[1 * 3 + d], Ji[2 * 3 + d]};
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        const int b = (a + 1) % 3, c = (a + 2) % 3;
-#pragma unroll
-        for (int m = 0; m < 3; m++) o[d * 9 + a * 3 + m] = qa1[a] * v[c] * Ji[b * 3 + m] - qa2[a] * v[b] * Ji[c * 3 + m];
-        o[kPtV + d * 3 + a] = v[a];
-      }
-    }
-    const T wh = (T)((double)wgt * h3);
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      o[kPtM1 + c] = wgt * m1[c];
-      o[kPtM2 + c] = wh * m2[c];
+      m1[c] += __shfl_xor(m1[c], 32);
+      m2[c] += __shfl_xor(m2[c], 32);
+    }
+    if (on) {
+      const T qa1[3] = {q0, q1, q2}, qa2[3] = {q1, q2, q0};
+      T *o = gpd[wv][pt];
+      const T wh = (T)((double)wgt * h3);
+      T w2[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) w2[c] = wh * m2[c];
+      // direction maps: half hf writes direction hf, then both halves direction 2
+#pragma unroll
+      for (int it = 0; it < 2; it++) {
+        const int d = it == 0 ? hf : 2;
+        T v[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[a] = it == 0 ? (hf ? Ji[a * 3 + 1] : Ji[a * 3]) : Ji[a * 3 + 2];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const int b = (a + 1) % 3, c = (a + 2) % 3;
+#pragma unroll
+          for (int m = 0; m < 3; m++)
+            o[d * 9 + a * 3 + m] = qa1[a] * v[c] * Ji[b * 3 + m] - qa2[a] * v[b] * Ji[c * 3 + m];
+          o[kPtV + d * 3 + a] = v[a];
+        }
+        if constexpr (PtRec<T, NN>::ud) {
+          o[kPtM2 + d * 3 + 0] = w2[0] * v[0] + w2[1] * v[1] + w2[2] * v[2];
+          o[kPtM2 + d * 3 + 1] = w2[1] * v[0] + w2[3] * v[1] + w2[4] * v[2];
+          o[kPtM2 + d * 3 + 2] = w2[2] * v[0] + w2[4] * v[1] + w2[5] * v[2];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        o[kPtM1 + c] = wgt * m1[c];
+        if constexpr (!PtRec<T, NN>::ud) o[kPtM2 + c] = w2[c];
+      }
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -510,17 +608,7 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
   for (int t = 0; t < NT; t++) dr[t] = lt.dir[t * 16 + r];
   const int64_t base = le * (int64_t)NP;
   auto store = [&](T *out, const A4 *acc) {
-    if (!live) return;
-    int p = 0;
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int t2 = t; t2 < NT; t2++, p++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int ii = 16 * t + mfma_drow<T>(l, v), jj = 16 * t2 + r;
-          if (ii <= jj && jj < NE) out[base + ii * NE - ii * (ii - 1) / 2 + (jj - ii)] = acc[p][v];
-        }
+    if (live) store_upper<T, NE, NT>(out + base, acc, l);
   };
   {  // K = C^T Q
     A4 acc[NPAIR];
@@ -568,8 +656,9 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
       const int gg = 4 * u + q;
       const T *pd = gpd[wv][gg < NG ? gg : 0];
       T mw[6];
+      if constexpr (!PtRec<T, NN>::ud)
 #pragma unroll
-      for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
+        for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
       T bv[NT][3];
 #pragma unroll
       for (int t = NT - 1; t >= 0; t--) {
@@ -579,9 +668,16 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma(
         av[0] = ph * vd[0];
         av[1] = ph * vd[1];
         av[2] = ph * vd[2];
-        bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
-        bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
-        bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+        if constexpr (PtRec<T, NN>::ud) {
+          const T *ud = pd + kPtM2 + dr[t] * 3;
+          bv[t][0] = ph * ud[0];
+          bv[t][1] = ph * ud[1];
+          bv[t][2] = ph * ud[2];
+        } else {
+          bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
+          bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
+          bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+        }
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -613,12 +709,12 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
   constexpr int NPAIR = NT * (NT + 1) / 2;
   constexpr int NP = NE * (NE + 1) / 2;
   typedef typename Acc4<T>::type A4;
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63, q = l >> 4, r = l & 15;
   // first element of this wave within the launch
   const int64_t le0 = (xcd_block(blockIdx.x, gridDim.x) * WPB + wv) * EPW;
   __shared__ double xs64[WPB][EPW][NN][3];
   __shared__ T nm[WPB][EPW][NN][12];
-  __shared__ T gpd[WPB][EPW][NG][kPt];
+  __shared__ T gpd[WPB][EPW][NG][PtRec<T, NN>::n];
   // lane -> (element of the wave, node); the element's lattice position
   const int ek0 = l / NN;
   const int64_t le_n = le0 + ek0;
@@ -650,7 +746,10 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
     // GPML stretch of the point first, while little else is live
     double h[3] = {1.0, 1.0, 1.0};
     if (gp.on) {
-      int fl = elem_flags(g, gp, prev, first_flags, e);
+      // the node lanes' (i, j, k) belong to the same element when NN == NG
+      int fi = i, fj = j, fk = k;
+      if constexpr (NN != NG) elem_index(g, e, fi, fj, fk);
+      int fl = elem_flags(g, gp, prev, first_flags, e, fi, fj, fk);
       double rr[3] = {0, 0, 0};
       for (int m = 0; m < NN; m++)
         for (int c = 0; c < 3; c++) rr[c] += tab->N[pt][m] * xs64[wv][ek][m][c];
@@ -705,6 +804,10 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
     }
     const T qa1[3] = {q0, q1, q2}, qa2[3] = {q1, q2, q0};
     T *o = gpd[wv][ek][pt];
+    const T wh = (T)((double)wgt * h3);
+    T w2[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) w2[c] = wh * m2[c];
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
       const T v[3] = {Ji[0 * 3 + d], Ji[1 * 3 + d], Ji[2 * 3 + d]};
@@ -715,12 +818,16 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
         for (int m = 0; m < 3; m++) o[d * 9 + a * 3 + m] = qa1[a] * v[c] * Ji[b * 3 + m] - qa2[a] * v[b] * Ji[c * 3 + m];
         o[kPtV + d * 3 + a] = v[a];
       }
+      if constexpr (PtRec<T, NN>::ud) {
+        o[kPtM2 + d * 3 + 0] = w2[0] * v[0] + w2[1] * v[1] + w2[2] * v[2];
+        o[kPtM2 + d * 3 + 1] = w2[1] * v[0] + w2[3] * v[1] + w2[4] * v[2];
+        o[kPtM2 + d * 3 + 2] = w2[2] * v[0] + w2[4] * v[1] + w2[5] * v[2];
+      }
     }
-    const T wh = (T)((double)wgt * h3);
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       o[kPtM1 + c] = wgt * m1[c];
-      o[kPtM2 + c] = wh * m2[c];
+      if constexpr (!PtRec<T, NN>::ud) o[kPtM2 + c] = w2[c];
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -733,19 +840,9 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
   const int64_t le = le0 + ek;
   const bool live = le < n;
   const int64_t base = le * (int64_t)NP;
-  const T(*gk)[kPt] = gpd[wv][ek];
+  const T(*gk)[PtRec<T, NN>::n] = gpd[wv][ek];
   auto store = [&](T *out, const A4 *acc) {
-    if (!live) return;
-    int p = 0;
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int t2 = t; t2 < NT; t2++, p++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int ii = 16 * t + mfma_drow<T>(l, v), jj = 16 * t2 + r;
-          if (ii <= jj && jj < NE) out[base + ii * NE - ii * (ii - 1) / 2 + (jj - ii)] = acc[p][v];
-        }
+    if (live) store_upper<T, NE, NT>(out + base, acc, l);
   };
   {  // K = C^T Q
     A4 acc[NPAIR];
@@ -793,8 +890,9 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
       const int gg = 4 * u + q;
       const T *pd = gk[gg < NG ? gg : 0];
       T mw[6];
+      if constexpr (!PtRec<T, NN>::ud)
 #pragma unroll
-      for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
+        for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
       T bv[NT][3];
 #pragma unroll
       for (int t = NT - 1; t >= 0; t--) {
@@ -804,9 +902,16 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
         av[0] = ph * vd[0];
         av[1] = ph * vd[1];
         av[2] = ph * vd[2];
-        bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
-        bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
-        bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+        if constexpr (PtRec<T, NN>::ud) {
+          const T *ud = pd + kPtM2 + dr[t] * 3;
+          bv[t][0] = ph * ud[0];
+          bv[t][1] = ph * ud[1];
+          bv[t][2] = ph * ud[2];
+        } else {
+          bv[t][0] = mw[0] * av[0] + mw[1] * av[1] + mw[2] * av[2];
+          bv[t][1] = mw[1] * av[0] + mw[3] * av[1] + mw[4] * av[2];
+          bv[t][2] = mw[2] * av[0] + mw[4] * av[1] + mw[5] * av[2];
+        }
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -875,7 +980,7 @@ __global__ __launch_bounds__(64) void k_elem_rhs(const BasisTab *__restrict__ ta
     double wgt = det * tab->w[t];
     double h[3] = {1.0, 1.0, 1.0};
     if (gp.on) {
-      int fl = elem_flags(g, gp, prev, first_flags, e);
+      int fl = elem_flags(g, gp, prev, first_flags, e, i, j, k);
       double r[3] = {0, 0, 0};
       for (int n = 0; n < NN; n++)
         for (int c = 0; c < 3; c++) r[c] += tab->N[t][n] * xs[n][c];
