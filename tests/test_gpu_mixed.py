@@ -51,12 +51,15 @@ def _norm_err(X32, X64, rp, col):
     return float((np.abs(X32 - X64) / scale).max())
 
 
-@pytest.mark.parametrize("case", ["halfspace_block", "hex20_gpml"])
+@pytest.mark.parametrize("case", ["halfspace_block", "hex20_gpml", "hex27_gpml_tensor_topo"])
 def test_fp32_integration_operators(case, need_gpu):
     if case == "halfspace_block":
         m = M.halfspace_block()
-    else:
+    elif case == "hex20_gpml":
         m = M.random_small(20, 4, 4, 4, seed=31, dirichlet=False, nextd=1, anisotropic=True, freqs=(1.0,))
+    else:
+        m = M.random_small(27, 3, 3, 3, seed=32, dirichlet=False, nextd=1, full_tensor=True, topo=150.0,
+                           mu_contrast=True, freqs=(1.0,))
     print(f"fp32 integration {case}", flush=True)
     g = GpuVfem(m)
     rp, col, K64, M64 = g.operators()

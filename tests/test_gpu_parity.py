@@ -32,8 +32,7 @@ def test_numbering_and_sizes(name, need_gpu):
     assert g.nnze == o.nnze
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_element_matrices(name, need_gpu):
+def _element_matrices(name):
     m = CASES[name]
     print(f"case {name}", flush=True)
     g = GpuVfem(m)
@@ -58,6 +57,19 @@ def test_element_matrices(name, need_gpu):
                 e += 1
     assert ea < 1e-12, ea
     assert eb < 1e-12, eb
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_element_matrices(name, need_gpu):
+    _element_matrices(name)
+
+
+@pytest.mark.parametrize("name", ["hex8_gpml_aniso", "hex20_gpml_tensor_topo", "hex27_gpml_mu_topo"])
+def test_element_matrices_valu_kernel(name, need_gpu, monkeypatch):
+    """The one-workgroup-per-element VALU kernel (MVF_ELEM_KERNEL=valu, read when
+    a handle is created) to the same bars."""
+    monkeypatch.setenv("MVF_ELEM_KERNEL", "valu")
+    _element_matrices(name)
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
