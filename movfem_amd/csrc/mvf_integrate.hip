@@ -404,9 +404,9 @@ struct Acc4<float> {
 // per-point record in LDS: Md for d = 0, 1, 2 (27), J^-1 columns v_d (9),
 // w m1 (6), then u_d = (w h1h2h3 m2) v_d for d = 0, 1, 2 (9), so that the
 // mass operand of a DOF is phi u_d (three products instead of a 3 x 3 one).
-// The 20-node FP32 kernel keeps w h1h2h3 m2 itself (6) instead: the longer
-// record would cost it one of its six workgroups per CU (measured: u_d makes
-// the FP64 kernels 6-12 % faster and the 20-node FP32 one 11 % slower).
+// The 20-node FP32 kernel keeps w h1h2h3 m2 itself (6) instead (measured: u_d
+// makes the FP64 kernels 6-12 % faster and the 20-node FP32 one 11 % slower,
+// with four or with three waves per workgroup under its 80-register budget).
 template <typename T, int NN>
 struct PtRec {
   static constexpr bool ud = !(sizeof(T) == 4 && NN == 20);
