@@ -655,10 +655,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 && NN == 20 ? 6 : 2) void k_ele
     for (int u = 0; u < NU; u++) {
       const int gg = 4 * u + q;
       const T *pd = gpd[wv][gg < NG ? gg : 0];
-      T mw[6];
-      if constexpr (!PtRec<T, NN>::ud)
+      T mw[6] = {};
+      if constexpr (!PtRec<T, NN>::ud) {
 #pragma unroll
         for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
+      }
       T bv[NT][3];
 #pragma unroll
       for (int t = NT - 1; t >= 0; t--) {
@@ -889,10 +890,11 @@ __global__ __launch_bounds__(256, 2) void k_elem_km_mfma_multi(
     for (int u = 0; u < NU; u++) {
       const int gg = 4 * u + q;
       const T *pd = gk[gg < NG ? gg : 0];
-      T mw[6];
-      if constexpr (!PtRec<T, NN>::ud)
+      T mw[6] = {};
+      if constexpr (!PtRec<T, NN>::ud) {
 #pragma unroll
         for (int c = 0; c < 6; c++) mw[c] = pd[kPtM2 + c];
+      }
       T bv[NT][3];
 #pragma unroll
       for (int t = NT - 1; t >= 0; t--) {
