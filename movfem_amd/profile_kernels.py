@@ -4,6 +4,7 @@
     python -m movfem_amd.profile_kernels spmv  [--nodes 20] [--iters 50]
     python -m movfem_amd.profile_kernels unet  [--steps 3]
     python -m movfem_amd.profile_kernels copy  [--mib 1024]
+    python -m movfem_amd.profile_kernels factor [--nodes 20]
 
 elem: mvf_assemble (element integration and assembly of K and M) repeated on
   the anisotropic GPML workload of the given element type (20-node: 64 x 64 x 48
@@ -16,6 +17,9 @@ unet: forward + backward + Adam steps of the learned-guess network on a batch
   of two 64 x 64 x 48 samples (random inputs).
 copy: three device-to-device copies of a known size, the reference the HBM
   traffic counters are scaled by.
+factor: one direct-preconditioned GMRES solve at 1e-3 Hz on the elem workload
+  with the library's verbose timing on: the factorization's cumulative time
+  after each tree level and its GEMM time per mode go to stderr.
 Each prints one JSON line with what it ran.
 """
 from __future__ import annotations
@@ -41,7 +45,7 @@ def model(nodes: int, full: bool = False):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["elem", "spmv", "unet", "copy"])
+    ap.add_argument("what", choices=["elem", "spmv", "unet", "copy", "factor"])
     ap.add_argument("--nodes", type=int, default=20)
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--reps", type=int, default=5)
@@ -78,9 +82,13 @@ def main():
         from .vfem import GpuVfem, MvfError
 
         m = model(a.nodes, a.full)
-        g = GpuVfem(m)
+        g = GpuVfem(m, verbose=1 if a.what == "factor" else 0)
         out.update(nodes=a.nodes, elements=m.ne, nne=g.nne, nnz=g.nnze)
-        if a.what == "elem":
+        if a.what == "factor":
+            info = g.solve_frequencies([1e-3], method="gmres", precond="direct", tol=1e-8)
+            out.update(iterations=info["iterations"], true_resid=info["true_resid"][:2],
+                       factor_seconds=info.get("factor_seconds"))
+        elif a.what == "elem":
             ms = [g.assemble(a.precision) for _ in range(a.reps)]
             out.update(precision=a.precision, reps=a.reps, kernel_ms=ms, elem_per_s=m.ne / (np.median(ms) * 1e-3))
         else:
