@@ -205,8 +205,8 @@ def main():
                     help="only run a device-to-device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every GPU box of this build has one, and the path has no multi-GPU "
-              f"code (asked for --gpus {args.gpus})", file=sys.stderr)
+        print(f"bench.py measures one GPU: the path runs on a single GPU and has no multi-GPU code "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe_mib:
         import torch

@@ -35,7 +35,7 @@ enum {
   MVF_E_DEVICE = -2,     /* HIP runtime error                                      */
   MVF_E_SINGULAR = -3,   /* zero Jacobian or singular sigma/mu tensor at a node   */
   MVF_E_INDEX = -4,      /* slot/index consistency failure                         */
-  MVF_E_UNSUPPORTED = -5,/* option or combination this build does not provide      */
+  MVF_E_UNSUPPORTED = -5,/* option or combination the library does not provide     */
   MVF_E_NOCONV = -6,     /* Krylov did not reach the tolerance                     */
   MVF_E_BREAKDOWN = -7,  /* Krylov breakdown (rho or omega vanished)               */
   MVF_E_MEMORY = -8      /* device allocation failed                               */

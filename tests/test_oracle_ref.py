@@ -1,6 +1,6 @@
 """The CPU oracle against the reference's own modules, live (oracle/_ref).
 
-Skipped where the reference build is absent (e.g. on a box without the
+Skipped where the reference build is absent (on a machine without the
 reference sources); tests/test_oracle_golden.py carries the same checks on
 committed vectors."""
 import numpy as np
