@@ -473,9 +473,20 @@ __global__ void k_round_c64(double2 *p, int64_t n) {
   }
 }
 
-__global__ void k_fill2(double2 *p, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = make_double2(0.0, 0.0);
+// zeros what the assembly and extend-add of a front accumulate into: the top
+// band's lower triangle T(i, 0..i) and the bottom band's F21 and lower F22,
+// B(r, 0..n1+r).  The rest is either written whole before it is read (the top
+// band's upper part, by k_fa_sym) or never read (F22 above the diagonal: the
+// Schur update and the extend-add use its lower triangle only), so the level
+// is cleared at half the bytes of a full fill.  One wave per row.
+__global__ __launch_bounds__(256) void k_fa_zero(const DNode *nodes, const int32_t *lev, double2 *fac, double2 *pool) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int lane = threadIdx.x & 63;
+  const double2 z = make_double2(0.0, 0.0);
+  for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < nd.m; r += (int64_t)gridDim.x * 4) {
+    double2 *row = r < nd.n1 ? fac + nd.fac + r * nd.m : pool + nd.pool + (r - nd.n1) * nd.m;
+    for (int64_t j = lane; j <= r; j += 64) row[j] = z;
+  }
 }
 
 // original entries: column j of the front = owned DOF d; every CSR entry (d, k)
@@ -1671,9 +1682,8 @@ static int direct_factor(mvf_handle *h, double w, int c64, int f32 = 0) {
     const int32_t *lev = P->d_lev + P->lev_off[d];
     double2 *pool = (d % 2 == 0) ? P->d_arena : P->d_arena + (P->arena - P->lev_pool[d]);
     const int64_t nfac = P->lev_fac1[d] - P->lev_fac0[d];
-    hipLaunchKernelGGL(k_fill2, dim3(cap_grid((nfac + 255) / 256)), dim3(256), 0, st, P->d_fac + P->lev_fac0[d], nfac);
-    hipLaunchKernelGGL(k_fill2, dim3(cap_grid((P->lev_pool[d] + 255) / 256)), dim3(256), 0, st, pool, P->lev_pool[d]);
     const int64_t mn1 = P->lev_maxn1[d], mn2 = P->lev_maxn2[d], mm = P->lev_maxm[d];
+    hipLaunchKernelGGL(k_fa_zero, dim3(cap_grid((mm + 3) / 4), nl), dim3(256), 0, st, P->d_nodes, lev, P->d_fac, pool);
     hipLaunchKernelGGL(k_fa_assemble, dim3(cap_grid((mn1 + 3) / 4), nl), dim3(256), 0, st, P->d_nodes, lev, P->d_idx,
                        h->d_rowptr, h->d_col, P->d_dst, h->d_K, h->d_M, w, c64, reg, P->d_fac, pool);
     if (d < P->maxdepth) {
