@@ -1237,14 +1237,17 @@ __global__ __launch_bounds__(256) void k_fa_urow(const DNode *nodes, const int32
   mfma_tiles<true>(inv + nd.inv + kNB * kNB * b, w, T + k * m + c0, m, T + k * m + c0, m, w, c1 - c0, w, false);
 }
 
-// row permutation of each front from its interchange sequence
-__global__ void k_fa_perm(const DNode *nodes, const int32_t *lev, int nl, const int32_t *piv, int32_t *perm) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nl) return;
-  const DNode nd = nodes[lev[t]];
+// row permutation of each front from its interchange sequence.  The pivot
+// search never leaves a kNB-row panel block, so the blocks' permutations are
+// independent: one thread per (block, front), kNB interchanges each.
+__global__ void k_fa_perm(const DNode *nodes, const int32_t *lev, const int32_t *piv, int32_t *perm) {
+  const DNode nd = nodes[lev[blockIdx.y]];
+  const int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kNB;
+  if (b0 >= nd.n1) return;
+  const int64_t b1 = b0 + kNB < nd.n1 ? b0 + kNB : nd.n1;
   int32_t *p = perm + nd.pos0;
-  for (int64_t q = 0; q < nd.n1; q++) p[q] = (int32_t)q;
-  for (int64_t q = 0; q < nd.n1; q++) {
+  for (int64_t q = b0; q < b1; q++) p[q] = (int32_t)q;
+  for (int64_t q = b0; q < b1; q++) {
     const int64_t r = piv[nd.pos0 + q];
     const int32_t a = p[q];
     p[q] = p[r];
@@ -1839,7 +1842,11 @@ static int direct_factor(mvf_handle *h, double w, int c64, int f32 = 0) {
         gemm(M2{}, 0, 0, st);
         round_level();
       }
-      hipLaunchKernelGGL(k_fa_perm, dim3((nl + 63) / 64), dim3(64), 0, st, P->d_nodes, lev, nl, P->d_piv, P->d_perm);
+      {
+        const int64_t nbk = (mn1 + kNB - 1) / kNB;
+        hipLaunchKernelGGL(k_fa_perm, dim3((unsigned)((nbk + 63) / 64), nl), dim3(64), 0, st, P->d_nodes, lev, P->d_piv,
+                           P->d_perm);
+      }
     }
     MVF_HIP(h, hipGetLastError());
     prev = pool;
