@@ -50,10 +50,12 @@ def test_unet_matches_torch(need_gpu):
 def test_training_reduces_loss(need_gpu):
     """Adam on six 16 x 16 x (12 + 4) solver fields.  The targets are normalized
     per polarization, so predicting zero costs about 0.5 (computed below as the
-    masked mean of t^2); within 150 steps (learning rate 5e-4 after a 30-step
-    linear warm-up; at 2e-3 without warm-up the loss can rise, and the weight
-    gradient's atomic sums make runs differ) the masked loss falls below half
-    of that, i.e. the network explains more than half of the targets' energy."""
+    masked mean of t^2).  Within 300 steps (learning rate 5e-4 after a 30-step
+    linear warm-up; at 2e-3 without warm-up the loss can rise) the masked loss
+    falls below half of that, i.e. the network explains more than half of the
+    targets' energy.  The weight gradient's atomic sums make runs differ: after
+    150 steps one run ended at 0.253, just above the bar; after 300 steps runs
+    end near 0.085."""
     data = []
     for s in range(6):
         m, f = Lm.training_model(900 + s, nx=16, ny=16, nz_earth=12, nz_air=4)
@@ -68,7 +70,7 @@ def test_training_reduces_loss(need_gpu):
     Mk = np.stack([d[2] for d in data])
     net = Lm.Net(nx=16, ny=16, nz=16, batch=2, seed=3)
     losses = []
-    for step in range(150):
+    for step in range(300):
         idx = [(2 * step) % 6, (2 * step + 1) % 6]
         losses.append(net.forward_backward(X[idx], T[idx], Mk[idx]))
         net.adam(5e-4 * min(1.0, (step + 1) / 30))
