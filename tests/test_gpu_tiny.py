@@ -85,3 +85,29 @@ def test_tiny_grid_direct_solve(nodes, dims, need_gpu):
     for d in range(2):
         r = np.linalg.norm(b[d] - A @ x[d]) / np.linalg.norm(b[d])
         assert r <= 1e-8 * 1.0001, (d, r)
+
+
+def test_element_count_limit(need_gpu):
+    """The element index is decomposed in 32-bit arithmetic, so mvf_init
+    refuses more than 2^31 - 1 elements (2048 x 2048 x 1024 here) with
+    MVF_E_ARG before it reads the lattice or model arrays (which are
+    one-element stand-ins here) or allocates anything for them."""
+    import ctypes as C
+
+    lib = L.lib()
+    one = np.zeros(2)
+    dp = one.ctypes.data_as(C.POINTER(C.c_double))
+    grid = L.Grid(2048, 2048, 1024, 8, dp, dp, dp)
+    model = L.Model(dp, dp)
+    lz = np.zeros(2)
+    bc = L.BC(1, 1, 0, 1, 1, 0, 1.0, 1.0, 2.0, 0.0, 0.01, 1, 0, lz.ctypes.data_as(C.POINTER(C.c_double)),
+              lz.ctypes.data_as(C.POINTER(C.c_double)))
+    opts = L.Options(L.MVF_EMU_ALL, 0, 0, 0)
+    h = C.c_void_p()
+    print("element count limit: 2048 x 2048 x 1024", flush=True)
+    rc = lib.mvf_init(C.byref(grid), C.byref(model), C.byref(bc), C.byref(opts), C.byref(h))
+    msg = lib.mvf_last_error(h).decode() if h.value else ""
+    if h.value:
+        lib.mvf_free(h)
+    assert rc == L.MVF_E_ARG, (rc, msg)
+    assert "2^31" in msg, msg
