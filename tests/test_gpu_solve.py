@@ -107,3 +107,32 @@ def test_halfspace_block_solve(need_gpu):
         e = np.linalg.norm(x[d][mask] - xref[d][mask]) / np.linalg.norm(xref[d][mask])
         print(f"pol {d}: determined-DOF rel L2 {e:.2e}, full-vector {np.linalg.norm(x[d]-xref[d])/np.linalg.norm(xref[d]):.2e}")
         assert e <= 1e-6, (d, e)
+
+
+def test_sixteen_frequencies_one_call(need_gpu):
+    """The 27-node workload's 16 frequencies (1e-3 to 1e3 Hz) in one
+    mvf_solve_frequencies call, which solves them in two batches of 8: every
+    frequency to the bar on its own system, and a frequency of the second
+    batch equal to its single-frequency solve.  Emulation off, so every
+    frequency's system is consistent (the reference's update_sigma range
+    leaves the second and later frequencies' air sources the null space
+    cannot absorb)."""
+    freqs = tuple(np.logspace(-3, 3, 16))
+    m = M.random_small(8, 4, 4, 4, seed=14, nz_air=1, freqs=freqs)
+    g = GpuVfem(m, emulate=0)
+    print("16 frequencies: one call", flush=True)
+    info = g.solve_frequencies(list(freqs), method="gmres", precond="direct", tol=1e-8)
+    assert info["converged"], info
+    assert len(info["true_resid"]) >= 32
+    assert max(info["true_resid"][:32]) <= 1e-8 * 1.0001, info["true_resid"][:32]
+    xb = g.solution(32)
+    i = 12
+    gs = GpuVfem(m, emulate=0)
+    gs.set_frequency(i)
+    A, b = reference_system(gs)
+    assert np.allclose(g.rhs(32)[2 * i:2 * i + 2], b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+    x, _ = gs.solve(b.ravel(), method="gmres", precond="direct", tol=1e-8)
+    x = x.reshape(2, -1)
+    xref = np.stack([spla.splu(A).solve(b[d]) for d in range(2)])
+    _check(A, b, xb[2 * i:2 * i + 2], xref)
+    _check(A, b, x, xref)
