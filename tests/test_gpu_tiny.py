@@ -111,3 +111,17 @@ def test_element_count_limit(need_gpu):
         lib.mvf_free(h)
     assert rc == L.MVF_E_ARG, (rc, msg)
     assert "2^31" in msg, msg
+
+
+def test_frequency_count_limits(need_gpu):
+    """mvf_solve_frequencies takes 1..32 frequencies per call (the 27-node
+    workload's 16 fit one call); none or 33 are refused with MVF_E_ARG."""
+    m = M.random_small(27, 1, 1, 2, seed=41, nz_air=1)
+    g = GpuVfem(m)
+    for freqs in ([], list(np.logspace(-3, 3, 33))):
+        print(f"frequency count {len(freqs)}", flush=True)
+        with pytest.raises(MvfError) as e:
+            g.solve_frequencies(freqs, method="gmres", precond="direct", tol=1e-8)
+        assert e.value.code == L.MVF_E_ARG
+    info = g.solve_frequencies(list(np.logspace(-3, 3, 32)), method="gmres", precond="direct", tol=1e-8)
+    assert info["converged"]
