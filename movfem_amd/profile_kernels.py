@@ -5,6 +5,7 @@
     python -m movfem_amd.profile_kernels unet  [--steps 3]
     python -m movfem_amd.profile_kernels copy  [--mib 1024]
     python -m movfem_amd.profile_kernels factor [--nodes 20]
+    python -m movfem_amd.profile_kernels boundary [--nodes 20] [--reps 2]
 
 elem: mvf_assemble (element integration and assembly of K and M) repeated on
   the anisotropic GPML workload of the given element type (20-node: 64 x 64 x 48
@@ -20,6 +21,12 @@ copy: three device-to-device copies of a known size, the reference the HBM
 factor: one direct-preconditioned GMRES solve at 1e-3 Hz on the elem workload
   with the library's verbose timing on: the factorization's cumulative time
   after each tree level and its GEMM time per mode go to stderr.
+boundary: one frequency (1e-3 Hz) through the host-buffer boundary the
+  reference's call sequence uses (global_vfem into host ia/ja/a/b, then the
+  zmumps(job=6)-shaped mvf_solve on a host right-hand side, solution back to
+  the host), timed per call with the bytes that cross PCIe, beside the same
+  frequency solved with the right-hand side and the solution kept on the device
+  (mvf_solve_frequencies).  The first repetition of each is a warm-up.
 Each prints one JSON line with what it ran.
 """
 from __future__ import annotations
@@ -30,22 +37,22 @@ import json
 import numpy as np
 
 
-def model(nodes: int, full: bool = False):
+def model(nodes: int, full: bool = False, freq: float = 1.0):
     from . import model as M
 
     if full:
         # the 27-node workload at full size: 128 x 128 x 96 cells under Gaussian hills, full-tensor sigma
         return M.hills_tensor()
     if nodes == 27:
-        return M.anisotropic_blocks(nx=32, ny=32, nz_earth=20, nz_air=4, h=500.0, nodes=27, freqs=(1.0,), block=4,
+        return M.anisotropic_blocks(nx=32, ny=32, nz_earth=20, nz_air=4, h=500.0, nodes=27, freqs=(freq,), block=4,
                                     seed=2, nextd=4)
-    return M.anisotropic_blocks(nx=64, ny=64, nz_earth=40, nz_air=8, h=500.0, nodes=nodes, freqs=(1.0,), block=4,
+    return M.anisotropic_blocks(nx=64, ny=64, nz_earth=40, nz_air=8, h=500.0, nodes=nodes, freqs=(freq,), block=4,
                                 seed=2, nextd=4)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["elem", "spmv", "unet", "copy", "factor"])
+    ap.add_argument("what", choices=["elem", "spmv", "unet", "copy", "factor", "boundary"])
     ap.add_argument("--nodes", type=int, default=20)
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--reps", type=int, default=5)
@@ -78,6 +85,32 @@ def main():
             net.adam(1e-3)
         out.update(net.stats())
         out["tflops"] = out["conv_flops"] / out["conv_ms"] / 1e9
+    elif a.what == "boundary":
+        import time
+
+        from . import _lib as L
+        from .vfem import GpuVfem
+
+        m = model(a.nodes, a.full, freq=1e-3)
+        g = GpuVfem(m, emulate=L.MVF_EMU_GPML_PREV_FLAGS)
+        out.update(nodes=a.nodes, elements=m.ne, nne=g.nne, nnz=g.nnze, freq_hz=1e-3)
+        host, dev = [], []
+        for r in range(a.reps):
+            t0 = time.perf_counter()
+            g.set_frequency(0)
+            ia, ja, av, b = g.global_vfem()
+            t1 = time.perf_counter()
+            x, info = g.solve(b, method="gmres", precond="direct", tol=1e-8)
+            t2 = time.perf_counter()
+            host.append({"global_vfem_s": t1 - t0, "solve_s": t2 - t1, "total_s": t2 - t0,
+                         "iterations": info["iterations"], "true_resid": info["true_resid"][:2]})
+            t3 = time.perf_counter()
+            d = g.solve_frequencies([1e-3], method="gmres", precond="direct", tol=1e-8)
+            dev.append({"total_s": time.perf_counter() - t3, "iterations": d["iterations"],
+                        "true_resid": d["true_resid"][:2]})
+            print(f"boundary rep {r}: host {host[-1]} device {dev[-1]}", flush=True)
+        out.update(host_boundary=host, device_resident=dev,
+                   bytes_to_host=ia.nbytes + ja.nbytes + av.nbytes + b.nbytes + x.nbytes, bytes_to_device=b.nbytes)
     else:
         from .vfem import GpuVfem, MvfError
 
