@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <new>
@@ -59,6 +60,29 @@ __global__ void k_apply_carry(const int32_t *gne0, int me, const int64_t *rowptr
       M[pos] += sign * dM[p];
       K32[pos] = (float)K[pos];
       M32[pos] = (float)M[pos];
+    }
+  }
+}
+
+// rows [r0, r1) of the reference's coordinate form, written at p - rowptr[r0]:
+// (ia, ja) 1-based, a = fl32(K) + i fl32(fl32(omega) M) on and above the
+// diagonal and 0 below (ga_sort_sparse leaves the lower slots unassigned).
+// One wave per row, lanes over the row's entries.
+__global__ void k_gv_format(int32_t r0, int32_t r1, const int64_t *rowptr, const int32_t *col, const double *K,
+                            const double *M, double w32, int32_t *ia, int32_t *ja, double *a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p0 = rowptr[r0];
+  const int64_t rstep = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t r = r0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; r < r1; r += rstep) {
+    const int64_t pe = rowptr[r + 1];
+    for (int64_t p = rowptr[r] + lane; p < pe; p += 64) {
+      const int32_t c = col[p];
+      const int64_t q = p - p0;
+      const bool up = c >= r;
+      ia[q] = (int32_t)r + 1;
+      ja[q] = c + 1;
+      a[2 * q] = up ? (double)(float)K[p] : 0.0;
+      a[2 * q + 1] = up ? (double)(float)(w32 * M[p]) : 0.0;
     }
   }
 }
@@ -472,31 +496,45 @@ int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *
   MVF_TRY(ds.alloc(h, &db, (size_t)nne * 4));
   MVF_TRY(build_rhs(h, h->omega, h->calls, db, 2, 0));
   std::vector<int64_t> rp(nne + 1);
-  std::vector<int32_t> col(h->nnz);
-  std::vector<double> K(h->nnz), M(h->nnz), bb((size_t)nne * 4);
+  std::vector<double> bb((size_t)nne * 4);
   MVF_HIP(h, hipMemcpyAsync(rp.data(), h->d_rowptr, 8 * (nne + 1), hipMemcpyDeviceToHost, h->stream));
-  MVF_HIP(h, hipMemcpyAsync(col.data(), h->d_col, 4 * h->nnz, hipMemcpyDeviceToHost, h->stream));
-  MVF_HIP(h, hipMemcpyAsync(K.data(), h->d_K, 8 * h->nnz, hipMemcpyDeviceToHost, h->stream));
-  MVF_HIP(h, hipMemcpyAsync(M.data(), h->d_M, 8 * h->nnz, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipMemcpyAsync(bb.data(), db, 32 * (size_t)nne, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   ds.drop(db);
-  const double w32 = (double)(float)h->omega;
-  for (int32_t r = 0; r < nne; r++)
-    for (int64_t p = rp[r]; p < rp[r + 1]; p++) {
-      ia[p] = r + 1;
-      ja[p] = col[p] + 1;
-      // ga_sort_sparse leaves the lower-slot value of (col,row) at the
-      // row-major position of (row,col): upper entries carry the symmetric
-      // value, lower ones the never-assigned zero; all rounded to complex64
-      if (col[p] >= r) {
-        a[2 * p] = (double)(float)K[p];
-        a[2 * p + 1] = (double)(float)(w32 * M[p]);
-      } else {
-        a[2 * p] = 0.0;
-        a[2 * p + 1] = 0.0;
-      }
+  // The coordinate form is written on the device in row chunks of at most
+  // `chunk` entries (a row is never split) and copied straight into the
+  // caller's arrays: ga_sort_sparse leaves the lower-slot value of (col,row)
+  // at the row-major position of (row,col), so upper entries carry the
+  // symmetric value and lower ones the never-assigned zero, all rounded to
+  // complex64.  MVF_GV_CHUNK overrides the chunk size (entries).
+  int64_t chunk = int64_t(1) << 24;
+  if (const char *e = getenv("MVF_GV_CHUNK")) chunk = std::max<int64_t>(1, atoll(e));
+  int64_t stage = 0;
+  for (int32_t r = 0, r1; r < nne; r = r1) {
+    for (r1 = r + 1; r1 < nne && rp[r1 + 1] - rp[r] <= chunk; r1++) {
     }
+    stage = std::max(stage, rp[r1] - rp[r]);
+  }
+  int32_t *sia, *sja;
+  double *sa;
+  MVF_TRY(ds.alloc(h, &sia, (size_t)std::max<int64_t>(stage, 1)));
+  MVF_TRY(ds.alloc(h, &sja, (size_t)std::max<int64_t>(stage, 1)));
+  MVF_TRY(ds.alloc(h, &sa, (size_t)std::max<int64_t>(2 * stage, 1)));
+  const double w32 = (double)(float)h->omega;
+  for (int32_t r = 0, r1; r < nne; r = r1) {
+    for (r1 = r + 1; r1 < nne && rp[r1 + 1] - rp[r] <= chunk; r1++) {
+    }
+    const int64_t p0 = rp[r], np = rp[r1] - rp[r];
+    if (np == 0) continue;
+    const int64_t waves = r1 - r;
+    hipLaunchKernelGGL(k_gv_format, dim3((unsigned)std::min<int64_t>((waves + 3) / 4, 65535 * 16)), dim3(256), 0,
+                       h->stream, r, r1, h->d_rowptr, h->d_col, h->d_K, h->d_M, w32, sia, sja, sa);
+    MVF_HIP(h, hipGetLastError());
+    MVF_HIP(h, hipMemcpyAsync(ia + p0, sia, 4 * np, hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipMemcpyAsync(ja + p0, sja, 4 * np, hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipMemcpyAsync(a + 2 * p0, sa, 16 * np, hipMemcpyDeviceToHost, h->stream));
+  }
+  MVF_HIP(h, hipStreamSynchronize(h->stream));
   for (int32_t r = 0; r < nne; r++)
     for (int d = 0; d < 2; d++) {
       b[2 * ((size_t)d * nne + r)] = bb[4 * (size_t)r + 2 * d];

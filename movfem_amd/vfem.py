@@ -175,12 +175,19 @@ class GpuVfem:
         A = A + np.triu(A, 1).T
         return A
 
-    def global_vfem(self):
-        """(ia, ja, a, b) exactly as the reference's global_vfem leaves them."""
-        ia = np.empty(self.nnze, np.int32)
-        ja = np.empty(self.nnze, np.int32)
-        a = np.empty(self.nnze, np.complex128)
-        b = np.empty(2 * self.nne, np.complex128)
+    def global_vfem(self, out=None):
+        """(ia, ja, a, b) exactly as the reference's global_vfem leaves them.
+        out: the four arrays of an earlier call to fill again (as a caller that
+        allocates them once before its frequency loop does)."""
+        if out is None:
+            out = (np.empty(self.nnze, np.int32), np.empty(self.nnze, np.int32), np.empty(self.nnze, np.complex128),
+                   np.empty(2 * self.nne, np.complex128))
+        ia, ja, a, b = out
+        for v, dt, n in ((ia, np.int32, self.nnze), (ja, np.int32, self.nnze), (a, np.complex128, self.nnze),
+                         (b, np.complex128, 2 * self.nne)):
+            if v.dtype != dt or v.size != n or not v.flags.c_contiguous:
+                raise ValueError("global_vfem: out arrays must be contiguous int32[nnze] x2, complex128[nnze], "
+                                 "complex128[2*nne]")
         self._check(self._lib.mvf_global_vfem(self.h, _ip(ia), _ip(ja), _dp(a.view(np.float64)),
                                                _dp(b.view(np.float64)), self.nnze, self.nne, 2))
         return ia, ja, a, b

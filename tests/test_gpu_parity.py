@@ -95,6 +95,45 @@ def test_global_vfem(name, need_gpu):
         assert np.array_equal(zi, oi) and np.array_equal(zj, oj)
 
 
+@pytest.mark.parametrize("name", ["hex20_gpml_tensor_topo", "hex8_dirichlet_topo_mu"])
+def test_global_vfem_chunks(name, monkeypatch, need_gpu):
+    """global_vfem writes the coordinate form on the device in row chunks
+    copied straight into the caller's arrays.  Chunks of 1 entry (one row
+    each, since a row is never split), 97 entries and the default give the
+    same bits, and a equals fl32(K) + i fl32(fl32(omega) M) of the device's
+    operators on and above the diagonal and 0 below, with ia/ja the 1-based
+    row and column of every pattern entry."""
+    m = CASES[name]
+    print(f"case {name}", flush=True)
+    g = GpuVfem(m)
+    g.set_frequency(0)
+    w32 = np.float64(np.float32(m.omega(0)))
+
+    def expected():
+        rp, col, K, Mv = g.operators()
+        rows = np.repeat(np.arange(g.nne), np.diff(rp))
+        up = col >= rows
+        re = np.where(up, K.astype(np.float32).astype(np.float64), 0.0)
+        im = np.where(up, (w32 * Mv).astype(np.float32).astype(np.float64), 0.0)
+        return rows + 1, col + 1, re + 1j * im
+
+    got = []
+    for ch in ("1", "97", None):
+        if ch is None:
+            monkeypatch.delenv("MVF_GV_CHUNK", raising=False)
+        else:
+            monkeypatch.setenv("MVF_GV_CHUNK", ch)
+        ia, ja, a, _ = g.global_vfem()
+        # call 0 uses the first-call GPML flags, later calls the carried ones
+        # (the operators then change once); the operators read after each call
+        # are the ones it formatted
+        ia_e, ja_e, a_e = expected()
+        assert np.array_equal(ia, ia_e) and np.array_equal(ja, ja_e)
+        assert np.array_equal(a.view(np.float64), a_e.view(np.float64))
+        got.append(a)
+    assert np.array_equal(got[1].view(np.float64), got[2].view(np.float64))
+
+
 def test_halfspace_block_global_vfem(need_gpu):
     """The 8-node Dirichlet half-space + block workload at full size."""
     m = M.halfspace_block()
