@@ -134,6 +134,17 @@ int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me,
 int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze,
                     int32_t nne, int32_t ndir);
 
+/* global_vfem followed by find_zeros/rem_zeros (MoVFEM_3DMT.f90:82-97,
+ * global_assembly.f90:123-150) in one call: the zero-removed triplet the
+ * reference hands MUMPS (ia, ja, a in mvf_global_vfem's order with every
+ * entry equal to (0, 0) dropped), formatted on the device, so that only the
+ * nonzero entries cross PCIe.  cap = capacity of ia/ja/a in entries (nnz_triu
+ * of mvf_sizes always suffices: every entry below the diagonal is zero);
+ * *nnz_out = entries written; b as mvf_global_vfem.  Counts as one
+ * global_vfem call.  Fewer than the nonzero entries in cap: MVF_E_ARG. */
+int mvf_global_vfem_nz(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t cap, int32_t nne,
+                       int32_t ndir, int64_t *nnz_out);
+
 /* find_zeros + rem_zeros (global_assembly.f90:123-150): in-place compaction of
  * a COO triplet, returns the new length in *n_out. Host only. */
 int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_out);

@@ -29,7 +29,8 @@ ERRORS = {0: "MVF_OK", -1: "MVF_E_ARG", -2: "MVF_E_DEVICE", -3: "MVF_E_SINGULAR"
 
 # every entry point of include/movfem_gpu.h
 EXPORTS = ("mvf_init", "mvf_free", "mvf_last_error", "mvf_sizes", "mvf_get_gne", "mvf_set_frequency",
-           "mvf_integrate", "mvf_global_vfem", "mvf_remove_zeros", "mvf_default_solver_opts", "mvf_solve",
+           "mvf_integrate", "mvf_global_vfem", "mvf_global_vfem_nz", "mvf_remove_zeros", "mvf_default_solver_opts",
+           "mvf_solve",
            "mvf_solve_frequencies", "mvf_get_solution", "mvf_get_rhs", "mvf_setup_times", "mvf_assemble",
            "mvf_direct_plan_stats", "mvf_readout", "mvf_write_solution", "mvf_write_receivers", "mvf_write_surface",
            "mvf_read_inputs", "mvf_inputs_view", "mvf_inputs_free", "mvf_grid3d", "mvf_mesh_sizes",
@@ -126,6 +127,7 @@ def lib():
     L.mvf_set_frequency.argtypes = [vp, C.c_double]
     L.mvf_integrate.argtypes = [vp, C.c_int64, C.c_int64, dp, dp, dp]
     L.mvf_global_vfem.argtypes = [vp, ip, ip, dp, dp, C.c_int64, C.c_int32, C.c_int32]
+    L.mvf_global_vfem_nz.argtypes = [vp, ip, ip, dp, dp, C.c_int64, C.c_int32, C.c_int32, i64p]
     L.mvf_remove_zeros.argtypes = [C.c_int64, ip, ip, dp, i64p]
     L.mvf_default_solver_opts.restype = None
     L.mvf_default_solver_opts.argtypes = [C.POINTER(SolverOpts)]

@@ -64,26 +64,62 @@ __global__ void k_apply_carry(const int32_t *gne0, int me, const int64_t *rowptr
   }
 }
 
-// rows [r0, r1) of the reference's coordinate form, written at p - rowptr[r0]:
-// (ia, ja) 1-based, a = fl32(K) + i fl32(fl32(omega) M) on and above the
-// diagonal and 0 below (ga_sort_sparse leaves the lower slots unassigned).
-// One wave per row, lanes over the row's entries.
+// rows [r0, r1) of the reference's coordinate form: (ia, ja) 1-based,
+// a = fl32(K) + i fl32(fl32(omega) M) on and above the diagonal and 0 below
+// (ga_sort_sparse leaves the lower slots unassigned).  One wave per row, lanes
+// over the row's entries.  DROP = false: every pattern entry, written at
+// p - rowptr[r0].  DROP = true: only the nonzero ones (find_zeros/rem_zeros),
+// in pattern order, row r's first at off[r] - off[r0].
+template <bool DROP>
 __global__ void k_gv_format(int32_t r0, int32_t r1, const int64_t *rowptr, const int32_t *col, const double *K,
-                            const double *M, double w32, int32_t *ia, int32_t *ja, double *a) {
+                            const double *M, double w32, const int64_t *off, int32_t *ia, int32_t *ja, double *a) {
   const int lane = threadIdx.x & 63;
-  const int64_t p0 = rowptr[r0];
+  const int64_t q0 = DROP ? off[r0] : rowptr[r0];
+  const int64_t rstep = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t r = r0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; r < r1; r += rstep) {
+    const int64_t pb = rowptr[r], pe = rowptr[r + 1];
+    int64_t q = DROP ? off[r] - q0 : pb - q0;
+    for (int64_t p1 = pb; p1 < pe; p1 += 64) {  // wave-uniform trip count
+      const int64_t p = p1 + lane;
+      const bool in = p < pe;
+      const int32_t c = in ? col[p] : 0;
+      const bool up = in && c >= r;
+      const double re = up ? (double)(float)K[p] : 0.0, im = up ? (double)(float)(w32 * M[p]) : 0.0;
+      bool keep = in;
+      int64_t qi = q + lane;
+      if (DROP) {
+        keep = in && (re != 0.0 || im != 0.0);
+        const uint64_t mask = __ballot(keep);
+        qi = q + __popcll(mask & ((uint64_t(1) << lane) - 1));
+        q += __popcll(mask);
+      } else {
+        q += 64;
+      }
+      if (keep && (!DROP || qi < off[r + 1] - q0)) {  // the row's counted slots bound its writes
+        ia[qi] = (int32_t)r + 1;
+        ja[qi] = c + 1;
+        a[2 * qi] = re;
+        a[2 * qi + 1] = im;
+      }
+    }
+  }
+}
+
+// cnt[r - r0] = nonzero entries of row r in the reference's coordinate form
+__global__ void k_gv_count(int32_t r0, int32_t r1, const int64_t *rowptr, const int32_t *col, const double *K,
+                           const double *M, double w32, int32_t *cnt) {
+  const int lane = threadIdx.x & 63;
   const int64_t rstep = (int64_t)gridDim.x * (blockDim.x / 64);
   for (int64_t r = r0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; r < r1; r += rstep) {
     const int64_t pe = rowptr[r + 1];
-    for (int64_t p = rowptr[r] + lane; p < pe; p += 64) {
-      const int32_t c = col[p];
-      const int64_t q = p - p0;
-      const bool up = c >= r;
-      ia[q] = (int32_t)r + 1;
-      ja[q] = c + 1;
-      a[2 * q] = up ? (double)(float)K[p] : 0.0;
-      a[2 * q + 1] = up ? (double)(float)(w32 * M[p]) : 0.0;
+    int n = 0;
+    for (int64_t p1 = rowptr[r]; p1 < pe; p1 += 64) {
+      const int64_t p = p1 + lane;
+      bool keep = false;
+      if (p < pe && col[p] >= r) keep = (double)(float)K[p] != 0.0 || (double)(float)(w32 * M[p]) != 0.0;
+      n += __popcll(__ballot(keep));
     }
+    if (lane == 0) cnt[r - r0] = n;
   }
 }
 
@@ -480,13 +516,15 @@ static int build_rhs(mvf_handle *h, double omega, int64_t call, double *d_b, int
   return rc;
 }
 
-int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze, int32_t nne,
-                    int32_t ndir) {
+// global_vfem's (ia, ja, a, b) into the caller's host arrays; with drop, the
+// triplet after find_zeros/rem_zeros, at most cap entries, its length in *n_out.
+// The coordinate form is written on the device in row chunks of at most
+// `chunk` entries (a row is never split) and copied straight into the
+// caller's arrays.  MVF_GV_CHUNK overrides the chunk size (entries).
+static int global_coo(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, bool drop, int64_t cap,
+                      int64_t *n_out) {
   DevScope ds;
-  if (!h || !ia || !ja || !a || !b) return MVF_E_ARG;
-  if (nnze != h->nnz || nne != h->nne || ndir != 2)
-    return set_err(h, MVF_E_ARG, "size mismatch: nnze %lld/%lld nne %d/%d ndir %d", (long long)nnze,
-                   (long long)h->nnz, nne, h->nne, ndir);
+  const int32_t nne = h->nne;
   if (!h->have_freq) return set_err(h, MVF_E_ARG, "mvf_set_frequency must precede mvf_global_vfem");
   if (h->ops_fp32)
     return set_err(h, MVF_E_ARG, "K and M were integrated in FP32: mvf_assemble(h, 0, ...) restores the FP64 matrix");
@@ -495,18 +533,34 @@ int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *
   double *db;
   MVF_TRY(ds.alloc(h, &db, (size_t)nne * 4));
   MVF_TRY(build_rhs(h, h->omega, h->calls, db, 2, 0));
+  const double w32 = (double)(float)h->omega;
+  auto waves_grid = [](int64_t rows) { return dim3((unsigned)std::min<int64_t>((rows + 3) / 4, 65535 * 16)); };
+  // rp: row starts of what is written (the pattern's, or the nonzero counts' prefix sums)
   std::vector<int64_t> rp(nne + 1);
   std::vector<double> bb((size_t)nne * 4);
-  MVF_HIP(h, hipMemcpyAsync(rp.data(), h->d_rowptr, 8 * (nne + 1), hipMemcpyDeviceToHost, h->stream));
+  int64_t *doff = nullptr;
+  if (drop) {
+    int32_t *dcnt;
+    MVF_TRY(ds.alloc(h, &dcnt, (size_t)nne));
+    MVF_TRY(ds.alloc(h, &doff, (size_t)nne + 1));
+    hipLaunchKernelGGL(k_gv_count, waves_grid(nne), dim3(256), 0, h->stream, 0, nne, h->d_rowptr, h->d_col, h->d_K,
+                       h->d_M, w32, dcnt);
+    MVF_HIP(h, hipGetLastError());
+    std::vector<int32_t> cnt(nne);
+    MVF_HIP(h, hipMemcpyAsync(cnt.data(), dcnt, 4 * (size_t)nne, hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipStreamSynchronize(h->stream));
+    rp[0] = 0;
+    for (int32_t r = 0; r < nne; r++) rp[r + 1] = rp[r] + cnt[r];
+    if (rp[nne] > cap)
+      return set_err(h, MVF_E_ARG, "%lld nonzero entries do not fit the arrays' %lld", (long long)rp[nne],
+                     (long long)cap);
+    MVF_HIP(h, hipMemcpyAsync(doff, rp.data(), 8 * ((size_t)nne + 1), hipMemcpyHostToDevice, h->stream));
+  } else {
+    MVF_HIP(h, hipMemcpyAsync(rp.data(), h->d_rowptr, 8 * ((size_t)nne + 1), hipMemcpyDeviceToHost, h->stream));
+  }
   MVF_HIP(h, hipMemcpyAsync(bb.data(), db, 32 * (size_t)nne, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   ds.drop(db);
-  // The coordinate form is written on the device in row chunks of at most
-  // `chunk` entries (a row is never split) and copied straight into the
-  // caller's arrays: ga_sort_sparse leaves the lower-slot value of (col,row)
-  // at the row-major position of (row,col), so upper entries carry the
-  // symmetric value and lower ones the never-assigned zero, all rounded to
-  // complex64.  MVF_GV_CHUNK overrides the chunk size (entries).
   int64_t chunk = int64_t(1) << 24;
   if (const char *e = getenv("MVF_GV_CHUNK")) chunk = std::max<int64_t>(1, atoll(e));
   int64_t stage = 0;
@@ -520,15 +574,17 @@ int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *
   MVF_TRY(ds.alloc(h, &sia, (size_t)std::max<int64_t>(stage, 1)));
   MVF_TRY(ds.alloc(h, &sja, (size_t)std::max<int64_t>(stage, 1)));
   MVF_TRY(ds.alloc(h, &sa, (size_t)std::max<int64_t>(2 * stage, 1)));
-  const double w32 = (double)(float)h->omega;
   for (int32_t r = 0, r1; r < nne; r = r1) {
     for (r1 = r + 1; r1 < nne && rp[r1 + 1] - rp[r] <= chunk; r1++) {
     }
     const int64_t p0 = rp[r], np = rp[r1] - rp[r];
     if (np == 0) continue;
-    const int64_t waves = r1 - r;
-    hipLaunchKernelGGL(k_gv_format, dim3((unsigned)std::min<int64_t>((waves + 3) / 4, 65535 * 16)), dim3(256), 0,
-                       h->stream, r, r1, h->d_rowptr, h->d_col, h->d_K, h->d_M, w32, sia, sja, sa);
+    if (drop)
+      hipLaunchKernelGGL(k_gv_format<true>, waves_grid(r1 - r), dim3(256), 0, h->stream, r, r1, h->d_rowptr, h->d_col,
+                         h->d_K, h->d_M, w32, doff, sia, sja, sa);
+    else
+      hipLaunchKernelGGL(k_gv_format<false>, waves_grid(r1 - r), dim3(256), 0, h->stream, r, r1, h->d_rowptr,
+                         h->d_col, h->d_K, h->d_M, w32, doff, sia, sja, sa);
     MVF_HIP(h, hipGetLastError());
     MVF_HIP(h, hipMemcpyAsync(ia + p0, sia, 4 * np, hipMemcpyDeviceToHost, h->stream));
     MVF_HIP(h, hipMemcpyAsync(ja + p0, sja, 4 * np, hipMemcpyDeviceToHost, h->stream));
@@ -540,8 +596,26 @@ int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *
       b[2 * ((size_t)d * nne + r)] = bb[4 * (size_t)r + 2 * d];
       b[2 * ((size_t)d * nne + r) + 1] = bb[4 * (size_t)r + 2 * d + 1];
     }
+  if (n_out) *n_out = rp[nne];
   h->calls++;
   return 0;
+}
+
+int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze, int32_t nne,
+                    int32_t ndir) {
+  if (!h || !ia || !ja || !a || !b) return MVF_E_ARG;
+  if (nnze != h->nnz || nne != h->nne || ndir != 2)
+    return set_err(h, MVF_E_ARG, "size mismatch: nnze %lld/%lld nne %d/%d ndir %d", (long long)nnze,
+                   (long long)h->nnz, nne, h->nne, ndir);
+  return global_coo(h, ia, ja, a, b, false, nnze, nullptr);
+}
+
+int mvf_global_vfem_nz(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t cap, int32_t nne,
+                       int32_t ndir, int64_t *nnz_out) {
+  if (!h || !ia || !ja || !a || !b || !nnz_out) return MVF_E_ARG;
+  if (cap < 0 || nne != h->nne || ndir != 2)
+    return set_err(h, MVF_E_ARG, "size mismatch: cap %lld nne %d/%d ndir %d", (long long)cap, nne, h->nne, ndir);
+  return global_coo(h, ia, ja, a, b, true, cap, nnz_out);
 }
 
 int mvf_remove_zeros(int64_t n, int32_t *ia, int32_t *ja, double *a, int64_t *n_out) {

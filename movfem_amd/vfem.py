@@ -192,6 +192,19 @@ class GpuVfem:
                                                _dp(b.view(np.float64)), self.nnze, self.nne, 2))
         return ia, ja, a, b
 
+    def global_vfem_nz(self):
+        """global_vfem followed by find_zeros/rem_zeros, formatted on the
+        device: (ia, ja, a, b) with the (0, 0) entries dropped."""
+        cap = self.nnz_triu
+        ia = np.empty(cap, np.int32)
+        ja = np.empty(cap, np.int32)
+        a = np.empty(cap, np.complex128)
+        b = np.empty(2 * self.nne, np.complex128)
+        n = C.c_int64()
+        self._check(self._lib.mvf_global_vfem_nz(self.h, _ip(ia), _ip(ja), _dp(a.view(np.float64)),
+                                                  _dp(b.view(np.float64)), cap, self.nne, 2, C.byref(n)))
+        return ia[:n.value], ja[:n.value], a[:n.value], b
+
     def remove_zeros(self, ia, ja, a):
         ia, ja = ia.copy(), ja.copy()
         a = np.ascontiguousarray(a, np.complex128).copy()

@@ -134,6 +134,48 @@ def test_global_vfem_chunks(name, monkeypatch, need_gpu):
     assert np.array_equal(got[1].view(np.float64), got[2].view(np.float64))
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_global_vfem_nz(name, monkeypatch, need_gpu):
+    """mvf_global_vfem_nz equals mvf_global_vfem followed by mvf_remove_zeros
+    (find_zeros/rem_zeros) bit for bit.  Two handles in lockstep over two
+    frequencies (the second call switches to the carried GPML flags), with
+    chunks of 1 entry (one row each), 97 entries and the default; then arrays
+    one entry too short are refused."""
+    import ctypes as C
+
+    from movfem_amd import _lib as L
+
+    m = CASES[name]
+    print(f"case {name}", flush=True)
+    g1, g2 = GpuVfem(m), GpuVfem(m)
+    for fi, ch in ((0, "1"), (1, "97"), (1, None)):
+        if ch is None:
+            monkeypatch.delenv("MVF_GV_CHUNK", raising=False)
+        else:
+            monkeypatch.setenv("MVF_GV_CHUNK", ch)
+        g1.set_frequency(fi)
+        g2.set_frequency(fi)
+        ia, ja, a, b = g1.global_vfem()
+        zi, zj, za = g1.remove_zeros(ia, ja, a)
+        ni, nj, na, nb = g2.global_vfem_nz()
+        assert len(ni) <= g2.nnz_triu
+        assert np.array_equal(ni, zi) and np.array_equal(nj, zj)
+        assert np.array_equal(na.view(np.float64), za.view(np.float64))
+        assert np.array_equal(nb.view(np.float64), b.view(np.float64))
+    n = len(ni)
+    ia = np.empty(n, np.int32)
+    ja = np.empty(n, np.int32)
+    a = np.empty(n, np.complex128)
+    b = np.empty(2 * g2.nne, np.complex128)
+    out = C.c_int64(-1)
+    rc = L.lib().mvf_global_vfem_nz(g2.h, ia.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    ja.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)),
+                                    b.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), n - 1, g2.nne, 2,
+                                    C.byref(out))
+    assert rc == L.MVF_E_ARG and out.value == -1
+
+
 def test_halfspace_block_global_vfem(need_gpu):
     """The 8-node Dirichlet half-space + block workload at full size."""
     m = M.halfspace_block()
