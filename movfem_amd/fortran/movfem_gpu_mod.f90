@@ -157,6 +157,17 @@ module movfem_gpu
       integer(c_int32_t), value :: nne, ndir
     end function mvf_global_vfem
 
+    integer(c_int) function mvf_global_vfem_nz(h, ia, ja, a, b, cap, nne, ndir, nnz_out) &
+        bind(C, name='mvf_global_vfem_nz')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double_complex
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: ia(*), ja(*)
+      complex(c_double_complex), intent(out) :: a(*), b(*)
+      integer(c_int64_t), value :: cap
+      integer(c_int32_t), value :: nne, ndir
+      integer(c_int64_t), intent(out) :: nnz_out
+    end function mvf_global_vfem_nz
+
     integer(c_int) function mvf_remove_zeros(n, ia, ja, a, n_out) bind(C, name='mvf_remove_zeros')
       import :: c_int, c_int32_t, c_int64_t, c_double_complex
       integer(c_int64_t), value :: n
