@@ -177,6 +177,8 @@ void mvf_free(mvf_handle *h) {
                   h->d_x,  h->d_b, h->d_ntab, h->d_bel, h->d_bidx, h->d_bK, h->d_bM, h->d_dirf, h->d_btab};
   for (void *p : ptrs) dfree(p);
   for (void *p : h->owned) dfree(p);
+  for (void *p : h->gv_pin)
+    if (p) (void)hipHostFree(p);
   direct_free(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -516,11 +518,45 @@ static int build_rhs(mvf_handle *h, double omega, int64_t call, double *d_b, int
   return rc;
 }
 
+// dst[i] = src[i] for the three arrays of a staged chunk, split over host
+// threads when the chunk is large (the caller's arrays are pageable memory,
+// written at the host's copy rate; one thread does not reach the DMA rate)
+static void par_copy(void *const dst[3], const void *const src[3], const size_t bytes[3]) {
+  const size_t total = bytes[0] + bytes[1] + bytes[2];
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = total < (size_t(8) << 20) ? 1 : (int)std::min<unsigned>(8, hw ? hw : 1);
+  auto part = [&](int t) {
+    for (int k = 0; k < 3; k++) {
+      const size_t b0 = bytes[k] * t / nt, b1 = bytes[k] * (t + 1) / nt;
+      if (b1 > b0) memcpy((char *)dst[k] + b0, (const char *)src[k] + b0, b1 - b0);
+    }
+  };
+  if (nt == 1) {
+    part(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (int t = 1; t < nt; t++) th.emplace_back(part, t);
+  part(0);
+  for (auto &x : th) x.join();
+}
+
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  ~EventPair() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
 // global_vfem's (ia, ja, a, b) into the caller's host arrays; with drop, the
 // triplet after find_zeros/rem_zeros, at most cap entries, its length in *n_out.
 // The coordinate form is written on the device in row chunks of at most
-// `chunk` entries (a row is never split) and copied straight into the
-// caller's arrays.  MVF_GV_CHUNK overrides the chunk size (entries).
+// `chunk` entries (a row is never split; MVF_GV_CHUNK overrides the size),
+// copied by DMA into one of two pinned host buffers, and from there into the
+// caller's arrays by host threads while the next chunk is formatted and
+// copied into the other buffer.
 static int global_coo(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, bool drop, int64_t cap,
                       int64_t *n_out) {
   DevScope ds;
@@ -561,7 +597,7 @@ static int global_coo(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double
   MVF_HIP(h, hipMemcpyAsync(bb.data(), db, 32 * (size_t)nne, hipMemcpyDeviceToHost, h->stream));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   ds.drop(db);
-  int64_t chunk = int64_t(1) << 24;
+  int64_t chunk = int64_t(1) << 22;
   if (const char *e = getenv("MVF_GV_CHUNK")) chunk = std::max<int64_t>(1, atoll(e));
   int64_t stage = 0;
   for (int32_t r = 0, r1; r < nne; r = r1) {
@@ -574,6 +610,41 @@ static int global_coo(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double
   MVF_TRY(ds.alloc(h, &sia, (size_t)std::max<int64_t>(stage, 1)));
   MVF_TRY(ds.alloc(h, &sja, (size_t)std::max<int64_t>(stage, 1)));
   MVF_TRY(ds.alloc(h, &sa, (size_t)std::max<int64_t>(2 * stage, 1)));
+  if (h->gv_pin_cap < stage) {
+    for (void *&p : h->gv_pin)
+      if (p) {
+        (void)hipHostFree(p);
+        p = nullptr;
+      }
+    h->gv_pin_cap = 0;
+    for (void *&p : h->gv_pin)
+      if (hipHostMalloc(&p, 24 * (size_t)stage, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        return set_err(h, MVF_E_MEMORY, "pinned host staging of %lld bytes failed", (long long)(48 * stage));
+      }
+    h->gv_pin_cap = stage;
+  }
+  // pinned buffer layout: ia (4 B per entry), ja (4 B), a (16 B)
+  auto pin = [&](int k, int part) {
+    char *b = (char *)h->gv_pin[k];
+    return part == 0 ? (void *)b : part == 1 ? (void *)(b + 4 * h->gv_pin_cap) : (void *)(b + 8 * h->gv_pin_cap);
+  };
+  EventPair ev;
+  for (hipEvent_t &e : ev.e) MVF_HIP(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  struct Staged {
+    int64_t p0 = -1, np = 0;
+    int k = 0;
+  } prev;
+  // waits for a staged chunk's DMA, then copies it into the caller's arrays
+  auto drain = [&](const Staged &c) -> int {
+    MVF_HIP(h, hipEventSynchronize(ev.e[c.k]));
+    void *dst[3] = {ia + c.p0, ja + c.p0, a + 2 * c.p0};
+    const void *src[3] = {pin(c.k, 0), pin(c.k, 1), pin(c.k, 2)};
+    const size_t bytes[3] = {4 * (size_t)c.np, 4 * (size_t)c.np, 16 * (size_t)c.np};
+    par_copy(dst, src, bytes);
+    return 0;
+  };
+  int k = 0;
   for (int32_t r = 0, r1; r < nne; r = r1) {
     for (r1 = r + 1; r1 < nne && rp[r1 + 1] - rp[r] <= chunk; r1++) {
     }
@@ -586,10 +657,16 @@ static int global_coo(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double
       hipLaunchKernelGGL(k_gv_format<false>, waves_grid(r1 - r), dim3(256), 0, h->stream, r, r1, h->d_rowptr,
                          h->d_col, h->d_K, h->d_M, w32, doff, sia, sja, sa);
     MVF_HIP(h, hipGetLastError());
-    MVF_HIP(h, hipMemcpyAsync(ia + p0, sia, 4 * np, hipMemcpyDeviceToHost, h->stream));
-    MVF_HIP(h, hipMemcpyAsync(ja + p0, sja, 4 * np, hipMemcpyDeviceToHost, h->stream));
-    MVF_HIP(h, hipMemcpyAsync(a + 2 * p0, sa, 16 * np, hipMemcpyDeviceToHost, h->stream));
+    // buffer k was drained (synchronously, below) before this chunk is queued into it
+    MVF_HIP(h, hipMemcpyAsync(pin(k, 0), sia, 4 * np, hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipMemcpyAsync(pin(k, 1), sja, 4 * np, hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipMemcpyAsync(pin(k, 2), sa, 16 * np, hipMemcpyDeviceToHost, h->stream));
+    MVF_HIP(h, hipEventRecord(ev.e[k], h->stream));
+    if (prev.p0 >= 0) MVF_TRY(drain(prev));
+    prev = {p0, np, k};
+    k ^= 1;
   }
+  if (prev.p0 >= 0) MVF_TRY(drain(prev));
   MVF_HIP(h, hipStreamSynchronize(h->stream));
   for (int32_t r = 0; r < nne; r++)
     for (int d = 0; d < 2; d++) {

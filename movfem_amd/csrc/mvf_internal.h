@@ -143,6 +143,9 @@ struct mvf_handle {
   int carry_applied = 0;  // element 0 currently holds the carried-flag contribution
   int flags_first = 0, flags_late = 0;
   int64_t calls = 0;  // global_vfem-equivalent calls performed (frequencies processed)
+  // pinned host staging of mvf_global_vfem's chunks (two buffers of gv_pin_cap entries, 24 B each)
+  void *gv_pin[2] = {nullptr, nullptr};
+  int64_t gv_pin_cap = 0;
   // current frequency
   double freq = 0.0, omega = 0.0;
   int have_freq = 0;
