@@ -535,9 +535,19 @@ static void par_copy(void *const dst[3], const void *const src[3], const size_t 
     part(0);
     return;
   }
+  // a thread that cannot be started leaves its part (and the later ones) to
+  // this thread: nothing is thrown across the C interface
   std::vector<std::thread> th;
-  th.reserve(nt - 1);
-  for (int t = 1; t < nt; t++) th.emplace_back(part, t);
+  int started = 1;
+  try {
+    th.reserve(nt - 1);
+    for (int t = 1; t < nt; t++) {
+      th.emplace_back(part, t);
+      started = t + 1;
+    }
+  } catch (...) {
+  }
+  for (int t = started; t < nt; t++) part(t);
   part(0);
   for (auto &x : th) x.join();
 }
