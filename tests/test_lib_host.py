@@ -53,6 +53,13 @@ def test_bad_arguments_return_codes():
     assert rc == -1
     assert b"inimod 3" in lib.mvf_last_error(h)
     lib.mvf_free(h)
+    # no handle: the global_vfem forms refuse before touching a device
+    n = C.c_int64(-1)
+    i32 = np.zeros(4, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+    f64 = np.zeros(8).ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.mvf_global_vfem(None, i32, i32, f64, f64, 4, 2, 2) == -1
+    assert lib.mvf_global_vfem_nz(None, i32, i32, f64, f64, 4, 2, 2, C.byref(n)) == -1
+    assert n.value == -1
 
 
 def test_remove_zeros_host():
