@@ -130,7 +130,9 @@ int mvf_integrate(mvf_handle *h, int64_t e0, int64_t e1, double *Ke, double *Me,
  * Fills exactly what the reference's global_vfem leaves after ga_sort_sparse
  * and before zero removal: ia/ja 1-based row-major over the full pattern,
  * a the complex64-rounded values in the reference's column-sorted order, b the
- * 2*nne RHS.  nnze/nne/ndir must match mvf_sizes; ndir must be 2. */
+ * 2*nne RHS.  nnze/nne/ndir must match mvf_sizes; ndir must be 2.  The
+ * handle keeps two pinned host staging buffers (up to 2^22 entries, 96 MiB,
+ * each) from the first call of either global_vfem form until mvf_free. */
 int mvf_global_vfem(mvf_handle *h, int32_t *ia, int32_t *ja, double *a, double *b, int64_t nnze,
                     int32_t nne, int32_t ndir);
 
